@@ -1,0 +1,151 @@
+"""Benchmark: decompressed GB/s of one gzip member on one MI355X.
+
+Workload (default): 1 GiB from the text generator (order-1 Markov chain over
+96 printable bytes, Zipf(1.2) rows, 35 % copies of U[3,258] bytes from
+log-uniform distances <= 32 KiB), compressed at level 6 into one RFC 1952
+member (libdeflate stand-in encoder), decoded by speculative parallel decode
+on a 512 KiB compressed-chunk grid.  Input and output stay resident in HBM;
+a step is one full decode (find, inflate, resolve, CRC32 + ISIZE verify).
+
+Prints ONE JSON line (see the keys below).  `roofline` uses the dominant
+kernel (k_inflate): algorithmic bytes C + D per step over its summed
+HIP-event duration per step.  `cpu_baseline` is the oracle's serial port (one
+core) on the same member.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "decompressed GB/s on 1 MI355X (1 GiB member, 10k-member batch); p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(gz, raw_size, budget_s=12.0, max_runs=3):
+    """Oracle serial decode (one core) of the same member; median GB/s."""
+    import numpy as np
+    from oracle import gzoracle
+    out = np.empty(raw_size + 65536, dtype=np.uint8)
+    times = []
+    t_start = time.time()
+    while len(times) < max_runs and (not times or time.time() - t_start < budget_s):
+        t0 = time.perf_counter()
+        import ctypes
+        n = ctypes.c_size_t()
+        m = ctypes.c_size_t()
+        st = gzoracle.lib().gzo_gunzip(gz.ctypes.data, gz.size, out.ctypes.data, out.size, ctypes.byref(n),
+                                       ctypes.byref(m))
+        times.append(time.perf_counter() - t0)
+        assert st == 0 and n.value == raw_size
+    t = sorted(times)[len(times) // 2]
+    return {"value": round(raw_size / t / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"the full {raw_size} B member decoded serially by oracle/gzoracle.c "
+                      f"(restatement of drive_thin_t1_oracle), median of {len(times)} runs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size-mib", type=int, default=1024, help="decompressed member size")
+    ap.add_argument("--level", type=int, default=6)
+    ap.add_argument("--chunk-kib", type=int, default=512)
+    ap.add_argument("--pieces-per-chunk", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=0x675A0001)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import gzippy_amd
+    from gzippy_amd import workloads as W
+
+    n = args.size_mib << 20
+    t0 = time.time()
+    raw = W.generate("text", n, args.seed)
+    gz = W.gzip_compress(raw, args.level)
+    log(f"workload: {n} B -> {gz.size} B ({W.encoder_name()} L{args.level}) in {time.time() - t0:.1f}s")
+
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(gz).to(dev)
+    d_raw = torch.from_numpy(raw).to(dev)
+    d_out = torch.empty(n, dtype=torch.uint8, device=dev)
+    ctx = gzippy_amd.context(0)
+    ctx.set_tuning(chunk_bytes=args.chunk_kib << 10, pieces_per_chunk=args.pieces_per_chunk)
+    torch.cuda.synchronize()
+
+    stats = None
+    for i in range(args.warmup):
+        d_out.zero_()
+        got, crc, stats = gzippy_amd.decompress_device(d_in, d_out, ctx=ctx)
+        torch.cuda.synchronize()
+        if not (got == n and torch.equal(d_out, d_raw)):
+            log("FAILED: output differs from the generator bytes")
+            sys.exit(1)
+        log(f"warmup {i}: ok, {stats['ms_total']:.2f} ms device, pieces {stats['pieces']}, "
+            f"redecoded {stats['redecoded']}, inflate launches {stats['inflate_launches']}")
+    del d_raw
+
+    ms_infl, launches, ms_dev = 0.0, 0, 0.0
+    per_step = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        got, crc, st = gzippy_amd.decompress_device(d_in, d_out, ctx=ctx)
+        per_step.append(time.perf_counter() - ts)
+        ms_infl += st["ms_inflate_kernel"]
+        launches += st["inflate_launches"]
+        ms_dev += st["ms_total"]
+        stats = st
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ms_per_step = t / args.steps * 1e3
+    gbs = n * args.steps / t / 1e9
+    C, D = gz.size, n
+    infl_ms_step = ms_infl / args.steps
+    achieved = (C + D) / (infl_ms_step / 1e3) / 1e9
+    lat = sorted(per_step)
+    p50 = lat[len(lat) // 2] * 1e3
+    p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))] * 1e3
+
+    line = {
+        "metric": METRIC, "value": round(gbs, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {
+            "workload": f"single gzip member: {args.size_mib} MiB text generator (order-1 Markov, Zipf 1.2, "
+                        f"35% copies), {W.encoder_name()} level {args.level}, speculative parallel decode "
+                        f"on {args.chunk_kib} KiB compressed chunks",
+            "compressed_bytes": int(C), "decompressed_bytes": int(D), "seed": args.seed,
+            "pieces": stats["pieces"], "redecoded": stats["redecoded"],
+            "inflate_launches_per_step": launches / args.steps,
+            "phase_ms": {k: round(stats[k], 3) for k in ("ms_find", "ms_inflate", "ms_resolve", "ms_crc", "ms_total")},
+            "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3)},
+        },
+        "roofline": {"bound": "hbm", "kernel": "k_inflate", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes_per_step": int(C + D), "kernel_ms_per_step": round(infl_ms_step, 3)},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(gz, n)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

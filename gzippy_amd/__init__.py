@@ -1,0 +1,224 @@
+"""gzippy_amd — MI355X (gfx950) parallel gzip decompression.
+
+Host-side mirror of gzippy's public decompression API (src/lib.rs:166-227),
+backed by the HIP library ``libgzgpu.so`` through its C ABI:
+
+    decompress(data)                              lib.rs:176
+    decompress_with_threads(data, threads)        lib.rs:189
+    decompress_to_writer(data, writer)            lib.rs:204
+    decompress_to_writer_with_threads(...)        lib.rs:221
+
+Semantics kept from the reference: non-gzip input returns empty output
+(lib.rs:50-52); multi-member input concatenates; bytes after a valid member
+that do not start a member end the walk cleanly; corruption raises
+``GzippyError`` (the ``GzippyError::Decompression`` arm, error.rs:17) — there
+is no fallback to another decoder.  ``threads`` is accepted for signature
+parity; the GPU picks its own parallelism.
+
+Device-resident entry points (inputs/outputs as torch CUDA tensors):
+``decompress_device``, ``decompress_batch``, ``find_blocks``, ``crc32``,
+``inflate_piece``.
+"""
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _native
+from ._native import GzgpuError, Stats, Tuning  # noqa: F401
+
+__all__ = [
+    "GzippyError", "decompress", "decompress_with_threads", "decompress_to_writer",
+    "decompress_to_writer_with_threads", "decompress_device", "decompress_batch", "find_blocks", "crc32",
+    "inflate_piece", "context",
+]
+
+
+class GzippyError(Exception):
+    """Mirror of gzippy's GzippyError (src/error.rs:7-46).  ``kind`` is the
+    ParallelError / device status name (single_member.rs:1384-1413)."""
+
+    def __init__(self, message, kind="Decompression", status=None):
+        super().__init__(f"Decompression error: {message}")
+        self.kind = kind
+        self.status = status
+
+
+_tls = threading.local()
+
+
+def context(device=0):
+    """The calling thread's device context (created on first use)."""
+    ctxs = getattr(_tls, "ctxs", None)
+    if ctxs is None:
+        ctxs = _tls.ctxs = {}
+    c = ctxs.get(device)
+    if c is None:
+        c = ctxs[device] = _native.Context(device)
+    return c
+
+
+def _raise(ctx, st):
+    e = ctx.error(st)
+    raise GzippyError(str(e), kind=e.kind, status=st)
+
+
+def _as_bytes_view(data):
+    if isinstance(data, np.ndarray):
+        return np.ascontiguousarray(data, dtype=np.uint8)
+    return np.frombuffer(memoryview(data), dtype=np.uint8)
+
+
+def decompress(data, stats=None):
+    """Decompress a gzip stream; returns bytes (lib.rs:176)."""
+    return decompress_with_threads(data, 0, stats=stats)
+
+
+def decompress_with_threads(data, threads, stats=None):
+    """lib.rs:189.  `threads` is accepted for API parity; unused."""
+    ctx = context()
+    a = _as_bytes_view(data)
+    n = ctypes.c_size_t()
+    st_obj = Stats()
+    cap = max(1 << 20, 4 * a.size)
+    if a.size >= 8 and a.size >= 2 and a[0] == 0x1F and a[1] == 0x8B:
+        cap = max(cap, int(a[-4:].view(np.uint32)[0]) + 1)
+    for _ in range(16):
+        out = np.empty(cap, dtype=np.uint8)
+        st = ctx.lib.gzgpu_decompress(ctx.h, a.ctypes.data, a.size, out.ctypes.data, cap, ctypes.byref(n),
+                                      ctypes.byref(st_obj))
+        if st == _native.OUTPUT_TOO_SMALL and n.value > cap:
+            cap = n.value
+            continue
+        if st != _native.OK:
+            _raise(ctx, st)
+        if stats is not None:
+            stats.update(st_obj.as_dict())
+        return out[: n.value].tobytes()
+    _raise(ctx, _native.OUTPUT_TOO_SMALL)
+
+
+def decompress_to_writer(data, writer, stats=None):
+    """Stream the decompressed bytes into `writer` (anything with .write);
+    returns the byte count (lib.rs:204)."""
+    return decompress_to_writer_with_threads(data, writer, 0, stats=stats)
+
+
+def decompress_to_writer_with_threads(data, writer, threads, stats=None):
+    ctx = context()
+    a = _as_bytes_view(data)
+    failure = []
+
+    @_native.WRITE_FN
+    def _cb(user, ptr, n):
+        try:
+            writer.write(ctypes.string_at(ptr, n))
+            return 0
+        except Exception as e:  # propagate as GZGPU_IO (ParallelError::Io)
+            failure.append(e)
+            return 1
+
+    total = ctypes.c_uint64()
+    st_obj = Stats()
+    st = ctx.lib.gzgpu_decompress_to_writer(ctx.h, a.ctypes.data, a.size, _cb, None, ctypes.byref(total),
+                                            ctypes.byref(st_obj))
+    if st != _native.OK:
+        if failure:
+            raise GzippyError(f"I/O error: {failure[0]}", kind="Io", status=st) from failure[0]
+        _raise(ctx, st)
+    if stats is not None:
+        stats.update(st_obj.as_dict())
+    return total.value
+
+
+# ---------------------------------------------------------------- device-resident
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def decompress_device(d_in, d_out, in_len=None, ctx=None):
+    """Decode one gzip member from device tensor `d_in` into `d_out`
+    (uint8 CUDA tensors).  Returns (out_len, crc32, stats dict)."""
+    ctx = ctx or context(d_in.device.index or 0)
+    n = ctypes.c_size_t()
+    crc = ctypes.c_uint32()
+    st_obj = Stats()
+    L = d_in.numel() if in_len is None else in_len
+    st = ctx.lib.gzgpu_decompress_device(ctx.h, _ptr(d_in), L, _ptr(d_out), d_out.numel(), ctypes.byref(n),
+                                         ctypes.byref(crc), ctypes.byref(st_obj))
+    if st != _native.OK:
+        _raise(ctx, st)
+    return n.value, crc.value, st_obj.as_dict()
+
+
+def decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None, raise_on_error=True):
+    """Decode n independent members in one persistent launch.
+    Offsets/lengths are host integer sequences.  Returns (status[], out_len[],
+    crc[], stats dict) as numpy arrays."""
+    ctx = ctx or context(d_in.device.index or 0)
+    n = len(in_off)
+    a = [np.ascontiguousarray(np.asarray(x, dtype=np.uint64)) for x in (in_off, in_len, out_off, out_cap)]
+    status = np.zeros(n, dtype=np.int32)
+    olen = np.zeros(n, dtype=np.uint64)
+    crcs = np.zeros(n, dtype=np.uint32)
+    st_obj = Stats()
+    P64 = ctypes.POINTER(ctypes.c_uint64)
+    st = ctx.lib.gzgpu_decompress_batch(
+        ctx.h, _ptr(d_in), a[0].ctypes.data_as(P64), a[1].ctypes.data_as(P64), _ptr(d_out),
+        a[2].ctypes.data_as(P64), a[3].ctypes.data_as(P64), n,
+        status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), olen.ctypes.data_as(P64),
+        crcs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(st_obj))
+    if st != _native.OK and raise_on_error:
+        _raise(ctx, st)
+    return status, olen, crcs, st_obj.as_dict()
+
+
+def find_blocks(d_in, start_bit=0, end_bit=None, in_len=None, ctx=None):
+    """Block-finder hits in [start_bit, end_bit) of a device buffer, sorted:
+    list of (bit_offset, seek_bit, kind)."""
+    ctx = ctx or context(d_in.device.index or 0)
+    L = d_in.numel() if in_len is None else in_len
+    if end_bit is None:
+        end_bit = L * 8
+    cnt = ctypes.c_size_t()
+    cap = 1 << 16
+    while True:
+        buf = (_native.Boundary * cap)()
+        st = ctx.lib.gzgpu_find_blocks(ctx.h, _ptr(d_in), L, start_bit, end_bit, buf, cap, ctypes.byref(cnt))
+        if st != _native.OK:
+            _raise(ctx, st)
+        if cnt.value <= cap:
+            return [(buf[i].bit_offset, buf[i].seek_bit, buf[i].kind) for i in range(cnt.value)]
+        cap = cnt.value
+
+
+def crc32(d_data, length=None, ctx=None):
+    """gzip CRC32 of a device buffer."""
+    ctx = ctx or context(d_data.device.index or 0)
+    c = ctypes.c_uint32()
+    L = d_data.numel() * d_data.element_size() if length is None else length
+    st = ctx.lib.gzgpu_crc32(ctx.h, _ptr(d_data), L, ctypes.byref(c))
+    if st != _native.OK:
+        _raise(ctx, st)
+    return c.value
+
+
+def inflate_piece(d_in, start_bit, stop_bit, d_out, window=None, in_len=None, ctx=None):
+    """One wavefront decodes raw DEFLATE from start_bit with the stop rule.
+    window=None: marker mode, d_out is int16/uint16-sized entries; window =
+    32768 host bytes: clean decode into uint8 d_out.  Returns PieceResult."""
+    ctx = ctx or context(d_in.device.index or 0)
+    res = _native.PieceResult()
+    L = d_in.numel() if in_len is None else in_len
+    w = None
+    if window is not None:
+        w = np.ascontiguousarray(np.frombuffer(bytes(window), dtype=np.uint8))
+        assert w.size == 32768
+    st = ctx.lib.gzgpu_inflate_piece(ctx.h, _ptr(d_in), L, start_bit, stop_bit,
+                                     None if w is None else w.ctypes.data, _ptr(d_out), d_out.numel(),
+                                     ctypes.byref(res))
+    if st != _native.OK:
+        _raise(ctx, st)
+    return res

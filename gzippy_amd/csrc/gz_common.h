@@ -1,0 +1,165 @@
+// gz_common.h — shared device definitions for the gfx950 DEFLATE kernels.
+//
+// Layout conventions
+//   * The compressed input is one byte buffer in HBM; every bit position is
+//     absolute (bit b = bit (b & 7) of byte b >> 3, LSB first, RFC 1951 3.1.1).
+//     Reads past the buffer end return zero bits (bit_reader.rs:46-133 reads
+//     zero-padded bits the same way).
+//   * A workgroup of the inflate kernel is ONE wavefront (64 lanes).  The
+//     decoder state (bit buffer, positions, counters) is wave-uniform; LDS
+//     lookups use a uniform address (broadcast, no bank conflicts) and are
+//     pulled back to SGPRs with readfirstlane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gz {
+
+constexpr int WAVE = 64;
+constexpr uint32_t WIN = 32768;            // DEFLATE window (RFC 1951 3.2)
+constexpr uint32_t MARKER_BASE = 32768;     // replace_markers.rs:23
+constexpr int LL_ROOT = 10;                 // litlen LUT root bits
+constexpr int D_ROOT = 8;                   // distance LUT root bits
+constexpr int PRE_ROOT = 7;                 // precode LUT covers every precode length
+
+// Piece status (device → host)
+enum : uint32_t {
+    PS_NONE = 0,
+    PS_STOPPED = 1,   // reached a non-final block header at or after stop_bit
+    PS_FINAL = 2,     // decoded the final (BFINAL) block
+    PS_OVERFLOW = 3,  // slab / output capacity exhausted
+    PS_ERROR = 4,     // invalid stream from this start
+};
+// error sub-codes (PieceOut::err)
+enum : uint32_t {
+    E_NONE = 0, E_BTYPE3, E_TRUNC, E_STORED_LEN, E_HDR_COUNTS, E_PRECODE, E_CODELEN, E_EOB_ZERO,
+    E_LL_TABLE, E_D_TABLE, E_BAD_SYM, E_BAD_DIST_SYM, E_DIST_FAR,
+};
+
+// LUT entry (u32): [0,4) code length (0 = resolve on the slow path),
+// [4,8) extra bits, [8,10) kind, [16,32) value.
+enum : uint32_t { K_LIT = 0, K_LEN = 1, K_EOB = 2, K_BAD = 3 };
+
+__constant__ uint16_t c_lbase[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
+                                     31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
+__constant__ uint8_t c_lextra[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2,
+                                     2, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
+__constant__ uint16_t c_dbase[30] = {1,    2,    3,    4,    5,    7,     9,     13,    17,  25,
+                                     33,   49,   65,   97,   129,  193,   257,   385,   513, 769,
+                                     1025, 1537, 2049, 3073, 4097, 6145, 8193, 12289, 16385, 24577};
+__constant__ uint8_t c_dextra[30] = {0, 0, 0, 0, 1, 1, 2, 2,  3,  3,  4,  4,  5,  5,  6,
+                                     6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
+// precode symbol order (RFC 1951 3.2.7)
+__constant__ uint8_t c_pre_order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (uint64_t)hi << 32 | lo;
+}
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// number of active lanes below this one in mask
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ void compiler_fence() { __asm__ __volatile__("" ::: "memory"); }
+
+// 32-bit little-endian word `wi` of the input; bytes past n read as zero.
+__device__ __forceinline__ uint32_t load_word(const uint8_t *__restrict__ in, uint64_t n, uint64_t wi) {
+    uint64_t b = wi * 4;
+    if (b + 4 <= n) return reinterpret_cast<const uint32_t *>(in)[wi];
+    uint32_t v = 0;
+    for (int i = 0; i < 4; i++)
+        if (b + i < n) v |= (uint32_t)in[b + i] << (8 * i);
+    return v;
+}
+
+// 64 bits starting at absolute bit `bit` (zero past the end).
+__device__ __forceinline__ uint64_t peek64(const uint8_t *__restrict__ in, uint64_t n, uint64_t bit) {
+    uint64_t wi = bit >> 5;
+    uint32_t sh = (uint32_t)(bit & 31);
+    uint64_t lo = (uint64_t)load_word(in, n, wi) | (uint64_t)load_word(in, n, wi + 1) << 32;
+    uint64_t hi = load_word(in, n, wi + 2);
+    uint64_t v = lo >> sh;
+    if (sh) v |= hi << (64 - sh);
+    return v;
+}
+
+// ---- wave-uniform bit reader -------------------------------------------
+// Holds 33..64 valid bits after refill(); position = next_word*32 - cnt.
+struct BitReader {
+    const uint8_t *in;
+    uint64_t n;
+    uint64_t buf;
+    uint64_t next_word;
+    uint32_t cnt;
+
+    __device__ __forceinline__ void init(const uint8_t *in_, uint64_t n_, uint64_t bit) {
+        in = in_;
+        n = n_;
+        uint64_t wi = bit >> 5;
+        uint32_t sh = (uint32_t)(bit & 31);
+        uint64_t lo = uni(load_word(in, n, wi)), hi = uni(load_word(in, n, wi + 1));
+        buf = (lo | hi << 32) >> sh;
+        cnt = 64 - sh;
+        next_word = wi + 2;
+    }
+    __device__ __forceinline__ void refill() {
+        if (cnt <= 32) {
+            buf |= (uint64_t)uni(load_word(in, n, next_word)) << cnt;
+            next_word++;
+            cnt += 32;
+        }
+    }
+    __device__ __forceinline__ uint32_t peek(uint32_t k) const { return (uint32_t)(buf & ((1ull << k) - 1)); }
+    __device__ __forceinline__ void consume(uint32_t k) {
+        buf >>= k;
+        cnt -= k;
+    }
+    __device__ __forceinline__ uint32_t read(uint32_t k) {
+        uint32_t v = peek(k);
+        consume(k);
+        return v;
+    }
+    __device__ __forceinline__ uint64_t pos() const { return next_word * 32 - cnt; }
+};
+
+__device__ __forceinline__ uint32_t make_ll_entry(uint32_t sym, uint32_t len) {
+    if (sym < 256) return len | (K_LIT << 8) | (sym << 16);
+    if (sym == 256) return len | (K_EOB << 8);
+    if (sym < 286) {
+        uint32_t i = sym - 257;
+        return len | ((uint32_t)c_lextra[i] << 4) | (K_LEN << 8) | ((uint32_t)c_lbase[i] << 16);
+    }
+    return len | (K_BAD << 8);
+}
+__device__ __forceinline__ uint32_t make_d_entry(uint32_t sym, uint32_t len) {
+    if (sym < 30) return len | ((uint32_t)c_dextra[sym] << 4) | (K_LEN << 8) | ((uint32_t)c_dbase[sym] << 16);
+    return len | (K_BAD << 8);
+}
+
+// Canonical decode (RFC 1951 3.2.2) of one codeword from `bits`, MSB of the
+// code first; `count` = codes per length.  Used for codes longer than the
+// LUT root.  Returns the symbol and its length, or -1.
+__device__ __forceinline__ int slow_decode(uint32_t bits, const uint16_t *count, const uint16_t *sorted,
+                                           uint32_t &len) {
+    int code = 0, first = 0, index = 0;
+#pragma unroll
+    for (int l = 1; l <= 15; l++) {
+        code |= (bits >> (l - 1)) & 1;
+        int c = count[l];
+        if (code - c < first) {
+            len = (uint32_t)l;
+            return sorted[index + (code - first)];
+        }
+        index += c;
+        first += c;
+        first <<= 1;
+        code <<= 1;
+    }
+    return -1;
+}
+
+}  // namespace gz

@@ -1,0 +1,1135 @@
+// gzgpu.hip — host runtime and C ABI of the gfx950 parallel gzip decoder.
+//
+// Pipeline for a set of gzip members resident in HBM (one member = the
+// single-member path, src/decompress/parallel/single_member.rs:987; many
+// members = the member-per-worker path, src/decompress/bgzf.rs:2370):
+//   1. envelope   RFC 1952 header / trailer per member (gzip_format.rs:47,135)
+//   2. find       block-finder hits over large members' deflate bits
+//                 (k_bf_stage_a / k_bf_stage_b)
+//   3. plan       pieces on a chunk grid: piece 0 of each member starts at the
+//                 deflate start with the all-zero window and writes straight
+//                 into the member output; the others start at the first hit
+//                 past each grid point and decode window-absent (markers)
+//   4. inflate    one persistent launch, one wavefront per piece (k_inflate)
+//   5. confirm    speculation check: piece j is accepted when piece j-1 ended
+//                 exactly at its start (chunk_fetcher.rs:2506-2729 semantics);
+//                 misses and slab overflows are re-decoded from the confirmed
+//                 boundary in further launches
+//   6. resolve    window chain + marker resolution (k_chain, k_resolve_body)
+//   7. verify     CRC32 fold + GF(2) combine (k_crc_*) and ISIZE against the
+//                 trailer (sm_driver.rs:212-224)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gzgpu.h"
+#include "gz_common.h"
+#include "k_blockfinder.h"
+#include "k_crc.h"
+#include "k_inflate.h"
+#include "k_resolve.h"
+
+using namespace gz;
+
+namespace {
+
+constexpr uint64_t KIB = 1024, MIB = 1024 * 1024;
+constexpr uint64_t NO_STOP = ~0ull;
+constexpr size_t ENVELOPE_HEAD = 512;
+
+struct HipError {
+    hipError_t e;
+    const char *what;
+};
+
+#define HIPCHK(x)                                              \
+    do {                                                       \
+        hipError_t e_ = (x);                                   \
+        if (e_ != hipSuccess) throw HipError{e_, #x};          \
+    } while (0)
+
+struct StatusError {
+    int status;
+    std::string msg;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n) {
+        if (n <= cap) return;
+        if (p) HIPCHK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        size_t c = std::max<size_t>(n, 4096);
+        hipError_t e = hipMalloc(&p, c);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw StatusError{GZGPU_OUT_OF_DEVICE_MEMORY, "hipMalloc(" + std::to_string(c) + ") failed"};
+        }
+        cap = c;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return reinterpret_cast<T *>(p);
+    }
+};
+
+// ---------------------------------------------------------------- host CRC tables
+struct CrcConst {
+    uint32_t tab[8][256];
+    uint32_t xbyte[16][16];
+    CrcConst() {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ CRC_POLY : c >> 1;
+            tab[0][i] = c;
+        }
+        for (uint32_t i = 0; i < 256; i++)
+            for (int t = 1; t < 8; t++) tab[t][i] = (tab[t - 1][i] >> 8) ^ tab[0][tab[t - 1][i] & 0xFF];
+        // x^(2^k) for k = 0..; x^(8 * j * 16^k) = product over set bits
+        uint32_t x2n[80];
+        x2n[0] = 1u << 30;
+        for (int k = 1; k < 80; k++) x2n[k] = mulmod(x2n[k - 1], x2n[k - 1]);
+        for (int k = 0; k < 16; k++)
+            for (int j = 0; j < 16; j++) {
+                // exponent 8 * j * 16^k = j << (4k + 3)
+                uint32_t p = 1u << 31;
+                for (int b = 0; b < 4; b++)
+                    if (j & (1 << b)) p = mulmod(x2n[4 * k + 3 + b], p);
+                xbyte[k][j] = p;
+            }
+    }
+    static uint32_t mulmod(uint32_t a, uint32_t b) {
+        uint32_t r = 0;
+        for (uint32_t m = 1u << 31; m; m >>= 1) {
+            if (a & m) r ^= b;
+            b = (b & 1) ? (b >> 1) ^ CRC_POLY : b >> 1;
+        }
+        return r;
+    }
+};
+const CrcConst &crc_const() {
+    static CrcConst c;
+    return c;
+}
+
+// RFC 1952 header length (gzip_format.rs:47-131).  0 = ok.
+int parse_header(const uint8_t *p, size_t n, size_t *hdr_len) {
+    if (n < 10) return GZGPU_INVALID_HEADER;
+    if (p[0] != 0x1f || p[1] != 0x8b || p[2] != 8) return GZGPU_INVALID_HEADER;
+    uint8_t flg = p[3];
+    size_t off = 10;
+    if (flg & 4) {
+        if (off + 2 > n) return GZGPU_INVALID_HEADER;
+        size_t xlen = (size_t)p[off] | (size_t)p[off + 1] << 8;
+        off += 2;
+        if (off + xlen > n) return GZGPU_INVALID_HEADER;
+        off += xlen;
+    }
+    for (int f = 8; f <= 16; f <<= 1) {
+        if (flg & f) {
+            while (off < n && p[off]) off++;
+            if (off >= n) return GZGPU_INVALID_HEADER;
+            off++;
+        }
+    }
+    if (flg & 2) {
+        if (off + 2 > n) return GZGPU_INVALID_HEADER;
+        off += 2;
+    }
+    *hdr_len = off;
+    return GZGPU_OK;
+}
+
+uint32_t le32(const uint8_t *p) { return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24; }
+
+__global__ void k_gather_envelope(const uint8_t *__restrict__ in, const uint64_t *__restrict__ off,
+                                  const uint64_t *__restrict__ len, uint32_t n, uint8_t *__restrict__ stage) {
+    const uint32_t per = ENVELOPE_HEAD + 8;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * per;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t m = (uint32_t)(i / per), k = (uint32_t)(i % per);
+        uint64_t L = len[m];
+        uint8_t v = 0;
+        if (k < ENVELOPE_HEAD) {
+            if (k < L) v = in[off[m] + k];
+        } else if (L >= 8) {
+            v = in[off[m] + L - 8 + (k - ENVELOPE_HEAD)];
+        }
+        stage[i] = v;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- context
+
+struct Member {
+    uint64_t ds = 0, de = 0;     // deflate bit range (absolute in d_in); de = trailer start
+    uint64_t limit = 0;          // readable bits
+    uint8_t *out = nullptr;
+    uint64_t out_cap = 0;
+    uint32_t exp_crc = 0, exp_isize = 0;
+    bool split = false;
+    // results
+    int status = GZGPU_OK;
+    std::string msg;
+    uint64_t out_len = 0;
+    uint64_t end_bit = 0;
+    uint32_t crc = 0;
+    std::vector<uint32_t> pieces;  // in stream order
+};
+
+struct PieceHost {
+    PieceDesc d;
+    uint64_t acc_lo = 0, acc_hi = 0;  // accepted predecessor end range
+    uint32_t member = 0;
+    bool confirmed = false, dropped = false;
+};
+
+struct gzgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    gzgpu_tuning tune{};
+    std::string err;
+    DevBuf d_in, d_out, d_pieces, d_pout, d_order, d_counters, d_surv, d_hits, d_slab, d_crc_tab, d_seg_off,
+        d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len;
+    std::vector<DevBuf> extra_slabs;
+    hipEvent_t ev[8]{};
+    gzgpu_stats st{};
+};
+
+namespace {
+
+void set_err(gzgpu_ctx *c, const std::string &m) {
+    if (c) c->err = m;
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return ms;
+}
+
+void upload(gzgpu_ctx *c, DevBuf &b, const void *src, size_t bytes) {
+    b.ensure(bytes);
+    if (bytes) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+}
+
+// ---- block finder over [start_bit, end_bit) of d_in -----------------------
+std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, uint64_t start_bit,
+                                      uint64_t end_bit) {
+    std::vector<gzgpu_boundary> out;
+    if (end_bit <= start_bit) return out;
+    uint64_t nbits = end_bit - start_bit;
+    uint32_t surv_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(nbits / 64, 4096), 1ull << 28);
+    uint32_t hit_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(nbits / 1024, 4096), 1ull << 26);
+    for (int attempt = 0; attempt < 3; attempt++) {
+        c->d_counters.ensure(64);
+        c->d_surv.ensure((size_t)surv_cap * 8);
+        c->d_hits.ensure((size_t)hit_cap * sizeof(gzgpu_boundary));
+        HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
+        uint32_t *cnt = c->d_counters.as<uint32_t>();
+        uint64_t blocks = std::min<uint64_t>((nbits + 255) / 256, (uint64_t)c->num_cus * 64);
+        hipLaunchKernelGGL(k_bf_stage_a, dim3((uint32_t)blocks), dim3(256), 0, c->stream, d_in, in_len, start_bit, end_bit,
+                           c->d_surv.as<uint64_t>(), surv_cap, cnt + 0, c->d_hits.as<gzgpu_boundary>(), hit_cap,
+                           cnt + 1);
+        HIPCHK(hipGetLastError());
+        uint32_t h_cnt[2];
+        HIPCHK(hipMemcpyAsync(h_cnt, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->st.launches++;
+        if (h_cnt[0] > surv_cap) {
+            surv_cap = h_cnt[0] + 1024;
+            continue;
+        }
+        uint32_t nsurv = h_cnt[0];
+        if (nsurv) {
+            uint32_t bb = std::min<uint32_t>((nsurv + 255) / 256, (uint32_t)c->num_cus * 16);
+            hipLaunchKernelGGL(k_bf_stage_b, dim3(bb), dim3(256), 0, c->stream, d_in, in_len, c->d_surv.as<uint64_t>(),
+                               nsurv, c->d_hits.as<gzgpu_boundary>(), hit_cap, cnt + 1);
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+        }
+        HIPCHK(hipMemcpyAsync(h_cnt, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (h_cnt[1] > hit_cap) {
+            hit_cap = h_cnt[1] + 1024;
+            continue;
+        }
+        out.resize(h_cnt[1]);
+        if (h_cnt[1])
+            HIPCHK(hipMemcpyAsync(out.data(), c->d_hits.p, out.size() * sizeof(gzgpu_boundary), hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::sort(out.begin(), out.end(), [](const gzgpu_boundary &a, const gzgpu_boundary &b) {
+            return a.bit_offset != b.bit_offset ? a.bit_offset < b.bit_offset : a.seek_bit < b.seek_bit;
+        });
+        return out;
+    }
+    throw StatusError{GZGPU_OUT_OF_DEVICE_MEMORY, "block finder: candidate buffers kept overflowing"};
+}
+
+uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+uint64_t marker_slab_bytes(uint64_t expect_entries) {
+    return round_up(2 * (uint64_t)WIN + 2 * (expect_entries + 64 * KIB) + WIN + 512, 256);
+}
+
+// ---- the member pipeline ---------------------------------------------------
+void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
+    auto t_host0 = std::chrono::steady_clock::now();
+    (void)t_host0;
+    const uint64_t chunk_bits = (c->tune.chunk_bytes ? c->tune.chunk_bytes : 512 * KIB) * 8;
+    const uint32_t ppc = c->tune.pieces_per_chunk ? c->tune.pieces_per_chunk : 8;
+    const uint64_t spacing = std::max<uint64_t>(chunk_bits / ppc, 8 * 32 * KIB);
+
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    // ---- 2/3: find + plan
+    std::vector<PieceHost> P;
+    for (uint32_t m = 0; m < mem.size(); m++) {
+        Member &M = mem[m];
+        if (M.status != GZGPU_OK) continue;
+        PieceHost p0;
+        p0.d.start_bit = M.ds;
+        p0.d.stop_bit = NO_STOP;
+        p0.d.limit_bit = M.limit;
+        p0.d.dst = M.out;
+        p0.d.cap = M.out_cap;
+        p0.d.flags = PF_DIRECT;
+        p0.d.member = m;
+        p0.member = m;
+        p0.acc_lo = p0.acc_hi = M.ds;
+        p0.confirmed = true;
+        M.pieces.clear();
+        M.pieces.push_back((uint32_t)P.size());
+        P.push_back(p0);
+        if (!M.split || M.de <= M.ds + spacing) continue;
+        std::vector<gzgpu_boundary> hits = find_hits(c, d_in, in_len, M.ds + 1, M.de);
+        c->st.candidates += (uint32_t)hits.size();
+        size_t h = 0;
+        uint64_t last = M.ds;
+        std::vector<size_t> chosen;
+        for (uint64_t g = M.ds + spacing; g < M.de; g += spacing) {
+            while (h < hits.size() && (hits[h].bit_offset < g || hits[h].bit_offset <= last)) h++;
+            if (h >= hits.size()) break;
+            if (hits[h].seek_bit >= M.de) break;
+            chosen.push_back(h);
+            last = hits[h].bit_offset;
+        }
+        const uint64_t total_bits = M.de - M.ds;
+        for (size_t k = 0; k < chosen.size(); k++) {
+            const gzgpu_boundary &b = hits[chosen[k]];
+            P[M.pieces.back()].d.stop_bit = b.bit_offset;
+            PieceHost q;
+            q.d.start_bit = b.seek_bit;
+            q.d.stop_bit = NO_STOP;
+            q.d.limit_bit = M.limit;
+            q.d.flags = PF_MARKER;
+            q.d.member = m;
+            q.member = m;
+            q.acc_lo = b.bit_offset;
+            q.acc_hi = b.seek_bit;
+            uint64_t end = k + 1 < chosen.size() ? hits[chosen[k + 1]].bit_offset : M.de;
+            double frac = (double)(end - b.bit_offset) / (double)total_bits;
+            q.d.cap = marker_slab_bytes((uint64_t)(1.3 * frac * (double)M.exp_isize));
+            M.pieces.push_back((uint32_t)P.size());
+            P.push_back(q);
+        }
+    }
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+
+    // slab pool for the first launch
+    uint64_t pool = 0;
+    for (auto &p : P)
+        if (p.d.flags & PF_MARKER) pool += p.d.cap;
+    c->d_slab.ensure(std::max<uint64_t>(pool, 256));
+    {
+        uint64_t off = 0;
+        for (auto &p : P)
+            if (p.d.flags & PF_MARKER) {
+                p.d.dst = c->d_slab.as<uint8_t>() + off;
+                off += p.d.cap;
+            }
+    }
+
+    std::vector<PieceOut> R(P.size());
+    std::vector<uint32_t> todo(P.size());
+    for (uint32_t i = 0; i < P.size(); i++) todo[i] = i;
+    c->st.pieces = (uint32_t)P.size();
+    c->d_counters.ensure(64);
+    int rounds = 0;
+    while (!todo.empty()) {
+        if (++rounds > 64) throw StatusError{GZGPU_DECODE_FAILED, "speculation did not converge"};
+        // largest first
+        std::sort(todo.begin(), todo.end(), [&](uint32_t a, uint32_t b) {
+            uint64_t ea = (P[a].d.stop_bit == NO_STOP ? mem[P[a].member].de : P[a].d.stop_bit) - P[a].d.start_bit;
+            uint64_t eb = (P[b].d.stop_bit == NO_STOP ? mem[P[b].member].de : P[b].d.stop_bit) - P[b].d.start_bit;
+            return ea > eb;
+        });
+        std::vector<PieceDesc> descs(todo.size());
+        for (size_t i = 0; i < todo.size(); i++) descs[i] = P[todo[i]].d;
+        std::vector<uint32_t> order(todo.size());
+        for (uint32_t i = 0; i < todo.size(); i++) order[i] = i;
+        upload(c, c->d_pieces, descs.data(), descs.size() * sizeof(PieceDesc));
+        upload(c, c->d_order, order.data(), order.size() * 4);
+        c->d_pout.ensure(descs.size() * sizeof(PieceOut));
+        HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
+        uint32_t wpc = c->tune.waves_per_cu ? c->tune.waves_per_cu : 16;
+        uint32_t grid = (uint32_t)std::min<uint64_t>(descs.size(), (uint64_t)c->num_cus * wpc);
+        HIPCHK(hipEventRecord(c->ev[5], c->stream));
+        hipLaunchKernelGGL(k_inflate, dim3(grid), dim3(64), 0, c->stream, d_in, in_len, c->d_pieces.as<PieceDesc>(),
+                           c->d_order.as<uint32_t>(), (uint32_t)descs.size(), c->d_counters.as<uint32_t>(),
+                           c->d_pout.as<PieceOut>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[6], c->stream));
+        c->st.launches++;
+        std::vector<PieceOut> ro(descs.size());
+        HIPCHK(hipMemcpyAsync(ro.data(), c->d_pout.p, ro.size() * sizeof(PieceOut), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->st.ms_inflate_kernel += ev_ms(c->ev[5], c->ev[6]);
+        c->st.inflate_launches++;
+        for (size_t i = 0; i < todo.size(); i++) R[todo[i]] = ro[i];
+        if (rounds > 1) c->st.redecoded += (uint32_t)todo.size();
+        todo.clear();
+
+        // confirm chains
+        uint64_t extra_need = 0;
+        std::vector<uint32_t> grow;
+        for (uint32_t m = 0; m < mem.size(); m++) {
+            Member &M = mem[m];
+            if (M.status != GZGPU_OK || M.pieces.empty()) continue;
+            for (size_t k = 0; k < M.pieces.size(); k++) {
+                uint32_t pi = M.pieces[k];
+                PieceHost &p = P[pi];
+                if (p.dropped) continue;
+                if (k > 0) {
+                    PieceHost &prev = P[M.pieces[k - 1]];
+                    const PieceOut &pr = R[M.pieces[k - 1]];
+                    if (!prev.confirmed) break;
+                    if (pr.status == PS_FINAL) {
+                        // stream ended before this speculative start: drop the rest
+                        for (size_t j = k; j < M.pieces.size(); j++) P[M.pieces[j]].dropped = true;
+                        break;
+                    }
+                    if (pr.status != PS_STOPPED) break;
+                    if (!p.confirmed) {
+                        if (pr.end_bit >= p.acc_lo && pr.end_bit <= p.acc_hi) {
+                            p.confirmed = true;
+                        } else {
+                            // speculation miss: re-decode from the confirmed boundary
+                            p.d.start_bit = p.acc_lo = p.acc_hi = pr.end_bit;
+                            p.confirmed = true;
+                            todo.push_back(pi);
+                            break;
+                        }
+                    }
+                }
+                const PieceOut &r = R[pi];
+                if (r.status == PS_ERROR) {
+                    M.status = GZGPU_DECODE_FAILED;
+                    M.msg = "deflate stream invalid at bit " + std::to_string(r.end_bit) + " (piece " +
+                            std::to_string(k) + ", error " + std::to_string(r.err) + ")";
+                    break;
+                }
+                if (r.status == PS_OVERFLOW) {
+                    if (p.d.flags & PF_DIRECT) {
+                        M.status = GZGPU_OUTPUT_TOO_SMALL;
+                        M.msg = "output capacity exceeded";
+                        break;
+                    }
+                    // grow the slab from the observed expansion and re-decode
+                    uint64_t entries = r.n16 + r.n8;
+                    uint64_t span = (p.d.stop_bit == NO_STOP ? M.de : p.d.stop_bit) - p.d.start_bit;
+                    double rate = r.bits_done ? (double)entries / (double)r.bits_done : 1.0;
+                    uint64_t want = (uint64_t)(rate * (double)span * 1.5) + entries;
+                    p.d.cap = std::max<uint64_t>(marker_slab_bytes(want), 2 * p.d.cap);
+                    extra_need += p.d.cap;
+                    grow.push_back(pi);
+                    todo.push_back(pi);
+                    break;
+                }
+            }
+        }
+        if (!grow.empty()) {
+            DevBuf b;
+            b.ensure(extra_need);
+            uint64_t off = 0;
+            for (uint32_t pi : grow) {
+                P[pi].d.dst = b.as<uint8_t>() + off;
+                off += P[pi].d.cap;
+            }
+            c->extra_slabs.push_back(b);
+        }
+    }
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+
+    // ---- assemble: offsets, sizes, resolve lists
+    std::vector<ResolvePiece> rps;
+    std::vector<uint32_t> chains;
+    std::vector<ResolveTile> tiles;
+    for (uint32_t m = 0; m < mem.size(); m++) {
+        Member &M = mem[m];
+        if (M.status != GZGPU_OK) continue;
+        uint64_t pos = 0;
+        bool final_seen = false;
+        bool chain_open = false;
+        for (size_t k = 0; k < M.pieces.size(); k++) {
+            uint32_t pi = M.pieces[k];
+            if (P[pi].dropped) break;
+            const PieceOut &r = R[pi];
+            uint64_t L = r.n16 + r.n8;
+            if (P[pi].d.flags & PF_MARKER) {
+                if (pos + L > M.out_cap) {
+                    M.status = GZGPU_OUTPUT_TOO_SMALL;
+                    M.msg = "output capacity exceeded";
+                    break;
+                }
+                if (!chain_open) {
+                    chains.push_back((uint32_t)rps.size());
+                    chain_open = true;
+                }
+                ResolvePiece rp;
+                rp.slab = P[pi].d.dst;
+                rp.mout = M.out;
+                rp.out_pos = pos;
+                rp.n16 = r.n16;
+                rp.n8 = r.n8;
+                rp.u8_off = r.u8_off;
+                uint32_t idx = (uint32_t)rps.size();
+                rps.push_back(rp);
+                if (L > WIN) {
+                    const uint64_t tile = 64 * KIB;
+                    for (uint64_t t = 0; t < L - WIN; t += tile) {
+                        ResolveTile tl;
+                        tl.piece = idx;
+                        tl.pad = 0;
+                        tl.t0 = t;
+                        tl.t1 = std::min<uint64_t>(t + tile, L - WIN);
+                        tiles.push_back(tl);
+                    }
+                }
+            }
+            pos += L;
+            if (r.status == PS_FINAL) {
+                final_seen = true;
+                M.end_bit = r.end_bit;
+                break;
+            }
+        }
+        if (M.status != GZGPU_OK) continue;
+        if (!final_seen) {
+            M.status = GZGPU_DECODE_FAILED;
+            M.msg = "deflate stream ended without a final block";
+            continue;
+        }
+        M.out_len = pos;
+    }
+    if (!rps.empty()) {
+        chains.push_back((uint32_t)rps.size());
+        upload(c, c->d_rp, rps.data(), rps.size() * sizeof(ResolvePiece));
+        upload(c, c->d_chains, chains.data(), chains.size() * 4);
+        uint32_t nch = (uint32_t)chains.size() - 1;
+        hipLaunchKernelGGL(k_chain, dim3(std::min<uint32_t>(nch, (uint32_t)c->num_cus * 4)), dim3(1024), 0, c->stream,
+                           c->d_rp.as<ResolvePiece>(), c->d_chains.as<uint32_t>(), nch);
+        HIPCHK(hipGetLastError());
+        c->st.launches++;
+        if (!tiles.empty()) {
+            upload(c, c->d_tiles, tiles.data(), tiles.size() * sizeof(ResolveTile));
+            hipLaunchKernelGGL(k_resolve_body, dim3((uint32_t)std::min<size_t>(tiles.size(), (size_t)c->num_cus * 32)),
+                               dim3(256), 0, c->stream, c->d_rp.as<ResolvePiece>(), c->d_tiles.as<ResolveTile>(),
+                               (uint32_t)tiles.size());
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+        }
+    }
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+
+    // ---- CRC32 per member
+    std::vector<uint64_t> seg_off;
+    std::vector<uint32_t> seg_len, mseg;
+    std::vector<uint64_t> mlen;
+    std::vector<uint32_t> crc_members;
+    const uint8_t *base = nullptr;
+    for (auto &M : mem)
+        if (M.status == GZGPU_OK && (!base || M.out < base)) base = M.out;
+    for (uint32_t m = 0; m < mem.size(); m++) {
+        Member &M = mem[m];
+        if (M.status != GZGPU_OK) continue;
+        crc_members.push_back(m);
+        mseg.push_back((uint32_t)seg_off.size());
+        mlen.push_back(M.out_len);
+        for (uint64_t o = 0; o < M.out_len; o += CRC_SEG) {
+            seg_off.push_back((uint64_t)(M.out - base) + o);
+            seg_len.push_back((uint32_t)std::min<uint64_t>(CRC_SEG, M.out_len - o));
+        }
+    }
+    if (!crc_members.empty()) {
+        mseg.push_back((uint32_t)seg_off.size());
+        const CrcConst &cc = crc_const();
+        if (c->d_crc_tab.cap == 0) {
+            c->d_crc_tab.ensure(sizeof(cc.tab));
+            HIPCHK(hipMemcpy(c->d_crc_tab.p, cc.tab, sizeof(cc.tab), hipMemcpyHostToDevice));
+        }
+        uint32_t nseg = (uint32_t)seg_off.size();
+        c->d_seg_crc.ensure(std::max<size_t>(nseg, 1) * 4);
+        if (nseg) {
+            upload(c, c->d_seg_off, seg_off.data(), seg_off.size() * 8);
+            upload(c, c->d_seg_len, seg_len.data(), seg_len.size() * 4);
+            hipLaunchKernelGGL(k_crc_segments, dim3(std::min<uint32_t>(nseg, (uint32_t)c->num_cus * 8)), dim3(256), 0,
+                               c->stream, base, c->d_seg_off.as<uint64_t>(), c->d_seg_len.as<uint32_t>(), nseg,
+                               c->d_crc_tab.as<uint32_t>(), c->d_seg_crc.as<uint32_t>());
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+        }
+        uint32_t nm = (uint32_t)crc_members.size();
+        upload(c, c->d_mseg, mseg.data(), mseg.size() * 4);
+        upload(c, c->d_mlen, mlen.data(), mlen.size() * 8);
+        c->d_mcrc.ensure(nm * 4);
+        hipLaunchKernelGGL(k_crc_members, dim3(std::min<uint32_t>(nm, (uint32_t)c->num_cus * 8)), dim3(256), 0, c->stream,
+                           c->d_seg_crc.as<uint32_t>(), c->d_seg_len.as<uint32_t>(), c->d_mseg.as<uint32_t>(),
+                           c->d_mlen.as<uint64_t>(), nm, c->d_mcrc.as<uint32_t>());
+        HIPCHK(hipGetLastError());
+        c->st.launches++;
+        std::vector<uint32_t> crcs(nm);
+        HIPCHK(hipMemcpyAsync(crcs.data(), c->d_mcrc.p, nm * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (uint32_t i = 0; i < nm; i++) {
+            Member &M = mem[crc_members[i]];
+            M.crc = crcs[i];
+            if ((uint32_t)M.out_len != M.exp_isize) {
+                M.status = GZGPU_SIZE_MISMATCH;
+                M.msg = "ISIZE " + std::to_string(M.exp_isize) + " != decoded " + std::to_string(M.out_len);
+            } else if (M.crc != M.exp_crc) {
+                M.status = GZGPU_CRC_MISMATCH;
+                char b[96];
+                snprintf(b, sizeof b, "CRC32 %08x != trailer %08x", M.crc, M.exp_crc);
+                M.msg = b;
+            }
+        }
+    } else {
+        HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->st.ms_find += ev_ms(c->ev[0], c->ev[1]);
+    c->st.ms_inflate += ev_ms(c->ev[1], c->ev[2]);
+    c->st.ms_resolve += ev_ms(c->ev[2], c->ev[3]);
+    c->st.ms_crc += ev_ms(c->ev[3], c->ev[4]);
+    c->st.ms_total += ev_ms(c->ev[0], c->ev[4]);
+    for (auto &b : c->extra_slabs) b.release();
+    c->extra_slabs.clear();
+}
+
+// Parse envelope from host-visible bytes of one member starting at `p`.
+int member_from_host(const uint8_t *p, size_t avail, uint64_t abs_off, Member &M, std::string &msg) {
+    size_t hl;
+    int st = parse_header(p, avail, &hl);
+    if (st) {
+        msg = "invalid gzip header";
+        return st;
+    }
+    if (avail < hl + 8) {
+        msg = "input shorter than header + trailer";
+        return GZGPU_INVALID_FORMAT;
+    }
+    M.ds = (abs_off + hl) * 8;
+    M.de = (abs_off + avail - 8) * 8;
+    M.limit = (abs_off + avail) * 8;
+    M.exp_crc = le32(p + avail - 8);
+    M.exp_isize = le32(p + avail - 4);
+    return GZGPU_OK;
+}
+
+int finish(gzgpu_ctx *c, int st, const std::string &msg) {
+    if (st != GZGPU_OK) set_err(c, msg);
+    return st;
+}
+
+template <typename F>
+int guarded(gzgpu_ctx *c, F &&f) {
+    try {
+        return f();
+    } catch (const HipError &e) {
+        set_err(c, std::string("HIP error: ") + hipGetErrorString(e.e) + " in " + e.what);
+        (void)hipGetLastError();
+        return GZGPU_GPU_UNAVAILABLE;
+    } catch (const StatusError &e) {
+        set_err(c, e.msg);
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        set_err(c, "host allocation failed");
+        return GZGPU_OUT_OF_DEVICE_MEMORY;
+    }
+}
+
+void init_constants(gzgpu_ctx *c) {
+    const CrcConst &cc = crc_const();
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(gz::c_xbyte), cc.xbyte, sizeof(cc.xbyte)));
+    (void)c;
+}
+
+// Decode the members of a host buffer one after another (multi-member walk,
+// src/decompress/mod.rs:725-741): each member runs the parallel pipeline.
+template <typename Sink>
+int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink &&sink, uint64_t *total,
+                           gzgpu_stats *stats) {
+    c->st = gzgpu_stats{};
+    *total = 0;
+    if (in_len < 2 || in[0] != 0x1f || in[1] != 0x8b) return GZGPU_OK;  // non-gzip: Ok(0)
+    c->d_in.ensure(in_len + 64);
+    HIPCHK(hipMemcpyAsync(c->d_in.p, in, in_len, hipMemcpyHostToDevice, c->stream));
+    size_t off = 0;
+    int members = 0;
+    std::vector<uint8_t> host_out;
+    while (off < in_len) {
+        if (members > 0 && (in_len - off < 2 || in[off] != 0x1f || in[off + 1] != 0x8b)) break;  // trailing bytes
+        Member M;
+        std::string msg;
+        int st = member_from_host(in + off, in_len - off, off, M, msg);
+        if (st) {
+            if (members > 0) break;
+            return finish(c, st, msg);
+        }
+        M.split = true;
+        // the member may be followed by others: size the output from the
+        // trailer at the end of the input, growing on overflow
+        uint64_t cap = std::max<uint64_t>((uint64_t)M.exp_isize + 64 * KIB, 1 * MIB);
+        for (int tries = 0;; tries++) {
+            c->d_out.ensure(cap);
+            M.out = c->d_out.as<uint8_t>();
+            M.out_cap = c->d_out.cap;
+            M.status = GZGPU_OK;
+            std::vector<Member> v{M};
+            run_members(c, c->d_in.as<uint8_t>(), in_len, v);
+            M = v[0];
+            if (M.status == GZGPU_OUTPUT_TOO_SMALL && tries < 8) {
+                cap = c->d_out.cap * 4;
+                continue;
+            }
+            break;
+        }
+        // the member's real end: trailer right after the final block
+        if (M.status == GZGPU_OK || M.status == GZGPU_SIZE_MISMATCH || M.status == GZGPU_CRC_MISMATCH) {
+            uint64_t tr = (M.end_bit + 7) / 8;
+            if (tr + 8 <= in_len && tr + 8 != in_len) {
+                // more bytes follow: re-check against this member's own trailer
+                uint32_t crc = le32(in + tr), isz = le32(in + tr + 4);
+                M.exp_crc = crc;
+                M.exp_isize = isz;
+                if ((uint32_t)M.out_len != isz) {
+                    M.status = GZGPU_SIZE_MISMATCH;
+                    M.msg = "ISIZE mismatch";
+                } else if (M.crc != crc) {
+                    M.status = GZGPU_CRC_MISMATCH;
+                    M.msg = "CRC32 mismatch";
+                } else {
+                    M.status = GZGPU_OK;
+                }
+            } else if (tr + 8 > in_len) {
+                M.status = GZGPU_DECODE_FAILED;
+                M.msg = "truncated gzip trailer";
+            }
+            if (M.status != GZGPU_OK) return finish(c, M.status, M.msg);
+            host_out.resize(M.out_len);
+            if (M.out_len)
+                HIPCHK(hipMemcpyAsync(host_out.data(), M.out, M.out_len, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            int sr = sink(host_out.data(), (size_t)M.out_len);
+            if (sr) return sr;
+            *total += M.out_len;
+            off = tr + 8;
+            members++;
+            continue;
+        }
+        return finish(c, M.status, M.msg);
+    }
+    c->st.compressed_bytes = in_len;
+    c->st.decompressed_bytes = *total;
+    if (stats) *stats = c->st;
+    return GZGPU_OK;
+}
+
+}  // namespace
+
+// =========================================================================
+// C ABI
+// =========================================================================
+extern "C" {
+
+int gzgpu_ctx_create(int device, gzgpu_ctx **out) {
+    if (!out) return GZGPU_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) {
+        (void)hipGetLastError();
+        return GZGPU_GPU_UNAVAILABLE;
+    }
+    gzgpu_ctx *c = new gzgpu_ctx();
+    c->device = device;
+    int st = guarded(c, [&]() {
+        HIPCHK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            throw StatusError{GZGPU_GPU_UNAVAILABLE, std::string("device is ") + prop.gcnArchName + ", need gfx950"};
+        c->num_cus = prop.multiProcessorCount;
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+        init_constants(c);
+        return (int)GZGPU_OK;
+    });
+    if (st != GZGPU_OK) {
+        fprintf(stderr, "gzgpu_ctx_create: %s\n", c->err.c_str());
+        gzgpu_ctx_destroy(c);
+        return st;
+    }
+    *out = c;
+    return GZGPU_OK;
+}
+
+void gzgpu_ctx_destroy(gzgpu_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (DevBuf *b : {&c->d_in, &c->d_out, &c->d_pieces, &c->d_pout, &c->d_order, &c->d_counters, &c->d_surv, &c->d_hits,
+                      &c->d_slab, &c->d_crc_tab, &c->d_seg_off, &c->d_seg_len, &c->d_seg_crc, &c->d_mseg, &c->d_mlen,
+                      &c->d_mcrc, &c->d_rp, &c->d_tiles, &c->d_chains, &c->d_env, &c->d_env_off, &c->d_env_len})
+        b->release();
+    for (auto &b : c->extra_slabs) b.release();
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int gzgpu_set_tuning(gzgpu_ctx *c, const gzgpu_tuning *t) {
+    if (!c || !t) return GZGPU_INVALID_ARGUMENT;
+    c->tune = *t;
+    return GZGPU_OK;
+}
+
+int gzgpu_get_tuning(gzgpu_ctx *c, gzgpu_tuning *t) {
+    if (!c || !t) return GZGPU_INVALID_ARGUMENT;
+    *t = c->tune;
+    if (!t->chunk_bytes) t->chunk_bytes = 512 * KIB;
+    if (!t->pieces_per_chunk) t->pieces_per_chunk = 8;
+    if (!t->waves_per_cu) t->waves_per_cu = 16;
+    if (!t->batch_split_bytes) t->batch_split_bytes = 4 * MIB;
+    return GZGPU_OK;
+}
+
+const char *gzgpu_last_error(gzgpu_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+const char *gzgpu_status_string(int s) {
+    switch (s) {
+    case GZGPU_OK: return "ok";
+    case GZGPU_INVALID_HEADER: return "invalid gzip header";
+    case GZGPU_INVALID_FORMAT: return "input below parallel SM minimum (routing bug)";
+    case GZGPU_DECODE_FAILED: return "chunk decode failed";
+    case GZGPU_SIZE_MISMATCH: return "output size mismatch";
+    case GZGPU_CRC_MISMATCH: return "CRC32 mismatch";
+    case GZGPU_IO: return "I/O error";
+    case GZGPU_GPU_UNAVAILABLE: return "GPU unavailable";
+    case GZGPU_OUT_OF_DEVICE_MEMORY: return "out of device memory";
+    case GZGPU_OUTPUT_TOO_SMALL: return "output buffer too small";
+    case GZGPU_INVALID_ARGUMENT: return "invalid argument";
+    default: return "unknown status";
+    }
+}
+
+int gzgpu_decompress(gzgpu_ctx *c, const uint8_t *in, size_t in_len, uint8_t *out, size_t out_cap, size_t *out_len,
+                     gzgpu_stats *stats) {
+    if (!c || (!in && in_len) || !out_len) return GZGPU_INVALID_ARGUMENT;
+    return guarded(c, [&]() {
+        HIPCHK(hipSetDevice(c->device));
+        size_t written = 0;
+        bool too_small = false;
+        auto sink = [&](const uint8_t *d, size_t n) -> int {
+            if (!too_small && written + n <= out_cap) {
+                if (n) std::memcpy(out + written, d, n);
+            } else {
+                too_small = true;
+            }
+            written += n;
+            return 0;
+        };
+        uint64_t total = 0;
+        int st = decompress_host_stream(c, in, in_len, sink, &total, stats);
+        *out_len = written;
+        if (st) return st;
+        if (too_small) {
+            set_err(c, "output buffer too small");
+            return (int)GZGPU_OUTPUT_TOO_SMALL;
+        }
+        return (int)GZGPU_OK;
+    });
+}
+
+int gzgpu_decompress_to_writer(gzgpu_ctx *c, const uint8_t *in, size_t in_len, gzgpu_write_fn write, void *user,
+                               uint64_t *total, gzgpu_stats *stats) {
+    if (!c || (!in && in_len) || !write || !total) return GZGPU_INVALID_ARGUMENT;
+    return guarded(c, [&]() {
+        HIPCHK(hipSetDevice(c->device));
+        auto sink = [&](const uint8_t *d, size_t n) -> int {
+            if (!n) return 0;
+            if (write(user, d, n) != 0) {
+                set_err(c, "writer callback failed");
+                return (int)GZGPU_IO;
+            }
+            return 0;
+        };
+        return decompress_host_stream(c, in, in_len, sink, total, stats);
+    });
+}
+
+int gzgpu_decompress_device(gzgpu_ctx *c, const void *d_in, size_t in_len, void *d_out, size_t out_cap,
+                            size_t *out_len, uint32_t *crc_out, gzgpu_stats *stats) {
+    if (!c || !d_in || !d_out || !out_len) return GZGPU_INVALID_ARGUMENT;
+    return guarded(c, [&]() {
+        HIPCHK(hipSetDevice(c->device));
+        c->st = gzgpu_stats{};
+        *out_len = 0;
+        // envelope: header from the first bytes, trailer from the last 8
+        size_t head = std::min<size_t>(in_len, 64 * KIB);
+        std::vector<uint8_t> h(head + 8);
+        HIPCHK(hipMemcpyAsync(h.data(), d_in, head, hipMemcpyDeviceToHost, c->stream));
+        if (in_len >= 8)
+            HIPCHK(hipMemcpyAsync(h.data() + head, (const uint8_t *)d_in + in_len - 8, 8, hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (in_len < 2 || h[0] != 0x1f || h[1] != 0x8b) return (int)GZGPU_OK;
+        size_t hl;
+        int st = parse_header(h.data(), head, &hl);
+        if (st) return finish(c, st, "invalid gzip header");
+        if (in_len < hl + 8) return finish(c, GZGPU_INVALID_FORMAT, "input shorter than header + trailer");
+        Member M;
+        M.ds = hl * 8;
+        M.de = (in_len - 8) * 8;
+        M.limit = in_len * 8;
+        M.exp_crc = le32(h.data() + head);
+        M.exp_isize = le32(h.data() + head + 4);
+        M.out = (uint8_t *)d_out;
+        M.out_cap = out_cap;
+        M.split = true;
+        std::vector<Member> v{M};
+        run_members(c, (const uint8_t *)d_in, in_len, v);
+        M = v[0];
+        *out_len = M.out_len;
+        if (crc_out) *crc_out = M.crc;
+        c->st.compressed_bytes = in_len;
+        c->st.decompressed_bytes = M.out_len;
+        if (stats) *stats = c->st;
+        return finish(c, M.status, M.msg);
+    });
+}
+
+int gzgpu_decompress_batch(gzgpu_ctx *c, const void *d_in, const uint64_t *in_off, const uint64_t *in_len, void *d_out,
+                           const uint64_t *out_off, const uint64_t *out_cap, size_t n, int32_t *status,
+                           uint64_t *out_len, uint32_t *crc, gzgpu_stats *stats) {
+    if (!c || (!n) || !d_in || !d_out || !in_off || !in_len || !out_off || !out_cap) return GZGPU_INVALID_ARGUMENT;
+    return guarded(c, [&]() {
+        HIPCHK(hipSetDevice(c->device));
+        c->st = gzgpu_stats{};
+        gzgpu_tuning tn;
+        gzgpu_get_tuning(c, &tn);
+        // gather every member's first bytes and trailer in one launch
+        const size_t per = ENVELOPE_HEAD + 8;
+        upload(c, c->d_env_off, in_off, n * 8);
+        upload(c, c->d_env_len, in_len, n * 8);
+        c->d_env.ensure(n * per);
+        hipLaunchKernelGGL(k_gather_envelope, dim3((uint32_t)std::min<size_t>((n * per + 255) / 256, 65535)), dim3(256), 0,
+                           c->stream, (const uint8_t *)d_in, c->d_env_off.as<uint64_t>(), c->d_env_len.as<uint64_t>(),
+                           (uint32_t)n, c->d_env.as<uint8_t>());
+        HIPCHK(hipGetLastError());
+        std::vector<uint8_t> env(n * per);
+        HIPCHK(hipMemcpyAsync(env.data(), c->d_env.p, env.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        uint64_t in_total = 0;
+        for (size_t i = 0; i < n; i++) in_total = std::max<uint64_t>(in_total, in_off[i] + in_len[i]);
+        std::vector<Member> mem(n);
+        for (size_t i = 0; i < n; i++) {
+            Member &M = mem[i];
+            const uint8_t *e = env.data() + i * per;
+            size_t hl = 0;
+            int st = parse_header(e, (size_t)std::min<uint64_t>(in_len[i], ENVELOPE_HEAD), &hl);
+            if (st == GZGPU_INVALID_HEADER && in_len[i] > ENVELOPE_HEAD) {
+                // long FEXTRA/FNAME: fetch the full header
+                std::vector<uint8_t> big((size_t)std::min<uint64_t>(in_len[i], 1 * MIB));
+                HIPCHK(hipMemcpy(big.data(), (const uint8_t *)d_in + in_off[i], big.size(), hipMemcpyDeviceToHost));
+                st = parse_header(big.data(), big.size(), &hl);
+            }
+            if (st || in_len[i] < hl + 8) {
+                M.status = st ? st : GZGPU_INVALID_FORMAT;
+                M.msg = "invalid gzip header";
+                continue;
+            }
+            M.ds = (in_off[i] + hl) * 8;
+            M.de = (in_off[i] + in_len[i] - 8) * 8;
+            M.limit = (in_off[i] + in_len[i]) * 8;
+            M.exp_crc = le32(e + ENVELOPE_HEAD);
+            M.exp_isize = le32(e + ENVELOPE_HEAD + 4);
+            M.out = (uint8_t *)d_out + out_off[i];
+            M.out_cap = out_cap[i];
+            M.split = in_len[i] > tn.batch_split_bytes;
+        }
+        run_members(c, (const uint8_t *)d_in, in_total, mem);
+        int first_bad = GZGPU_OK;
+        for (size_t i = 0; i < n; i++) {
+            if (status) status[i] = mem[i].status;
+            if (out_len) out_len[i] = mem[i].out_len;
+            if (crc) crc[i] = mem[i].crc;
+            if (mem[i].status != GZGPU_OK && first_bad == GZGPU_OK) {
+                first_bad = mem[i].status;
+                set_err(c, "member " + std::to_string(i) + ": " + mem[i].msg);
+            }
+        }
+        c->st.decompressed_bytes = 0;
+        for (auto &M : mem) c->st.decompressed_bytes += M.out_len;
+        for (size_t i = 0; i < n; i++) c->st.compressed_bytes += in_len[i];
+        if (stats) *stats = c->st;
+        return first_bad;
+    });
+}
+
+int gzgpu_find_blocks(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t start_bit, uint64_t end_bit,
+                      gzgpu_boundary *out, size_t cap, size_t *count) {
+    if (!c || !d_in || !count) return GZGPU_INVALID_ARGUMENT;
+    return guarded(c, [&]() {
+        HIPCHK(hipSetDevice(c->device));
+        uint64_t eb = std::min<uint64_t>(end_bit, (uint64_t)in_len * 8);
+        std::vector<gzgpu_boundary> h = find_hits(c, (const uint8_t *)d_in, in_len, start_bit, eb);
+        *count = h.size();
+        if (out) std::memcpy(out, h.data(), std::min(cap, h.size()) * sizeof(gzgpu_boundary));
+        return (int)GZGPU_OK;
+    });
+}
+
+int gzgpu_crc32(gzgpu_ctx *c, const void *d_data, size_t len, uint32_t *crc) {
+    if (!c || (!d_data && len) || !crc) return GZGPU_INVALID_ARGUMENT;
+    return guarded(c, [&]() {
+        HIPCHK(hipSetDevice(c->device));
+        const CrcConst &cc = crc_const();
+        if (c->d_crc_tab.cap == 0) {
+            c->d_crc_tab.ensure(sizeof(cc.tab));
+            HIPCHK(hipMemcpy(c->d_crc_tab.p, cc.tab, sizeof(cc.tab), hipMemcpyHostToDevice));
+        }
+        std::vector<uint64_t> so;
+        std::vector<uint32_t> sl;
+        for (uint64_t o = 0; o < len; o += CRC_SEG) {
+            so.push_back(o);
+            sl.push_back((uint32_t)std::min<uint64_t>(CRC_SEG, len - o));
+        }
+        uint32_t nseg = (uint32_t)so.size();
+        c->d_seg_crc.ensure(std::max<size_t>(nseg, 1) * 4);
+        if (nseg) {
+            upload(c, c->d_seg_off, so.data(), so.size() * 8);
+            upload(c, c->d_seg_len, sl.data(), sl.size() * 4);
+            hipLaunchKernelGGL(k_crc_segments, dim3(std::min<uint32_t>(nseg, (uint32_t)c->num_cus * 8)), dim3(256), 0,
+                               c->stream, (const uint8_t *)d_data, c->d_seg_off.as<uint64_t>(),
+                               c->d_seg_len.as<uint32_t>(), nseg, c->d_crc_tab.as<uint32_t>(),
+                               c->d_seg_crc.as<uint32_t>());
+            HIPCHK(hipGetLastError());
+        }
+        uint32_t ms[2] = {0, nseg};
+        uint64_t ml = len;
+        upload(c, c->d_mseg, ms, 8);
+        upload(c, c->d_mlen, &ml, 8);
+        c->d_mcrc.ensure(4);
+        hipLaunchKernelGGL(k_crc_members, dim3(1), dim3(256), 0, c->stream, c->d_seg_crc.as<uint32_t>(),
+                           c->d_seg_len.as<uint32_t>(), c->d_mseg.as<uint32_t>(), c->d_mlen.as<uint64_t>(), 1u,
+                           c->d_mcrc.as<uint32_t>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(crc, c->d_mcrc.p, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return (int)GZGPU_OK;
+    });
+}
+
+int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t start_bit, uint64_t stop_bit,
+                        const uint8_t *window, void *d_out, size_t out_cap_elems, gzgpu_piece_result *res) {
+    if (!c || !d_in || !d_out || !res) return GZGPU_INVALID_ARGUMENT;
+    return guarded(c, [&]() {
+        HIPCHK(hipSetDevice(c->device));
+        PieceDesc pd{};
+        pd.start_bit = start_bit;
+        pd.stop_bit = stop_bit;
+        pd.limit_bit = (uint64_t)in_len * 8;
+        pd.member = 0;
+        DevBuf slab;
+        if (window) {
+            // clean decode against a known window: place it in front of the
+            // output in a scratch buffer, decode as a marker-free DIRECT piece
+            slab.ensure(WIN + out_cap_elems + 64);
+            HIPCHK(hipMemcpyAsync(slab.p, window, WIN, hipMemcpyHostToDevice, c->stream));
+            pd.dst = slab.as<uint8_t>() + WIN;
+            pd.cap = out_cap_elems;
+            pd.flags = PF_WINDOW;
+        } else {
+            uint64_t cap = marker_slab_bytes(out_cap_elems);
+            slab.ensure(cap);
+            pd.dst = slab.as<uint8_t>();
+            pd.cap = cap;
+            pd.flags = PF_MARKER;
+        }
+        uint32_t order = 0;
+        upload(c, c->d_pieces, &pd, sizeof pd);
+        upload(c, c->d_order, &order, 4);
+        c->d_pout.ensure(sizeof(PieceOut));
+        c->d_counters.ensure(64);
+        HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
+        hipLaunchKernelGGL(k_inflate, dim3(1), dim3(64), 0, c->stream, (const uint8_t *)d_in, (uint64_t)in_len,
+                           c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u, c->d_counters.as<uint32_t>(),
+                           c->d_pout.as<PieceOut>());
+        HIPCHK(hipGetLastError());
+        PieceOut po;
+        HIPCHK(hipMemcpyAsync(&po, c->d_pout.p, sizeof po, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        res->end_bit = po.end_bit;
+        res->bfinal_hit = po.status == PS_FINAL;
+        res->status = po.status == PS_FINAL || po.status == PS_STOPPED ? GZGPU_OK
+                      : po.status == PS_OVERFLOW                      ? GZGPU_OUTPUT_TOO_SMALL
+                                                                      : GZGPU_DECODE_FAILED;
+        uint64_t n = po.n16 + po.n8;
+        res->out_len = n;
+        if (window) {
+            HIPCHK(hipMemcpyAsync(d_out, pd.dst, std::min<uint64_t>(n, out_cap_elems), hipMemcpyDeviceToDevice,
+                                  c->stream));
+        } else {
+            // u16 view of every entry: marker part as is, clean part widened
+            if (n > out_cap_elems) n = out_cap_elems;
+            uint64_t a = std::min<uint64_t>(po.n16, n);
+            HIPCHK(hipMemcpyAsync(d_out, pd.dst + 2 * WIN, a * 2, hipMemcpyDeviceToDevice, c->stream));
+            if (n > a) {
+                std::vector<uint8_t> tmp(n - a);
+                HIPCHK(hipMemcpyAsync(tmp.data(), pd.dst + po.u8_off + WIN, n - a, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                std::vector<uint16_t> w(tmp.begin(), tmp.end());
+                HIPCHK(hipMemcpyAsync((uint16_t *)d_out + a, w.data(), w.size() * 2, hipMemcpyHostToDevice, c->stream));
+            }
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        slab.release();
+        return (int)GZGPU_OK;
+    });
+}
+
+}  // extern "C"

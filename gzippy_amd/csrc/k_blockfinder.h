@@ -1,0 +1,231 @@
+// k_blockfinder.h — bit-granular DEFLATE block finder.
+//
+// Candidate predicate (blockfinder_validation.rs), evaluated independently at
+// every bit offset, one lane per offset:
+//   dynamic (BTYPE=10): is_deflate_candidate_n on 13 bits (:68-98: BFINAL=0,
+//     BTYPE=10, HLIT<=29, HDIST<=29); the bit offset + 17 must lie inside the
+//     data (scalar scan break, :693-696); precode leaf count 128, or 64 with
+//     no length > 1 (validate_precode :190 + parse_precode/
+//     is_valid_huffman_lengths :772-834); then the full code-length decode of
+//     validate_huffman_codes (:836-944): repeat rules, is_eof before each
+//     read, EOB length > 0, litlen and distance lengths Kraft-complete (or a
+//     single 1-bit code).
+//   stored (BTYPE=00): find_uncompressed_blocks (:370-470): LEN == ~NLEN at a
+//     byte boundary, 3 zero header bits before it, earliest start =
+//     byte_bit - trailing zero bits (<= 10), seek = byte_bit - 3, LEN > 0 and
+//     the payload fits.
+// Stage A tests the cheap header/precode conditions and compacts survivors
+// (wave ballot + one atomic per wave); stage B runs the code-length decode
+// densely over the survivors, one lane each.
+#pragma once
+#include "gz_common.h"
+#include "gzgpu.h"
+
+namespace gz {
+
+// inverse of the precode transmission order: position of symbol s
+__constant__ uint8_t c_pre_pos[19] = {3, 17, 15, 13, 11, 9, 7, 5, 4, 6, 8, 10, 12, 14, 16, 18, 0, 1, 2};
+
+__device__ __forceinline__ void wave_append(bool ok, uint32_t *count, uint32_t &slot) {
+    uint64_t m = ballot(ok);
+    uint32_t base = 0;
+    if (m) {
+        if (lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1)) base = atomicAdd(count, (uint32_t)__popcll(m));
+        base = __shfl(base, __ffsll((unsigned long long)m) - 1);
+    }
+    slot = base + mbcnt(m);
+}
+
+__global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ in, uint64_t n, uint64_t start_bit,
+                                                    uint64_t end_bit, uint64_t *__restrict__ surv, uint32_t surv_cap,
+                                                    uint32_t *surv_count, gzgpu_boundary *__restrict__ hits,
+                                                    uint32_t hit_cap, uint32_t *hit_count) {
+    const uint64_t total = n * 8;
+    const uint64_t sb = ((start_bit + 3 + 7) / 8) > 1 ? (start_bit + 3 + 7) / 8 : 1;
+    uint64_t until = (end_bit + 7) / 8;
+    if (n < 4) until = 0;
+    else if (until > n - 4) until = n - 4;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // iterate in wave-aligned steps so every lane reaches the ballots
+    for (uint64_t base = start_bit + (uint64_t)blockIdx.x * blockDim.x; base < end_bit; base += stride) {
+        const uint64_t o = base + threadIdx.x;
+        bool in_range = o < end_bit;
+        // ---- dynamic stage A
+        bool dyn = false;
+        if (in_range && o + 17 < total) {
+            uint64_t h = peek64(in, n, o);
+            uint32_t hclen = (uint32_t)(h >> 13) & 15;
+            dyn = (h & 7) == 4 /* BFINAL=0, BTYPE=10 */ && ((h >> 3) & 31) <= 29 && ((h >> 8) & 31) <= 29;
+            if (dyn) {
+                uint64_t p = peek64(in, n, o + 17);
+                uint32_t leaves = 0, gt1 = 0;
+#pragma unroll
+                for (uint32_t i = 0; i < 19; i++) {
+                    uint32_t l = (uint32_t)(p >> (3 * i)) & 7;
+                    if (i >= hclen + 4) l = 0;
+                    leaves += l ? (1u << (7 - l)) : 0u;
+                    gt1 += l > 1;
+                }
+                dyn = leaves == 128 || (leaves == 64 && gt1 == 0);
+            }
+        }
+        uint32_t slot;
+        wave_append(dyn, surv_count, slot);
+        if (dyn && slot < surv_cap) surv[slot] = o;
+        // ---- stored
+        bool st = false;
+        uint64_t earliest = 0, latest = 0;
+        if (in_range && (o & 7) == 0) {
+            uint64_t by = o >> 3;
+            if (by >= sb && by < until && o >= 10) {
+                uint32_t size = (uint32_t)in[by] | (uint32_t)in[by + 1] << 8 | (uint32_t)in[by + 2] << 16 |
+                                (uint32_t)in[by + 3] << 24;
+                if (((size ^ (size >> 16)) & 0xFFFF) == 0xFFFF) {
+                    uint64_t first = o - 10;
+                    uint32_t raw = (uint32_t)in[first >> 3] | (uint32_t)in[(first >> 3) + 1] << 8;
+                    uint32_t pre = (raw >> (first & 7)) & 0x3FF;
+                    if ((pre & (7u << 7)) == 0) {
+                        uint32_t tz = 3;
+                        for (uint32_t j = 4; j <= 10; j++) {
+                            if (pre & (1u << (10 - j))) break;
+                            tz = j;
+                        }
+                        earliest = o - tz;
+                        latest = o - 3;
+                        uint32_t len = size & 0xFFFF;
+                        st = earliest >= start_bit && latest < end_bit && len > 0 && by + 4 + len <= n;
+                    }
+                }
+            }
+        }
+        wave_append(st, hit_count, slot);
+        if (st && slot < hit_cap) {
+            hits[slot].bit_offset = earliest;
+            hits[slot].seek_bit = latest;
+            hits[slot].kind = 0;
+            hits[slot].info = 0;
+        }
+    }
+}
+
+// Full dynamic-header validation of one survivor (one lane).
+__device__ bool validate_dynamic(const uint8_t *__restrict__ in, uint64_t n, uint64_t o, uint8_t *lut,
+                                 uint32_t &info) {
+    const uint64_t total = n * 8;
+    uint64_t h = peek64(in, n, o);
+    uint32_t hlit = (uint32_t)(h >> 3) & 31, hdist = (uint32_t)(h >> 8) & 31, hclen = (uint32_t)(h >> 13) & 15;
+    info = hlit | hdist << 8 | hclen << 16;
+    uint64_t p = peek64(in, n, o + 17);
+    uint32_t pc = hclen + 4;
+    // canonical precode: counts, then codes assigned by symbol order
+    uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (uint32_t i = 0; i < 19; i++) {
+        uint32_t l = i < pc ? (uint32_t)(p >> (3 * i)) & 7 : 0;
+#pragma unroll
+        for (uint32_t L = 1; L < 8; L++) cnt[L] += l == L;
+    }
+    uint32_t next[8];
+    uint32_t code = 0;
+    next[0] = 0;
+#pragma unroll
+    for (uint32_t L = 1; L < 8; L++) {
+        code = (code + (L > 1 ? cnt[L - 1] : 0)) << 1;
+        next[L] = code;
+    }
+    uint32_t *lw = reinterpret_cast<uint32_t *>(lut);
+#pragma unroll
+    for (int i = 0; i < 32; i++) lw[i] = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < 19; s++) {
+        uint32_t ps = c_pre_pos[s];
+        uint32_t l = ps < pc ? (uint32_t)(p >> (3 * ps)) & 7 : 0;
+        if (l) {
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t L = 1; L < 8; L++)
+                if (l == L) c = next[L]++;
+            uint32_t rev = __brev(c) >> (32 - l);
+            for (uint32_t f = rev; f < 128; f += 1u << l) lut[f] = (uint8_t)(s | l << 5);
+        }
+    }
+    uint64_t pos = o + 17 + 3 * pc;
+    const uint32_t hl = hlit + 257, total_codes = hl + hdist + 1;
+    uint32_t i = 0, prev = 0, len256 = 0;
+    uint32_t sum_ll = 0, sum_d = 0, gt1_ll = 0, gt1_d = 0;
+    while (i < total_codes) {
+        if (pos >= total) return false;  // is_eof
+        uint64_t w = peek64(in, n, pos);
+        uint32_t e = lut[w & 127];
+        uint32_t L = e >> 5, sym = e & 31;
+        if (L == 0) return false;
+        pos += L;
+        w >>= L;
+        uint32_t val, rep;
+        if (sym < 16) {
+            val = sym;
+            rep = 1;
+        } else if (sym == 16) {
+            if (i == 0) return false;
+            val = prev;
+            rep = 3 + (uint32_t)(w & 3);
+            pos += 2;
+        } else if (sym == 17) {
+            val = 0;
+            rep = 3 + (uint32_t)(w & 7);
+            pos += 3;
+        } else {
+            val = 0;
+            rep = 11 + (uint32_t)(w & 127);
+            pos += 7;
+        }
+        if (i + rep > total_codes) return false;
+        if (i <= 256 && 256 < i + rep) len256 = val;
+        uint32_t n_ll = i < hl ? (hl - i < rep ? hl - i : rep) : 0;
+        uint32_t n_d = rep - n_ll;
+        if (val) {
+            uint32_t c = 1u << (15 - val);
+            sum_ll += n_ll * c;
+            sum_d += n_d * c;
+            if (val > 1) {
+                gt1_ll += n_ll;
+                gt1_d += n_d;
+            }
+        }
+        prev = val;
+        i += rep;
+        if (sum_ll > 32768 || sum_d > 32768) return false;  // over-subscribed: can only grow
+    }
+    if (len256 == 0) return false;
+    bool ok_ll = sum_ll == 32768 || (sum_ll == 16384 && gt1_ll == 0);
+    bool ok_d = sum_d == 32768 || (sum_d == 16384 && gt1_d == 0);
+    return ok_ll && ok_d;
+}
+
+__global__ void __launch_bounds__(256) k_bf_stage_b(const uint8_t *__restrict__ in, uint64_t n,
+                                                    const uint64_t *__restrict__ surv, uint32_t nsurv,
+                                                    gzgpu_boundary *__restrict__ hits, uint32_t hit_cap,
+                                                    uint32_t *hit_count) {
+    __shared__ __align__(16) uint8_t luts[256][128];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nsurv; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool ok = false;
+        uint64_t o = 0;
+        uint32_t info = 0;
+        if (i < nsurv) {
+            o = surv[i];
+            ok = validate_dynamic(in, n, o, luts[threadIdx.x], info);
+        }
+        uint32_t slot;
+        wave_append(ok, hit_count, slot);
+        if (ok && slot < hit_cap) {
+            hits[slot].bit_offset = o;
+            hits[slot].seek_bit = o;
+            hits[slot].kind = 2;
+            hits[slot].info = info;
+        }
+    }
+}
+
+}  // namespace gz

@@ -1,0 +1,128 @@
+// k_crc.h — gzip CRC32 (reflected 0xEDB88320) on the device.
+//
+// Stage 1 (k_crc_segments): every workgroup folds one segment of <= 64 KiB;
+// each of its 256 lanes runs a slicing-by-8 table CRC over 256 contiguous
+// bytes from a zero register, then the lanes are combined in a tree with the
+// GF(2) shift crc(A||B) = crc(A) * x^(8|B|) mod P ^ crc(B) (combine_crc32,
+// crc32.rs:127-133; polynomial_multiply_modulo :64; x_power_modulo :104).
+// Stage 2 (k_crc_members): per member, fold its segment CRCs the same way
+// and apply the init/final inversion: crc = ~(x^(8D) * 0xFFFFFFFF ^ raw).
+#pragma once
+#include "gz_common.h"
+
+namespace gz {
+
+constexpr uint32_t CRC_POLY = 0xEDB88320u;
+constexpr uint32_t CRC_SEG = 65536;
+
+__device__ __forceinline__ uint32_t gf2_mulmod(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 31; i >= 0; i--) {
+        r ^= b & (0u - ((a >> i) & 1u));
+        b = (b >> 1) ^ (CRC_POLY & (0u - (b & 1u)));
+    }
+    return r;
+}
+
+// c_xbyte[k][j] = x^(8 * j * 16^k) mod P (reflected), filled by the host:
+// shifting over n zero bytes costs one multiply per non-zero hex digit of n.
+__constant__ uint32_t c_xbyte[16][16];
+
+// x^(8n) mod P
+__device__ __forceinline__ uint32_t xpow_bytes(uint64_t n) {
+    uint32_t p = 1u << 31;
+    for (int k = 0; n; k++, n >>= 4)
+        if (n & 15) p = gf2_mulmod(c_xbyte[k][n & 15], p);
+    return p;
+}
+
+// shift a raw CRC register over `len` zero bytes
+__device__ __forceinline__ uint32_t crc_shift(uint32_t c, uint64_t len) { return len ? gf2_mulmod(xpow_bytes(len), c) : c; }
+
+__device__ __forceinline__ uint32_t crc_byte(const uint32_t *t, uint32_t c, uint8_t b) { return (c >> 8) ^ t[(c ^ b) & 0xFF]; }
+
+// Segments: seg_off[i] bytes into `base`, seg_len[i] <= CRC_SEG.
+__global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict__ base, const uint64_t *__restrict__ seg_off,
+                                                      const uint32_t *__restrict__ seg_len, uint32_t nseg,
+                                                      const uint32_t *__restrict__ tables, uint32_t *__restrict__ seg_crc) {
+    __shared__ uint32_t t[8][256];
+    __shared__ uint32_t part[256];
+    for (uint32_t i = threadIdx.x; i < 8 * 256; i += blockDim.x) t[i >> 8][i & 255] = tables[i];
+    __syncthreads();
+    for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const uint8_t *p = base + seg_off[s];
+        const uint32_t len = seg_len[s];
+        const uint32_t b0 = threadIdx.x * 256;
+        uint32_t mine = b0 < len ? (len - b0 < 256 ? len - b0 : 256) : 0;
+        const uint8_t *q = p + b0;
+        uint32_t c = 0, k = 0;
+        if ((((uintptr_t)q) & 3) == 0) {
+            const uint32_t *w = reinterpret_cast<const uint32_t *>(q);
+            for (; k + 8 <= mine; k += 8) {
+                uint32_t lo = c ^ w[k / 4], hi = w[k / 4 + 1];
+                c = t[7][lo & 0xFF] ^ t[6][(lo >> 8) & 0xFF] ^ t[5][(lo >> 16) & 0xFF] ^ t[4][lo >> 24] ^
+                    t[3][hi & 0xFF] ^ t[2][(hi >> 8) & 0xFF] ^ t[1][(hi >> 16) & 0xFF] ^ t[0][hi >> 24];
+            }
+        }
+        for (; k < mine; k++) c = crc_byte(t[0], c, q[k]);
+        part[threadIdx.x] = c;
+        __syncthreads();
+        // tree: node t covers [t*256, ...); right neighbour at stride `w`
+        for (uint32_t w = 1; w < 256; w <<= 1) {
+            uint32_t i = threadIdx.x;
+            if ((i & (2 * w - 1)) == 0 && i + w < 256) {
+                uint64_t rs = (uint64_t)(i + w) * 256;
+                uint64_t re = rs + (uint64_t)w * 256;
+                if (re > len) re = len;
+                uint64_t rlen = re > rs ? re - rs : 0;
+                if (rlen) part[i] = crc_shift(part[i], rlen) ^ part[i + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) seg_crc[s] = part[0];
+        __syncthreads();
+    }
+}
+
+// Per member m: segments [mseg[m], mseg[m+1]) of lengths seg_len; total
+// bytes mlen[m].  One workgroup (256 lanes) per member.
+__global__ void __launch_bounds__(256) k_crc_members(const uint32_t *__restrict__ seg_crc,
+                                                     const uint32_t *__restrict__ seg_len,
+                                                     const uint32_t *__restrict__ mseg, const uint64_t *__restrict__ mlen,
+                                                     uint32_t nmem, uint32_t *__restrict__ crc_out) {
+    __shared__ uint32_t part[256];
+    __shared__ uint64_t plen[256];
+    for (uint32_t m = blockIdx.x; m < nmem; m += gridDim.x) {
+        const uint32_t s0 = mseg[m], s1 = mseg[m + 1];
+        const uint32_t ns = s1 - s0;
+        const uint32_t per = (ns + 255) / 256;
+        uint32_t a = s0 + threadIdx.x * per, b = a + per;
+        if (a > s1) a = s1;
+        if (b > s1) b = s1;
+        uint32_t c = 0;
+        uint64_t l = 0;
+        for (uint32_t s = a; s < b; s++) {
+            c = crc_shift(c, seg_len[s]) ^ seg_crc[s];
+            l += seg_len[s];
+        }
+        part[threadIdx.x] = c;
+        plen[threadIdx.x] = l;
+        __syncthreads();
+        for (uint32_t w = 1; w < 256; w <<= 1) {
+            uint32_t i = threadIdx.x;
+            if ((i & (2 * w - 1)) == 0 && i + w < 256) {
+                uint64_t rl = plen[i + w];
+                if (rl) {
+                    part[i] = crc_shift(part[i], rl) ^ part[i + w];
+                    plen[i] += rl;
+                }
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) crc_out[m] = ~(crc_shift(0xFFFFFFFFu, mlen[m]) ^ part[0]);
+        __syncthreads();
+    }
+}
+
+}  // namespace gz

@@ -1,0 +1,146 @@
+"""Seeded synthetic gzip workloads.
+
+Content comes from ``csrc/synth.c`` (four kinds: text, random, runs, u32
+deltas); members are encoded with libdeflate (``libdeflate.so.0``) when it is
+present, else zlib.  gzippy's own Rust encoder is not available here, so the
+encoder used is reported with every workload; decoded bytes, CRC32 and ISIZE
+do not depend on the encoder.
+"""
+import ctypes
+import ctypes.util
+import os
+import struct
+import zlib
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import build as _build
+
+KINDS = {"text": 0, "random": 1, "runs": 2, "u32": 3}
+SEG = 64 << 20  # mixed-entropy segment size
+
+_synth = None
+
+
+def _synth_lib():
+    global _synth
+    if _synth is None:
+        path = _build.synth_path()
+        if not os.path.exists(path):
+            _build.build_synth()
+        L = ctypes.CDLL(path)
+        L.gzs_generate.restype = ctypes.c_int
+        L.gzs_generate.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
+        _synth = L
+    return _synth
+
+
+def generate(kind, n, seed):
+    """n bytes of `kind` ("text" | "random" | "runs" | "u32") from `seed`."""
+    out = np.empty(int(n), dtype=np.uint8)
+    if n:
+        st = _synth_lib().gzs_generate(KINDS[kind], out.ctypes.data, int(n), int(seed))
+        if st:
+            raise ValueError(kind)
+    return out
+
+
+def generate_mixed(n, seed, seg=SEG, threads=8):
+    """Cycle `seg`-byte segments of text / random / runs / u32 (one seed each)."""
+    out = np.empty(int(n), dtype=np.uint8)
+    order = ["text", "random", "runs", "u32"]
+    jobs = []
+    for i, off in enumerate(range(0, int(n), seg)):
+        jobs.append((order[i % 4], off, min(seg, int(n) - off), seed * 1000003 + i))
+
+    def one(job):
+        kind, off, ln, s = job
+        view = out[off:off + ln]
+        _synth_lib().gzs_generate(KINDS[kind], view.ctypes.data, ln, s)
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(one, jobs))
+    return out
+
+
+# ---------------------------------------------------------------- encoders
+
+_ld = None
+
+
+def _libdeflate():
+    global _ld
+    if _ld is None:
+        try:
+            L = ctypes.CDLL("libdeflate.so.0")
+        except OSError:
+            _ld = False
+            return _ld
+        L.libdeflate_alloc_compressor.restype = ctypes.c_void_p
+        L.libdeflate_alloc_compressor.argtypes = [ctypes.c_int]
+        L.libdeflate_free_compressor.argtypes = [ctypes.c_void_p]
+        L.libdeflate_gzip_compress.restype = ctypes.c_size_t
+        L.libdeflate_gzip_compress.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                               ctypes.c_void_p, ctypes.c_size_t]
+        L.libdeflate_gzip_compress_bound.restype = ctypes.c_size_t
+        L.libdeflate_gzip_compress_bound.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        _ld = L
+    return _ld
+
+
+def encoder_name():
+    return "libdeflate" if _libdeflate() else "zlib-" + zlib.ZLIB_RUNTIME_VERSION
+
+
+def gzip_compress(data, level=6, encoder=None):
+    """One RFC 1952 member of `data` at `level` (libdeflate, else zlib)."""
+    a = np.ascontiguousarray(np.frombuffer(data, dtype=np.uint8) if not isinstance(data, np.ndarray) else data)
+    L = _libdeflate() if encoder in (None, "libdeflate") else False
+    if L:
+        c = L.libdeflate_alloc_compressor(int(level))
+        try:
+            bound = L.libdeflate_gzip_compress_bound(c, a.size)
+            out = np.empty(bound, dtype=np.uint8)
+            n = L.libdeflate_gzip_compress(c, a.ctypes.data, a.size, out.ctypes.data, bound)
+            if n == 0:
+                raise RuntimeError("libdeflate_gzip_compress failed")
+            return out[:n].copy()
+        finally:
+            L.libdeflate_free_compressor(c)
+    co = zlib.compressobj(int(level), zlib.DEFLATED, 31)
+    body = co.compress(memoryview(a)) + co.flush()
+    return np.frombuffer(body, dtype=np.uint8).copy()
+
+
+def trailer(gz):
+    """(crc32, isize) from a member's 8-byte trailer."""
+    crc, isize = struct.unpack("<II", bytes(gz[-8:]))
+    return crc, isize
+
+
+# ---------------------------------------------------------------- configs
+
+def text_member(n, seed, level=6):
+    """Config-0/1 shape: `n` bytes of the text generator, one member at `level`."""
+    raw = generate("text", n, seed)
+    return raw, gzip_compress(raw, level)
+
+
+def batch_members(count, seed, min_size=1 << 10, max_size=64 << 20, median=64 << 10, sigma=1.5,
+                  levels=(1, 6, 9), threads=8):
+    """Config-3 shape: `count` independent members with lognormal sizes, content
+    uniform over the four kinds and level uniform over `levels`.
+    Returns list of (raw, gz, kind, level)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.exp(rng.normal(np.log(median), sigma, count)), min_size, max_size).astype(np.int64)
+    kinds = rng.integers(0, 4, count)
+    lv = rng.choice(np.array(levels), count)
+    names = list(KINDS)
+
+    def one(i):
+        raw = generate(names[kinds[i]], int(sizes[i]), seed * 7919 + i)
+        return raw, gzip_compress(raw, int(lv[i])), names[kinds[i]], int(lv[i])
+
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(one, range(count)))

@@ -1,0 +1,155 @@
+/*
+ * gzgpu.h — C ABI of the MI355X (gfx950) parallel gzip decompressor.
+ *
+ * The library replaces gzippy's parallel single-member decode path behind
+ * gzippy::decompress / decompress_to_writer.  Every entry point takes plain
+ * pointers and sizes; none retains a caller buffer past the call.
+ *
+ * Reference interfaces replaced (paths into the gzippy tree):
+ *   gzgpu_decompress            src/lib.rs:176 decompress / :189 decompress_with_threads
+ *                               → src/decompress/mod.rs:317 decompress_bytes
+ *   gzgpu_decompress_to_writer  src/lib.rs:204 decompress_to_writer / :221 ..._with_threads
+ *   gzgpu_decompress_device     src/decompress/parallel/single_member.rs:987
+ *                               decompress_parallel (the ParallelSM arm,
+ *                               src/decompress/mod.rs:629-650), device-resident
+ *   gzgpu_decompress_batch      src/decompress/bgzf.rs:2370
+ *                               decompress_multi_member_parallel_to_vec (member-per-worker)
+ *   gzgpu_find_blocks           src/decompress/parallel/blockfinder_validation.rs:520 find_blocks
+ *   gzgpu_crc32                 src/decompress/parallel/crc32.rs:155 crc32_fold
+ *   gzgpu_status_string         src/decompress/parallel/single_member.rs:1415-1437 Display
+ *                               for ParallelError
+ */
+#ifndef GZGPU_H
+#define GZGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1..6 map 1:1 onto ParallelError
+ * (src/decompress/parallel/single_member.rs:1384-1413); the rest are the
+ * device-side conditions a CPU build cannot hit. */
+typedef enum {
+    GZGPU_OK = 0,
+    GZGPU_INVALID_HEADER = 1,      /* ParallelError::InvalidHeader */
+    GZGPU_INVALID_FORMAT = 2,      /* ParallelError::InvalidGzipFormat */
+    GZGPU_DECODE_FAILED = 3,       /* ParallelError::DecodeFailed */
+    GZGPU_SIZE_MISMATCH = 4,       /* ParallelError::SizeMismatch */
+    GZGPU_CRC_MISMATCH = 5,        /* ParallelError::CrcMismatch */
+    GZGPU_IO = 6,                  /* ParallelError::Io (writer callback failed) */
+    GZGPU_GPU_UNAVAILABLE = 7,     /* no gfx950 device / HIP runtime error */
+    GZGPU_OUT_OF_DEVICE_MEMORY = 8,
+    GZGPU_OUTPUT_TOO_SMALL = 9,    /* out_cap < decoded size; *out_len holds the size needed */
+    GZGPU_INVALID_ARGUMENT = 10,
+} gzgpu_status;
+
+typedef struct gzgpu_ctx gzgpu_ctx;
+
+/* Tunables (all 0 = library default / autotuned). */
+typedef struct {
+    uint64_t chunk_bytes;       /* speculation grid, compressed bytes per chunk (default 512 KiB) */
+    uint32_t pieces_per_chunk;  /* wavefronts per chunk: sub-splits at found block boundaries */
+    uint32_t waves_per_cu;      /* persistent-grid occupancy target */
+    uint64_t batch_split_bytes; /* batch: members above this compressed size are split speculatively */
+} gzgpu_tuning;
+
+/* Per-call timing and shape, filled when a non-NULL pointer is passed. */
+typedef struct {
+    double ms_total;            /* device time of the whole call (HIP events) */
+    double ms_find;             /* block finder */
+    double ms_inflate;          /* speculative inflate (all launches) */
+    double ms_resolve;          /* window chain + marker resolution */
+    double ms_crc;              /* CRC32 fold + combine */
+    uint64_t compressed_bytes;
+    uint64_t decompressed_bytes;
+    uint32_t pieces;            /* speculative pieces decoded */
+    uint32_t redecoded;         /* pieces re-decoded after a speculation miss or slab overflow */
+    uint32_t candidates;        /* block-finder hits */
+    uint32_t launches;          /* kernel launches */
+    double ms_inflate_kernel;   /* summed k_inflate durations (HIP events around each launch) */
+    uint32_t inflate_launches;  /* k_inflate launches (1 + speculation re-decode rounds) */
+    uint32_t pad_;
+} gzgpu_stats;
+
+/* ---- context ---------------------------------------------------------- */
+/* One context per calling thread (or lock around it).  Holds the HIP stream,
+ * grown device buffers and tuning. */
+int gzgpu_ctx_create(int device, gzgpu_ctx **out);
+void gzgpu_ctx_destroy(gzgpu_ctx *ctx);
+int gzgpu_set_tuning(gzgpu_ctx *ctx, const gzgpu_tuning *t);
+int gzgpu_get_tuning(gzgpu_ctx *ctx, gzgpu_tuning *t);
+const char *gzgpu_last_error(gzgpu_ctx *ctx);
+const char *gzgpu_status_string(int status);
+
+/* ---- whole-stream decode (host buffers) -------------------------------- */
+/* Non-gzip input (no 1f 8b magic) returns GZGPU_OK with *out_len = 0
+ * (src/lib.rs:50-52, src/decompress/mod.rs:337-339).  Multi-member input is
+ * the concatenation of member outputs; bytes after a valid member that do not
+ * start a new member end the walk cleanly (src/decompress/mod.rs:736-741). */
+int gzgpu_decompress(gzgpu_ctx *ctx, const uint8_t *in, size_t in_len, uint8_t *out, size_t out_cap,
+                     size_t *out_len, gzgpu_stats *stats);
+
+/* Streaming sink: `write` is called in order with consecutive output pieces;
+ * a non-zero return aborts with GZGPU_IO.  Partial output may have been
+ * delivered before a CRC/ISIZE error (single_member.rs:17-22). */
+typedef int (*gzgpu_write_fn)(void *user, const uint8_t *data, size_t len);
+int gzgpu_decompress_to_writer(gzgpu_ctx *ctx, const uint8_t *in, size_t in_len, gzgpu_write_fn write,
+                               void *user, uint64_t *total, gzgpu_stats *stats);
+
+/* ---- device-resident single member -------------------------------------- */
+/* `d_in` and `d_out` are device pointers (d_in readable for in_len bytes).
+ * Decodes one gzip member into d_out; verifies ISIZE and CRC32 against the
+ * trailer.  `*out_len` receives the decoded size (also on
+ * GZGPU_OUTPUT_TOO_SMALL). */
+int gzgpu_decompress_device(gzgpu_ctx *ctx, const void *d_in, size_t in_len, void *d_out, size_t out_cap,
+                            size_t *out_len, uint32_t *crc_out, gzgpu_stats *stats);
+
+/* ---- batched members (one persistent launch) ---------------------------- */
+/* n independent gzip members, member i at d_in + in_off[i] (in_len[i]
+ * bytes); output of member i goes to d_out + out_off[i] with room for
+ * out_cap[i] bytes.  status[i], out_len[i] and crc[i] are filled per
+ * member; the return value is GZGPU_OK when every member decoded and
+ * verified, else the first failing member's status. */
+int gzgpu_decompress_batch(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_off, const uint64_t *in_len,
+                           void *d_out, const uint64_t *out_off, const uint64_t *out_cap, size_t n,
+                           int32_t *status, uint64_t *out_len, uint32_t *crc, gzgpu_stats *stats);
+
+/* ---- exposed kernels (parity tests, tools) ------------------------------- */
+typedef struct {
+    uint64_t bit_offset; /* earliest valid start */
+    uint64_t seek_bit;   /* where decoding starts */
+    uint32_t kind;       /* 0 stored, 2 dynamic */
+    uint32_t info;       /* dynamic: hlit | hdist << 8 | hclen << 16 */
+} gzgpu_boundary;
+
+/* Every block-finder hit in bit range [start_bit, end_bit) of the device
+ * buffer d_in[0..in_len), sorted by (bit_offset, seek_bit).  Returns the
+ * hit count in *count even when it exceeds cap (then only cap are written). */
+int gzgpu_find_blocks(gzgpu_ctx *ctx, const void *d_in, size_t in_len, uint64_t start_bit, uint64_t end_bit,
+                      gzgpu_boundary *out, size_t cap, size_t *count);
+
+/* gzip CRC32 of a device buffer (finalized value, like crc32fast / zlib). */
+int gzgpu_crc32(gzgpu_ctx *ctx, const void *d_data, size_t len, uint32_t *crc);
+
+/* Raw-DEFLATE decode of bit range [start_bit, ...) with the stop rule of
+ * chunk_decode.rs:2029-2040 (stop before a non-final block whose header bit is
+ * >= stop_bit).  window == NULL: marker mode, output u16 values
+ * (< 256 literal, >= 32768 window marker) into d_out16; window != NULL (host,
+ * 32768 bytes): clean decode into d_out8.  One wavefront, one piece: the
+ * building block of the parallel path, exposed for parity tests. */
+typedef struct {
+    uint64_t end_bit;
+    uint64_t out_len;
+    int32_t status;      /* gzgpu_status */
+    int32_t bfinal_hit;
+} gzgpu_piece_result;
+int gzgpu_inflate_piece(gzgpu_ctx *ctx, const void *d_in, size_t in_len, uint64_t start_bit, uint64_t stop_bit,
+                        const uint8_t *window, void *d_out, size_t out_cap_elems, gzgpu_piece_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

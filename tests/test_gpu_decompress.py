@@ -1,0 +1,145 @@
+"""End-to-end GPU decode through the C ABI: byte-exact output, CRC32 and
+ISIZE, error behaviour, multi-member, batch == per-member."""
+import io
+import zlib
+
+import numpy as np
+import pytest
+
+from gzippy_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def dev(arr):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(arr)).cuda()
+
+
+@pytest.fixture
+def fine_grid(gpu):
+    """Small chunks so even MiB-sized members split into many speculative pieces."""
+    ctx = gpu.context()
+    old = ctx.tuning()
+    ctx.set_tuning(chunk_bytes=64 << 10, pieces_per_chunk=4)
+    yield ctx
+    ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
+
+
+def test_config0_text_16mib(gpu):
+    raw, gz = W.text_member(16 << 20, seed=0x675A0000)
+    say("config0", raw.size, gz.size)
+    st = {}
+    out = gpu.decompress(gz, stats=st)
+    assert out == raw.tobytes()
+    say("stats", st)
+
+
+@pytest.mark.parametrize("kind", ["text", "random", "runs", "u32"])
+@pytest.mark.parametrize("level", [1, 6, 9])
+def test_kinds_levels_split(gpu, fine_grid, kind, level):
+    say("split", kind, level)
+    raw = W.generate(kind, (3 << 20) + 1234, seed=7 * level + len(kind))
+    for enc in ("libdeflate", "zlib"):
+        gz = W.gzip_compress(raw, level, encoder=enc)
+        st = {}
+        out = gpu.decompress(gz, stats=st)
+        assert out == raw.tobytes(), (kind, level, enc, st)
+        if kind == "text":
+            assert st["pieces"] > 4
+
+
+def test_mixed_segments(gpu, fine_grid):
+    raw = W.generate_mixed(12 << 20, seed=3, seg=1 << 20)
+    for level in (1, 9):
+        gz = W.gzip_compress(raw, level)
+        say("mixed", level, gz.size)
+        assert gpu.decompress(gz) == raw.tobytes()
+
+
+def test_device_api(gpu):
+    import torch
+    raw, gz = W.text_member(8 << 20, seed=9)
+    out = torch.empty(raw.size, dtype=torch.uint8, device="cuda")
+    n, crc, st = gpu.decompress_device(dev(gz), out)
+    assert n == raw.size and crc == zlib.crc32(raw.tobytes())
+    assert np.array_equal(out.cpu().numpy(), raw)
+
+
+def test_writer_api(gpu):
+    raw, gz = W.text_member(2 << 20, seed=10)
+    sink = io.BytesIO()
+    n = gpu.decompress_to_writer(gz, sink)
+    assert n == raw.size and sink.getvalue() == raw.tobytes()
+
+
+def test_small_and_edge_members(gpu):
+    for raw in [b"", b"a", b"ab" * 3, bytes(range(256)) * 300]:
+        for level in (0, 1, 6, 9):
+            gz = zlib.compress(raw, level, wbits=31) if level else zlib.compressobj(0, 8, 31).compress(raw)
+            if not level:
+                c = zlib.compressobj(0, zlib.DEFLATED, 31)
+                gz = c.compress(raw) + c.flush()
+            assert gpu.decompress(gz) == raw
+
+
+def test_non_gzip_returns_empty(gpu):
+    assert gpu.decompress(b"") == b""
+    assert gpu.decompress(b"plain text, not gzip") == b""
+
+
+def test_multi_member_and_trailing_garbage(gpu):
+    a, b = W.generate("text", 300_000, 1), W.generate("u32", 500_001, 2)
+    gz = np.concatenate([W.gzip_compress(a, 6), W.gzip_compress(b, 1)]).tobytes()
+    assert gpu.decompress(gz) == a.tobytes() + b.tobytes()
+    assert gpu.decompress(gz + b"garbage!") == a.tobytes() + b.tobytes()
+
+
+def test_corruption_is_an_error(gpu):
+    raw, gz = W.text_member(1 << 20, seed=11)
+    bad = gz.copy()
+    bad[-6] ^= 0x10
+    with pytest.raises(gpu.GzippyError) as e:
+        gpu.decompress(bad)
+    assert e.value.kind == "CrcMismatch"
+    bad = gz.copy()
+    bad[-3] ^= 0x01
+    with pytest.raises(gpu.GzippyError) as e:
+        gpu.decompress(bad)
+    assert e.value.kind == "SizeMismatch"
+    with pytest.raises(gpu.GzippyError):
+        gpu.decompress(gz[: gz.size // 2])
+    bad = gz.copy()
+    bad[gz.size // 2: gz.size // 2 + 64] ^= 0x5A
+    with pytest.raises(gpu.GzippyError):
+        gpu.decompress(bad)
+
+
+def test_batch_matches_per_member(gpu):
+    import torch
+    mem = W.batch_members(300, seed=12, max_size=4 << 20)
+    ins = [m[1] for m in mem]
+    raws = [m[0] for m in mem]
+    in_off = np.cumsum([0] + [x.size for x in ins[:-1]])
+    in_len = np.array([x.size for x in ins])
+    out_cap = np.array([r.size for r in raws])
+    out_off = np.cumsum([0] + list(out_cap[:-1]))
+    d_in = dev(np.concatenate(ins))
+    d_out = torch.zeros(int(out_cap.sum()) + 1, dtype=torch.uint8, device="cuda")
+    say("batch", len(mem), int(out_cap.sum()))
+    status, olen, crc, st = gpu.decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap)
+    assert (status == 0).all()
+    host = d_out.cpu().numpy()
+    for i, r in enumerate(raws):
+        assert olen[i] == r.size
+        assert np.array_equal(host[out_off[i]: out_off[i] + r.size], r), i
+        assert crc[i] == zlib.crc32(r.tobytes())
+    # each member alone through the plain single-member path
+    for i in range(0, 300, 37):
+        out = torch.zeros(max(raws[i].size, 1), dtype=torch.uint8, device="cuda")
+        n, c, _ = gpu.decompress_device(dev(ins[i]), out)
+        assert n == raws[i].size and np.array_equal(out[:n].cpu().numpy(), raws[i])

@@ -1,0 +1,119 @@
+"""GPU parity: every kernel against the oracle on the same seeded inputs,
+through the C ABI.  Integer/byte work: bit-exact everywhere."""
+import hashlib
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+from gzippy_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "silesia_blocks.npz")
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def dev(arr):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(arr)).cuda()
+
+
+def deflate_raw(data, level=6):
+    c = zlib.compressobj(level, zlib.DEFLATED, -15, 8)
+    return np.frombuffer(c.compress(bytes(data)) + c.flush(), np.uint8)
+
+
+# ---------------------------------------------------------------- CRC32
+
+@pytest.mark.parametrize("n", [0, 1, 255, 256, 257, 4095, 65535, 65536, 65537, (1 << 20) + 13, (16 << 20) + 5])
+def test_crc32_device(gpu, n):
+    say("crc32 n=", n)
+    rng = np.random.default_rng(n)
+    d = rng.integers(0, 256, max(n, 1), dtype=np.uint8)[:n]
+    t = dev(d if n else np.zeros(1, np.uint8))
+    assert gpu.crc32(t, n) == zlib.crc32(d.tobytes())
+
+
+def test_crc32_unaligned(gpu):
+    rng = np.random.default_rng(7)
+    d = rng.integers(0, 256, 300_001, dtype=np.uint8)
+    t = dev(d)
+    for off in (1, 2, 3, 5):
+        sub = t[off:]
+        assert gpu.crc32(sub, d.size - off) == zlib.crc32(d[off:].tobytes())
+
+
+# ---------------------------------------------------------------- block finder
+
+FINDER_CASES = [("text", 6, "libdeflate", 2 << 20), ("text", 1, "zlib", 1 << 20), ("u32", 9, "zlib", 1 << 20),
+                ("random", 6, "libdeflate", 1 << 20), ("runs", 6, "zlib", 1 << 20), ("random", 1, "zlib", 300_000)]
+
+
+@pytest.mark.parametrize("kind,level,enc,size", FINDER_CASES)
+def test_find_blocks_equals_oracle(gpu, oracle, kind, level, enc, size):
+    say("find_blocks", kind, level, enc, size)
+    raw = W.generate(kind, size, seed=101)
+    gz = W.gzip_compress(raw, level, encoder=enc)
+    want = oracle.find_blocks(gz, 0, gz.size * 8)
+    got = gpu.find_blocks(dev(gz), 0, gz.size * 8)
+    assert got == want
+    # a sub-range with unaligned ends
+    lo, hi = 12345, gz.size * 8 - 777
+    assert gpu.find_blocks(dev(gz), lo, hi) == oracle.find_blocks(gz, lo, hi)
+
+
+def test_find_blocks_recall_true_boundaries(gpu, oracle):
+    raw = W.generate("text", 4 << 20, seed=5)
+    gz = W.gzip_compress(raw, 6)
+    st, hl = oracle.parse_header(gz)
+    res, _, blocks = oracle.inflate(gz[hl:-8], 0, blocks_cap=1 << 16)
+    hits = {h[0] for h in gpu.find_blocks(dev(gz))}
+    true_dyn = [hl * 8 + b[0] for b in blocks[1:] if b[2] == 2 and not b[3]]
+    assert len(true_dyn) > 50
+    assert all(b in hits for b in true_dyn)
+
+
+# ---------------------------------------------------------------- one piece
+
+def test_inflate_piece_silesia_vectors(gpu, oracle):
+    import torch
+    z = np.load(GOLD)
+    for i, (idx, btype, bitoff, bitlen, declen, crc) in enumerate(z["meta"]):
+        comp = np.concatenate([z[f"comp{i}"], np.zeros(8, np.uint8)])
+        win = z[f"win{i}"]
+        out = torch.zeros(int(declen) + 4096, dtype=torch.uint8, device="cuda")
+        r = gpu.inflate_piece(dev(comp), int(bitoff), int(bitoff) + 1, out, window=win.tobytes())
+        assert r.status == 0 and r.out_len == declen and r.end_bit - bitoff == bitlen, i
+        got = out[: int(declen)].cpu().numpy()
+        assert hashlib.sha256(got.tobytes()).digest() == z[f"sha{i}"].tobytes(), i
+        # window-absent: u16 markers, resolved on the CPU by the oracle
+        m = torch.zeros(int(declen) + 4096, dtype=torch.int16, device="cuda")
+        r = gpu.inflate_piece(dev(comp), int(bitoff), int(bitoff) + 1, m)
+        assert r.status == 0 and r.out_len == declen, i
+        marks = m[: int(declen)].cpu().numpy().view(np.uint16)
+        st, resolved = oracle.resolve_markers(marks, win)
+        assert st == 0 and np.array_equal(resolved, got), i
+
+
+@pytest.mark.parametrize("kind,level", [("text", 6), ("u32", 1), ("runs", 9), ("random", 6), ("text", 9)])
+def test_inflate_piece_markers_equal_oracle(gpu, oracle, kind, level):
+    import torch
+    say("piece markers", kind, level)
+    raw = W.generate(kind, 3 << 20, seed=33)
+    d = deflate_raw(raw, level)
+    res, _, blocks = oracle.inflate(d, 0, blocks_cap=1 << 16)
+    td = dev(d)
+    for bi in range(1, min(len(blocks), 5)):
+        bit, off = blocks[bi][0], blocks[bi][1]
+        stop = blocks[bi + 2][0] if bi + 2 < len(blocks) else 2**64 - 1
+        ores, omarks, _ = oracle.inflate(d, bit, stop, marker_mode=True)
+        m = torch.zeros(int(ores.out_len) + 8192, dtype=torch.int16, device="cuda")
+        r = gpu.inflate_piece(td, bit, stop, m)
+        assert r.status == 0 and r.out_len == ores.out_len and r.end_bit == ores.end_bit
+        got = m[: r.out_len].cpu().numpy().view(np.uint16)
+        assert np.array_equal(got, omarks), (kind, level, bi)
