@@ -138,10 +138,18 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _sync_torch(t):
+    """The library runs on its own HIP stream: drain torch's current stream so
+    work it queued on these tensors (copies, fills) is complete first."""
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
+
+
 def decompress_device(d_in, d_out, in_len=None, ctx=None):
     """Decode one gzip member from device tensor `d_in` into `d_out`
     (uint8 CUDA tensors).  Returns (out_len, crc32, stats dict)."""
     ctx = ctx or context(d_in.device.index or 0)
+    _sync_torch(d_in)
     n = ctypes.c_size_t()
     crc = ctypes.c_uint32()
     st_obj = Stats()
@@ -158,6 +166,7 @@ def decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None, ra
     Offsets/lengths are host integer sequences.  Returns (status[], out_len[],
     crc[], stats dict) as numpy arrays."""
     ctx = ctx or context(d_in.device.index or 0)
+    _sync_torch(d_in)
     n = len(in_off)
     a = [np.ascontiguousarray(np.asarray(x, dtype=np.uint64)) for x in (in_off, in_len, out_off, out_cap)]
     status = np.zeros(n, dtype=np.int32)
@@ -179,6 +188,7 @@ def find_blocks(d_in, start_bit=0, end_bit=None, in_len=None, ctx=None):
     """Block-finder hits in [start_bit, end_bit) of a device buffer, sorted:
     list of (bit_offset, seek_bit, kind)."""
     ctx = ctx or context(d_in.device.index or 0)
+    _sync_torch(d_in)
     L = d_in.numel() if in_len is None else in_len
     if end_bit is None:
         end_bit = L * 8
@@ -197,6 +207,7 @@ def find_blocks(d_in, start_bit=0, end_bit=None, in_len=None, ctx=None):
 def crc32(d_data, length=None, ctx=None):
     """gzip CRC32 of a device buffer."""
     ctx = ctx or context(d_data.device.index or 0)
+    _sync_torch(d_data)
     c = ctypes.c_uint32()
     L = d_data.numel() * d_data.element_size() if length is None else length
     st = ctx.lib.gzgpu_crc32(ctx.h, _ptr(d_data), L, ctypes.byref(c))
@@ -210,6 +221,7 @@ def inflate_piece(d_in, start_bit, stop_bit, d_out, window=None, in_len=None, ct
     window=None: marker mode, d_out is int16/uint16-sized entries; window =
     32768 host bytes: clean decode into uint8 d_out.  Returns PieceResult."""
     ctx = ctx or context(d_in.device.index or 0)
+    _sync_torch(d_in)
     res = _native.PieceResult()
     L = d_in.numel() if in_len is None else in_len
     w = None

@@ -71,6 +71,11 @@ def load_library():
         path = lib_path()
         if not os.path.exists(path):
             raise RuntimeError(f"gzippy_amd: HIP library {path} is not built (run __graft_entry__.build())")
+        # torch bundles its own libamdhip64.so.7 (same soname as /opt/rocm's).
+        # Load torch first so the process holds exactly one HIP runtime: ours
+        # then binds to it instead of pulling in a second HSA runtime, which
+        # leaves torch without devices.
+        import torch  # noqa: F401
         L = ctypes.CDLL(path)
         vp, sz, u64, u32, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32
         P = ctypes.POINTER

@@ -249,7 +249,6 @@ __device__ uint32_t decode_symbols(InflateShared &sh, BitReader &br, T *out, uin
                                    uint64_t base, uint64_t limit_bit, uint64_t &last_marker_end, uint32_t &err) {
     const uint32_t lane = lane_id();
     for (;;) {
-        if (pos + 258 > cap) return R_OVERFLOW;
         if (br.pos() > limit_bit) {
             err = E_TRUNC;
             return R_ERROR;
@@ -269,6 +268,7 @@ __device__ uint32_t decode_symbols(InflateShared &sh, BitReader &br, T *out, uin
         }
         uint32_t kind = (e >> 8) & 3;
         if (kind == K_LIT) {
+            if (pos >= cap) return R_OVERFLOW;
             br.consume(L);
             if (lane == 0) out[pos] = (T)(e >> 16);
             pos++;
@@ -285,6 +285,7 @@ __device__ uint32_t decode_symbols(InflateShared &sh, BitReader &br, T *out, uin
         }
         uint32_t E = (e >> 4) & 15;
         uint32_t len = (e >> 16) + ((uint32_t)(br.buf >> L) & ((1u << E) - 1));
+        if (pos + len > cap) return R_OVERFLOW;
         br.consume(L + E);
         br.refill();
         uint32_t d = uni(sh.dl[br.peek(D_ROOT)]);
