@@ -18,8 +18,8 @@ namespace gz {
 constexpr int WAVE = 64;
 constexpr uint32_t WIN = 32768;            // DEFLATE window (RFC 1951 3.2)
 constexpr uint32_t MARKER_BASE = 32768;     // replace_markers.rs:23
-constexpr int LL_ROOT = 10;                 // litlen LUT root bits
-constexpr int D_ROOT = 8;                   // distance LUT root bits
+constexpr int LL_ROOT = 11;                 // litlen LUT root bits
+constexpr int D_ROOT = 10;                  // distance LUT root bits
 constexpr int PRE_ROOT = 7;                 // precode LUT covers every precode length
 
 // Piece status (device → host)
@@ -66,10 +66,26 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
 }
 __device__ __forceinline__ void compiler_fence() { __asm__ __volatile__("" ::: "memory"); }
 
+// Pointers that reach a kernel through a struct are generic; accesses through
+// them compile to FLAT instructions, which count against lgkmcnt as well as
+// vmcnt, so every later LDS wait would also wait for outstanding HBM stores.
+// Casting to the global address space keeps them GLOBAL (vmcnt only).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p) {
+    return (gptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gptr<const T> to_global(const T *p) {
+    return (gptr<const T>)p;
+}
+
 // 32-bit little-endian word `wi` of the input; bytes past n read as zero.
-__device__ __forceinline__ uint32_t load_word(const uint8_t *__restrict__ in, uint64_t n, uint64_t wi) {
+__device__ __forceinline__ uint32_t load_word(const uint8_t *__restrict__ in_, uint64_t n, uint64_t wi) {
+    gptr<const uint8_t> in = to_global(in_);
     uint64_t b = wi * 4;
-    if (b + 4 <= n) return reinterpret_cast<const uint32_t *>(in)[wi];
+    if (b + 4 <= n) return ((gptr<const uint32_t>)in)[wi];
     uint32_t v = 0;
     for (int i = 0; i < 4; i++)
         if (b + i < n) v |= (uint32_t)in[b + i] << (8 * i);
@@ -89,26 +105,69 @@ __device__ __forceinline__ uint64_t peek64(const uint8_t *__restrict__ in, uint6
 
 // ---- wave-uniform bit reader -------------------------------------------
 // Holds 33..64 valid bits after refill(); position = next_word*32 - cnt.
+// The input is staged through the wave's registers: lane j holds input words
+// wbase + 4j .. wbase + 4j + 3 (1 KiB per wave, one coalesced 16-byte load
+// per lane).  A refill is a readlane; the window is reloaded synchronously
+// once per 1 KiB.  No load is left in flight across decode iterations: on
+// this ISA vmcnt also counts stores, so a wait for a prefetch would also wait
+// for every output store still on its way to HBM.
 struct BitReader {
     const uint8_t *in;
     uint64_t n;
     uint64_t buf;
     uint64_t next_word;
+    uint64_t wbase;  // multiple of 4
+    uint32_t w0, w1, w2, w3;
     uint32_t cnt;
 
+    __device__ __forceinline__ void load_window(uint64_t base) {
+        wbase = base;
+        const uint64_t wi = base + 4 * (uint64_t)lane_id();
+        if ((wi + 4) * 4 <= n && ((reinterpret_cast<uintptr_t>(in) & 15) == 0)) {
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            u32x4 v = ((gptr<const u32x4>)to_global(in))[wi >> 2];
+            w0 = v.x;
+            w1 = v.y;
+            w2 = v.z;
+            w3 = v.w;
+        } else {
+            w0 = load_word(in, n, wi);
+            w1 = load_word(in, n, wi + 1);
+            w2 = load_word(in, n, wi + 2);
+            w3 = load_word(in, n, wi + 3);
+        }
+        // wait here (vmcnt(0); expcnt and lgkmcnt left at their maxima) so no
+        // load is pending where this path merges with the common refill path
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+    }
+    __device__ __forceinline__ uint32_t word_at(uint64_t wi) {
+        uint64_t k = wi - wbase;
+        if (k >= 256) {
+            load_window(wi & ~3ull);
+            k = wi - wbase;
+        }
+        const int l = (int)(k >> 2);
+        const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)w0, l);
+        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)w1, l);
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)w2, l);
+        const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)w3, l);
+        const uint32_t r = (uint32_t)k & 3;
+        return r == 0 ? a : r == 1 ? b : r == 2 ? c : d;
+    }
     __device__ __forceinline__ void init(const uint8_t *in_, uint64_t n_, uint64_t bit) {
         in = in_;
         n = n_;
         uint64_t wi = bit >> 5;
         uint32_t sh = (uint32_t)(bit & 31);
-        uint64_t lo = uni(load_word(in, n, wi)), hi = uni(load_word(in, n, wi + 1));
+        load_window(wi & ~3ull);
+        uint64_t lo = word_at(wi), hi = word_at(wi + 1);
         buf = (lo | hi << 32) >> sh;
         cnt = 64 - sh;
         next_word = wi + 2;
     }
     __device__ __forceinline__ void refill() {
         if (cnt <= 32) {
-            buf |= (uint64_t)uni(load_word(in, n, next_word)) << cnt;
+            buf |= (uint64_t)word_at(next_word) << cnt;
             next_word++;
             cnt += 32;
         }
@@ -124,6 +183,13 @@ struct BitReader {
         return v;
     }
     __device__ __forceinline__ uint64_t pos() const { return next_word * 32 - cnt; }
+    // re-assert that the reader state is wave-uniform (keeps it in SGPRs)
+    __device__ __forceinline__ void make_uniform() {
+        buf = uni64(buf);
+        cnt = uni(cnt);
+        next_word = uni64(next_word);
+        wbase = uni64(wbase);
+    }
 };
 
 __device__ __forceinline__ uint32_t make_ll_entry(uint32_t sym, uint32_t len) {

@@ -210,7 +210,7 @@ struct gzgpu_ctx {
     DevBuf d_in, d_out, d_pieces, d_pout, d_order, d_counters, d_surv, d_hits, d_slab, d_crc_tab, d_seg_off,
         d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len;
     std::vector<DevBuf> extra_slabs;
-    hipEvent_t ev[8]{};
+    hipEvent_t ev[10]{};
     gzgpu_stats st{};
 };
 
@@ -290,17 +290,96 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
 
 uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-uint64_t marker_slab_bytes(uint64_t expect_entries) {
-    return round_up(2 * (uint64_t)WIN + 2 * (expect_entries + 64 * KIB) + WIN + 512, 256);
+uint64_t marker_slab_bytes(uint64_t expect_entries) { return round_up(2 * (expect_entries + 64 * KIB) + 512, 256); }
+
+bool debug_enabled() {
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("GZGPU_DEBUG");
+        v = e && *e && *e != '0';
+    }
+    return v == 1;
+}
+
+// One persistent inflate launch over `todo`; results land in R.
+void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<PieceHost> &P,
+                    const std::vector<Member> &mem, std::vector<uint32_t> &todo, std::vector<PieceOut> &R) {
+    // longest first
+    std::sort(todo.begin(), todo.end(), [&](uint32_t a, uint32_t b) {
+        uint64_t ea = (P[a].d.stop_bit == NO_STOP ? mem[P[a].member].de : P[a].d.stop_bit) - P[a].d.start_bit;
+        uint64_t eb = (P[b].d.stop_bit == NO_STOP ? mem[P[b].member].de : P[b].d.stop_bit) - P[b].d.start_bit;
+        return ea > eb;
+    });
+    bool any_marker = false;
+    std::vector<PieceDesc> descs(todo.size());
+    for (size_t i = 0; i < todo.size(); i++) {
+        descs[i] = P[todo[i]].d;
+        any_marker |= (descs[i].flags & PF_MARKER) != 0;
+    }
+    std::vector<uint32_t> order(todo.size());
+    for (uint32_t i = 0; i < todo.size(); i++) order[i] = i;
+    upload(c, c->d_pieces, descs.data(), descs.size() * sizeof(PieceDesc));
+    upload(c, c->d_order, order.data(), order.size() * 4);
+    c->d_pout.ensure(descs.size() * sizeof(PieceOut));
+    c->d_counters.ensure(64);
+    HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
+    // LDS bounds residency (160 KiB per CU): u16 ring -> 2 waves/CU, u8 ring -> 3
+    uint32_t per_cu = any_marker ? 2 : 3;
+    if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
+    uint32_t grid = (uint32_t)std::min<uint64_t>(descs.size(), (uint64_t)c->num_cus * per_cu);
+    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    if (any_marker)
+        hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(grid), dim3(64), 0, c->stream, d_in, in_len,
+                           c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), (uint32_t)descs.size(),
+                           c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
+    else
+        hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(grid), dim3(64), 0, c->stream, d_in, in_len,
+                           c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), (uint32_t)descs.size(),
+                           c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[6], c->stream));
+    c->st.launches++;
+    std::vector<PieceOut> ro(descs.size());
+    HIPCHK(hipMemcpyAsync(ro.data(), c->d_pout.p, ro.size() * sizeof(PieceOut), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = ev_ms(c->ev[5], c->ev[6]);
+    c->st.ms_inflate_kernel += ms;
+    c->st.inflate_launches++;
+    for (size_t i = 0; i < todo.size(); i++) R[todo[i]] = ro[i];
+    if (debug_enabled()) {
+        std::vector<double> us;
+        uint64_t t0 = ~0ull, t1 = 0, bytes = 0;
+        for (auto &r : ro) {
+            us.push_back((double)(r.t_end - r.t_begin) / 100.0);
+            t0 = std::min(t0, r.t_begin);
+            t1 = std::max(t1, r.t_end);
+            bytes += r.n16 + r.n8;
+        }
+        std::sort(us.begin(), us.end());
+        fprintf(stderr,
+                "[gzgpu] inflate<%s> pieces=%zu grid=%u kernel=%.3f ms span=%.3f ms piece_us min=%.1f p50=%.1f "
+                "max=%.1f entries=%llu per-wave MB/s p50=%.1f\n",
+                any_marker ? "u16" : "u8", ro.size(), grid, ms, (double)(t1 - t0) / 1e5, us.front(),
+                us[us.size() / 2], us.back(), (unsigned long long)bytes,
+                us[us.size() / 2] > 0 ? (double)bytes / ro.size() / us[us.size() / 2] : 0.0);
+    }
 }
 
 // ---- the member pipeline ---------------------------------------------------
 void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
-    auto t_host0 = std::chrono::steady_clock::now();
-    (void)t_host0;
     const uint64_t chunk_bits = (c->tune.chunk_bytes ? c->tune.chunk_bytes : 512 * KIB) * 8;
-    const uint32_t ppc = c->tune.pieces_per_chunk ? c->tune.pieces_per_chunk : 8;
-    const uint64_t spacing = std::max<uint64_t>(chunk_bits / ppc, 8 * 32 * KIB);
+    // wavefronts per chunk: explicit, or enough pieces to fill every resident
+    // inflate slot (2 per CU with the u16 ring) across the split members
+    uint64_t split_bits = 0;
+    for (auto &M : mem)
+        if (M.status == GZGPU_OK && M.split) split_bits += M.de - M.ds;
+    uint64_t slots = (uint64_t)c->num_cus * 2;
+    uint64_t ppc = c->tune.pieces_per_chunk;
+    if (!ppc) {
+        uint64_t chunks = std::max<uint64_t>(1, (split_bits + chunk_bits - 1) / chunk_bits);
+        ppc = std::max<uint64_t>(1, slots / chunks);
+    }
+    const uint64_t spacing = std::max<uint64_t>(chunk_bits / ppc, 8 * 4 * KIB);
 
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     // ---- 2/3: find + plan
@@ -314,6 +393,7 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
         p0.d.limit_bit = M.limit;
         p0.d.dst = M.out;
         p0.d.cap = M.out_cap;
+        p0.d.window = nullptr;
         p0.d.flags = PF_DIRECT;
         p0.d.member = m;
         p0.member = m;
@@ -325,15 +405,26 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
         if (!M.split || M.de <= M.ds + spacing) continue;
         std::vector<gzgpu_boundary> hits = find_hits(c, d_in, in_len, M.ds + 1, M.de);
         c->st.candidates += (uint32_t)hits.size();
+        // piece starts: per grid point the first hit past it; a stored hit
+        // only when the block after its payload also parses as a header
+        // (info bit 0), which keeps LEN/~NLEN coincidences inside compressed
+        // data from becoming starts.  A scheduling choice only: any start is
+        // verified by the predecessor's end.
         size_t h = 0;
         uint64_t last = M.ds;
         std::vector<size_t> chosen;
+        auto eligible = [&](size_t i) { return hits[i].kind == 2 || (hits[i].info & 1); };
         for (uint64_t g = M.ds + spacing; g < M.de; g += spacing) {
-            while (h < hits.size() && (hits[h].bit_offset < g || hits[h].bit_offset <= last)) h++;
+            while (h < hits.size() && (hits[h].bit_offset < g || hits[h].bit_offset <= last || !eligible(h))) h++;
             if (h >= hits.size()) break;
-            if (hits[h].seek_bit >= M.de) break;
-            chosen.push_back(h);
-            last = hits[h].bit_offset;
+            // within this grid cell a dynamic hit beats a stored one
+            size_t pick = h;
+            for (size_t j = h; j < hits.size() && hits[j].bit_offset < g + spacing && hits[pick].kind != 2; j++)
+                if (hits[j].kind == 2) pick = j;
+            if (hits[pick].seek_bit >= M.de) break;
+            chosen.push_back(pick);
+            last = hits[pick].bit_offset;
+            h = pick + 1;
         }
         const uint64_t total_bits = M.de - M.ds;
         for (size_t k = 0; k < chosen.size(); k++) {
@@ -343,6 +434,7 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
             q.d.start_bit = b.seek_bit;
             q.d.stop_bit = NO_STOP;
             q.d.limit_bit = M.limit;
+            q.d.window = nullptr;
             q.d.flags = PF_MARKER;
             q.d.member = m;
             q.member = m;
@@ -375,76 +467,66 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
     std::vector<uint32_t> todo(P.size());
     for (uint32_t i = 0; i < P.size(); i++) todo[i] = i;
     c->st.pieces = (uint32_t)P.size();
-    c->d_counters.ensure(64);
+    std::vector<uint8_t> linked(P.size(), 0);
     int rounds = 0;
     while (!todo.empty()) {
         if (++rounds > 64) throw StatusError{GZGPU_DECODE_FAILED, "speculation did not converge"};
-        // largest first
-        std::sort(todo.begin(), todo.end(), [&](uint32_t a, uint32_t b) {
-            uint64_t ea = (P[a].d.stop_bit == NO_STOP ? mem[P[a].member].de : P[a].d.stop_bit) - P[a].d.start_bit;
-            uint64_t eb = (P[b].d.stop_bit == NO_STOP ? mem[P[b].member].de : P[b].d.stop_bit) - P[b].d.start_bit;
-            return ea > eb;
-        });
-        std::vector<PieceDesc> descs(todo.size());
-        for (size_t i = 0; i < todo.size(); i++) descs[i] = P[todo[i]].d;
-        std::vector<uint32_t> order(todo.size());
-        for (uint32_t i = 0; i < todo.size(); i++) order[i] = i;
-        upload(c, c->d_pieces, descs.data(), descs.size() * sizeof(PieceDesc));
-        upload(c, c->d_order, order.data(), order.size() * 4);
-        c->d_pout.ensure(descs.size() * sizeof(PieceOut));
-        HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
-        uint32_t wpc = c->tune.waves_per_cu ? c->tune.waves_per_cu : 16;
-        uint32_t grid = (uint32_t)std::min<uint64_t>(descs.size(), (uint64_t)c->num_cus * wpc);
-        HIPCHK(hipEventRecord(c->ev[5], c->stream));
-        hipLaunchKernelGGL(k_inflate, dim3(grid), dim3(64), 0, c->stream, d_in, in_len, c->d_pieces.as<PieceDesc>(),
-                           c->d_order.as<uint32_t>(), (uint32_t)descs.size(), c->d_counters.as<uint32_t>(),
-                           c->d_pout.as<PieceOut>());
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(c->ev[6], c->stream));
-        c->st.launches++;
-        std::vector<PieceOut> ro(descs.size());
-        HIPCHK(hipMemcpyAsync(ro.data(), c->d_pout.p, ro.size() * sizeof(PieceOut), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        c->st.ms_inflate_kernel += ev_ms(c->ev[5], c->ev[6]);
-        c->st.inflate_launches++;
-        for (size_t i = 0; i < todo.size(); i++) R[todo[i]] = ro[i];
         if (rounds > 1) c->st.redecoded += (uint32_t)todo.size();
+        launch_inflate(c, d_in, in_len, P, mem, todo, R);
         todo.clear();
 
-        // confirm chains
+        // Speculation check.  Piece k is linked when piece k-1 stopped inside
+        // k's accepted start range.  Every unlinked piece whose predecessor is
+        // itself linked (so most likely decoded from a true boundary) is
+        // re-decoded from the predecessor's end in the next launch — all
+        // misses of a round at once.
         uint64_t extra_need = 0;
         std::vector<uint32_t> grow;
         for (uint32_t m = 0; m < mem.size(); m++) {
             Member &M = mem[m];
             if (M.status != GZGPU_OK || M.pieces.empty()) continue;
+            linked[M.pieces[0]] = 1;
+            for (size_t k = 1; k < M.pieces.size(); k++) {
+                uint32_t pi = M.pieces[k];
+                PieceHost &p = P[pi];
+                if (p.dropped) break;
+                const PieceOut &pr = R[M.pieces[k - 1]];
+                if (pr.status == PS_FINAL) {
+                    if (linked[M.pieces[k - 1]])
+                        for (size_t j = k; j < M.pieces.size(); j++) P[M.pieces[j]].dropped = true;
+                    linked[pi] = 0;
+                    if (linked[M.pieces[k - 1]]) break;
+                    continue;
+                }
+                if (pr.status != PS_STOPPED) {
+                    linked[pi] = 0;
+                    continue;
+                }
+                if (pr.end_bit >= p.acc_lo && pr.end_bit <= p.acc_hi) {
+                    linked[pi] = 1;
+                } else {
+                    linked[pi] = 0;
+                    if (linked[M.pieces[k - 1]]) {
+                        if (debug_enabled())
+                            fprintf(stderr, "[gzgpu] miss: member %u piece %zu start [%llu,%llu] prev end %llu\n", m, k,
+                                    (unsigned long long)p.acc_lo, (unsigned long long)p.acc_hi,
+                                    (unsigned long long)pr.end_bit);
+                        p.d.start_bit = p.acc_lo = p.acc_hi = pr.end_bit;
+                        todo.push_back(pi);
+                    }
+                }
+            }
+            // errors and overflows: fatal / regrow inside the linked prefix;
+            // overflowed speculative pieces are regrown too (most are valid)
+            bool prefix = true;
             for (size_t k = 0; k < M.pieces.size(); k++) {
                 uint32_t pi = M.pieces[k];
                 PieceHost &p = P[pi];
-                if (p.dropped) continue;
-                if (k > 0) {
-                    PieceHost &prev = P[M.pieces[k - 1]];
-                    const PieceOut &pr = R[M.pieces[k - 1]];
-                    if (!prev.confirmed) break;
-                    if (pr.status == PS_FINAL) {
-                        // stream ended before this speculative start: drop the rest
-                        for (size_t j = k; j < M.pieces.size(); j++) P[M.pieces[j]].dropped = true;
-                        break;
-                    }
-                    if (pr.status != PS_STOPPED) break;
-                    if (!p.confirmed) {
-                        if (pr.end_bit >= p.acc_lo && pr.end_bit <= p.acc_hi) {
-                            p.confirmed = true;
-                        } else {
-                            // speculation miss: re-decode from the confirmed boundary
-                            p.d.start_bit = p.acc_lo = p.acc_hi = pr.end_bit;
-                            p.confirmed = true;
-                            todo.push_back(pi);
-                            break;
-                        }
-                    }
-                }
+                if (p.dropped) break;
+                prefix = prefix && linked[pi];
+                if (std::find(todo.begin(), todo.end(), pi) != todo.end()) continue;
                 const PieceOut &r = R[pi];
-                if (r.status == PS_ERROR) {
+                if (prefix && r.status == PS_ERROR) {
                     M.status = GZGPU_DECODE_FAILED;
                     M.msg = "deflate stream invalid at bit " + std::to_string(r.end_bit) + " (piece " +
                             std::to_string(k) + ", error " + std::to_string(r.err) + ")";
@@ -456,7 +538,6 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
                         M.msg = "output capacity exceeded";
                         break;
                     }
-                    // grow the slab from the observed expansion and re-decode
                     uint64_t entries = r.n16 + r.n8;
                     uint64_t span = (p.d.stop_bit == NO_STOP ? M.de : p.d.stop_bit) - p.d.start_bit;
                     double rate = r.bits_done ? (double)entries / (double)r.bits_done : 1.0;
@@ -465,9 +546,11 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
                     extra_need += p.d.cap;
                     grow.push_back(pi);
                     todo.push_back(pi);
-                    break;
                 }
             }
+            if (M.status != GZGPU_OK)
+                todo.erase(std::remove_if(todo.begin(), todo.end(), [&](uint32_t pi) { return P[pi].member == m; }),
+                           todo.end());
         }
         if (!grow.empty()) {
             DevBuf b;
@@ -495,6 +578,11 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
         for (size_t k = 0; k < M.pieces.size(); k++) {
             uint32_t pi = M.pieces[k];
             if (P[pi].dropped) break;
+            if (!linked[pi]) {
+                M.status = GZGPU_DECODE_FAILED;
+                M.msg = "speculative piece " + std::to_string(k) + " never linked to its predecessor";
+                break;
+            }
             const PieceOut &r = R[pi];
             uint64_t L = r.n16 + r.n8;
             if (P[pi].d.flags & PF_MARKER) {
@@ -548,9 +636,11 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
         upload(c, c->d_rp, rps.data(), rps.size() * sizeof(ResolvePiece));
         upload(c, c->d_chains, chains.data(), chains.size() * 4);
         uint32_t nch = (uint32_t)chains.size() - 1;
+        HIPCHK(hipEventRecord(c->ev[7], c->stream));
         hipLaunchKernelGGL(k_chain, dim3(std::min<uint32_t>(nch, (uint32_t)c->num_cus * 4)), dim3(1024), 0, c->stream,
                            c->d_rp.as<ResolvePiece>(), c->d_chains.as<uint32_t>(), nch);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[8], c->stream));
         c->st.launches++;
         if (!tiles.empty()) {
             upload(c, c->d_tiles, tiles.data(), tiles.size() * sizeof(ResolveTile));
@@ -630,6 +720,9 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
         HIPCHK(hipEventRecord(c->ev[4], c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
+    if (debug_enabled() && !rps.empty())
+        fprintf(stderr, "[gzgpu] resolve: chain %.3f ms (%zu pieces), body %.3f ms (%zu tiles)\n",
+                ev_ms(c->ev[7], c->ev[8]), rps.size(), ev_ms(c->ev[8], c->ev[3]), tiles.size());
     c->st.ms_find += ev_ms(c->ev[0], c->ev[1]);
     c->st.ms_inflate += ev_ms(c->ev[1], c->ev[2]);
     c->st.ms_resolve += ev_ms(c->ev[2], c->ev[3]);
@@ -1074,13 +1167,13 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
         pd.stop_bit = stop_bit;
         pd.limit_bit = (uint64_t)in_len * 8;
         pd.member = 0;
-        DevBuf slab;
+        DevBuf slab, win;
         if (window) {
-            // clean decode against a known window: place it in front of the
-            // output in a scratch buffer, decode as a marker-free DIRECT piece
-            slab.ensure(WIN + out_cap_elems + 64);
-            HIPCHK(hipMemcpyAsync(slab.p, window, WIN, hipMemcpyHostToDevice, c->stream));
-            pd.dst = slab.as<uint8_t>() + WIN;
+            // clean decode against a known window, straight into d_out
+            win.ensure(WIN);
+            HIPCHK(hipMemcpyAsync(win.p, window, WIN, hipMemcpyHostToDevice, c->stream));
+            pd.window = win.as<uint8_t>();
+            pd.dst = (uint8_t *)d_out;
             pd.cap = out_cap_elems;
             pd.flags = PF_WINDOW;
         } else {
@@ -1096,9 +1189,14 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
         c->d_pout.ensure(sizeof(PieceOut));
         c->d_counters.ensure(64);
         HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
-        hipLaunchKernelGGL(k_inflate, dim3(1), dim3(64), 0, c->stream, (const uint8_t *)d_in, (uint64_t)in_len,
-                           c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u, c->d_counters.as<uint32_t>(),
-                           c->d_pout.as<PieceOut>());
+        if (window)
+            hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(1), dim3(64), 0, c->stream, (const uint8_t *)d_in,
+                               (uint64_t)in_len, c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u,
+                               c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
+        else
+            hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(1), dim3(64), 0, c->stream, (const uint8_t *)d_in,
+                               (uint64_t)in_len, c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u,
+                               c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
         HIPCHK(hipGetLastError());
         PieceOut po;
         HIPCHK(hipMemcpyAsync(&po, c->d_pout.p, sizeof po, hipMemcpyDeviceToHost, c->stream));
@@ -1110,17 +1208,14 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
                                                                       : GZGPU_DECODE_FAILED;
         uint64_t n = po.n16 + po.n8;
         res->out_len = n;
-        if (window) {
-            HIPCHK(hipMemcpyAsync(d_out, pd.dst, std::min<uint64_t>(n, out_cap_elems), hipMemcpyDeviceToDevice,
-                                  c->stream));
-        } else {
+        if (!window) {
             // u16 view of every entry: marker part as is, clean part widened
             if (n > out_cap_elems) n = out_cap_elems;
             uint64_t a = std::min<uint64_t>(po.n16, n);
-            HIPCHK(hipMemcpyAsync(d_out, pd.dst + 2 * WIN, a * 2, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(d_out, pd.dst, a * 2, hipMemcpyDeviceToDevice, c->stream));
             if (n > a) {
                 std::vector<uint8_t> tmp(n - a);
-                HIPCHK(hipMemcpyAsync(tmp.data(), pd.dst + po.u8_off + WIN, n - a, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipMemcpyAsync(tmp.data(), pd.dst + po.u8_off, n - a, hipMemcpyDeviceToHost, c->stream));
                 HIPCHK(hipStreamSynchronize(c->stream));
                 std::vector<uint16_t> w(tmp.begin(), tmp.end());
                 HIPCHK(hipMemcpyAsync((uint16_t *)d_out + a, w.data(), w.size() * 2, hipMemcpyHostToDevice, c->stream));
@@ -1128,6 +1223,7 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
         }
         HIPCHK(hipStreamSynchronize(c->stream));
         slab.release();
+        win.release();
         return (int)GZGPU_OK;
     });
 }

@@ -36,6 +36,24 @@ __device__ __forceinline__ void wave_append(bool ok, uint32_t *count, uint32_t &
     slot = base + mbcnt(m);
 }
 
+// Header + precode conditions of the dynamic predicate at `o` (stage A).
+__device__ __forceinline__ bool dyn_stage_a(const uint8_t *__restrict__ in, uint64_t n, uint64_t o) {
+    uint64_t h = peek64(in, n, o);
+    uint32_t hclen = (uint32_t)(h >> 13) & 15;
+    bool ok = (h & 7) == 4 /* BFINAL=0, BTYPE=10 */ && ((h >> 3) & 31) <= 29 && ((h >> 8) & 31) <= 29;
+    if (!ok) return false;
+    uint64_t p = peek64(in, n, o + 17);
+    uint32_t leaves = 0, gt1 = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 19; i++) {
+        uint32_t l = (uint32_t)(p >> (3 * i)) & 7;
+        if (i >= hclen + 4) l = 0;
+        leaves += l ? (1u << (7 - l)) : 0u;
+        gt1 += l > 1;
+    }
+    return leaves == 128 || (leaves == 64 && gt1 == 0);
+}
+
 __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ in, uint64_t n, uint64_t start_bit,
                                                     uint64_t end_bit, uint64_t *__restrict__ surv, uint32_t surv_cap,
                                                     uint32_t *surv_count, gzgpu_boundary *__restrict__ hits,
@@ -52,28 +70,12 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
         bool in_range = o < end_bit;
         // ---- dynamic stage A
         bool dyn = false;
-        if (in_range && o + 17 < total) {
-            uint64_t h = peek64(in, n, o);
-            uint32_t hclen = (uint32_t)(h >> 13) & 15;
-            dyn = (h & 7) == 4 /* BFINAL=0, BTYPE=10 */ && ((h >> 3) & 31) <= 29 && ((h >> 8) & 31) <= 29;
-            if (dyn) {
-                uint64_t p = peek64(in, n, o + 17);
-                uint32_t leaves = 0, gt1 = 0;
-#pragma unroll
-                for (uint32_t i = 0; i < 19; i++) {
-                    uint32_t l = (uint32_t)(p >> (3 * i)) & 7;
-                    if (i >= hclen + 4) l = 0;
-                    leaves += l ? (1u << (7 - l)) : 0u;
-                    gt1 += l > 1;
-                }
-                dyn = leaves == 128 || (leaves == 64 && gt1 == 0);
-            }
-        }
+        if (in_range && o + 17 < total) dyn = dyn_stage_a(in, n, o);
         uint32_t slot;
         wave_append(dyn, surv_count, slot);
         if (dyn && slot < surv_cap) surv[slot] = o;
         // ---- stored
-        bool st = false;
+        bool st = false, succ = false;
         uint64_t earliest = 0, latest = 0;
         if (in_range && (o & 7) == 0) {
             uint64_t by = o >> 3;
@@ -94,6 +96,23 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
                         latest = o - 3;
                         uint32_t len = size & 0xFFFF;
                         st = earliest >= start_bit && latest < end_bit && len > 0 && by + 4 + len <= n;
+                        // successor header after the payload (planning hint only)
+                        uint64_t nb = by + 4 + len;
+                        if (st && nb < n) {
+                            uint32_t h3 = in[nb] & 7, bt = h3 >> 1;
+                            if (bt == 0) {
+                                if (nb + 5 <= n) {
+                                    uint32_t s2 = (uint32_t)in[nb + 1] | (uint32_t)in[nb + 2] << 8 |
+                                                  (uint32_t)in[nb + 3] << 16 | (uint32_t)in[nb + 4] << 24;
+                                    succ = ((s2 ^ (s2 >> 16)) & 0xFFFF) == 0xFFFF;
+                                }
+                            } else if (bt == 2) {
+                                // a final dynamic block would fail BFINAL=0: test the
+                                // header with BFINAL masked off
+                                succ = (in[nb] & 1) ? ((peek64(in, n, nb * 8) >> 3 & 31) <= 29)
+                                                    : dyn_stage_a(in, n, nb * 8);
+                            }
+                        }
                     }
                 }
             }
@@ -103,7 +122,7 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
             hits[slot].bit_offset = earliest;
             hits[slot].seek_bit = latest;
             hits[slot].kind = 0;
-            hits[slot].info = 0;
+            hits[slot].info = succ ? 1u : 0u;
         }
     }
 }

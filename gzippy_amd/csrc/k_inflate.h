@@ -5,37 +5,39 @@
 // the first non-final block header at or after its stop bit, or at the end of
 // the final block (stop rule of chunk_decode.rs:2029-2040).
 //
-// Output modes
-//   DIRECT : window is all zeros (stream start, WindowMap{0: zeros},
-//            chunk_fetcher.rs:694-696); bytes go straight to the member's
-//            final output.  Back-references before the start read zero.
-//   MARKER : window unknown; u16 entries into the piece's slab: literals
-//            < 256, window markers 32768 + index-from-oldest
-//            (replace_markers.rs:6-23; init_marker_zone,
-//            marker_inflate.rs:255-268).  The slab starts with a 32768-entry
-//            u16 preamble holding the marker values themselves, so a copy
-//            that reaches before the piece start copies markers for free.
-//   CLEAN  : after the last 32 KiB of a MARKER piece hold no marker
-//            (WidthRing::should_flip, width_ring.rs:310), the tail is
-//            narrowed into a u8 preamble and decoding continues in u8.
+// The 32 KiB sliding window lives in LDS as a ring of 32768 entries, so a
+// back-reference never reads HBM: copies read the ring and write both the
+// ring and the piece's output (store-only HBM traffic).
 //
-// Slab layout (bytes): [0, 64 KiB) u16 preamble | u16 marker entries |
-// (256-aligned) u8 region: 32 KiB u8 preamble | clean bytes.
+// Window / output modes
+//   DIRECT : the window is all zeros (stream start, WindowMap{0: zeros},
+//            chunk_fetcher.rs:694-696); bytes go straight to the member's
+//            final output.
+//   WINDOW : the caller supplies the 32 KiB window; bytes to dst.
+//   MARKER : window unknown.  The ring starts with the marker values
+//            32768 + index-from-oldest (replace_markers.rs:6-23;
+//            init_marker_zone, marker_inflate.rs:255-268), so a copy that
+//            reaches before the piece start copies markers for free.  Output
+//            is u16 entries (< 256 literal, >= 32768 marker) until the last
+//            32768 entries hold no marker (WidthRing::should_flip,
+//            width_ring.rs:310); from then on the output is u8.
+//
+// Slab layout of a MARKER piece (bytes): [u16 entries: 2*n16] (256-aligned)
+// [u8 entries: n8].
 #pragma once
 #include "gz_common.h"
 
 namespace gz {
 
-// PF_DIRECT: all-zero window, no preamble.  PF_WINDOW: clean u8 decode whose
-// 32 KiB window the caller placed at dst[-32768, 0).  PF_MARKER: window absent.
 enum : uint32_t { PF_DIRECT = 1, PF_MARKER = 2, PF_WINDOW = 4 };
 
 struct PieceDesc {
-    uint64_t start_bit;   // first block header to decode
-    uint64_t stop_bit;    // stop before a non-final block header at >= stop_bit
-    uint64_t limit_bit;   // end of the member's readable deflate bits
-    uint8_t *dst;         // DIRECT: output bytes; MARKER: slab base
-    uint64_t cap;         // DIRECT: output capacity (bytes); MARKER: slab bytes
+    uint64_t start_bit;      // first block header to decode
+    uint64_t stop_bit;       // stop before a non-final block header at >= stop_bit
+    uint64_t limit_bit;      // end of the member's readable deflate bits
+    uint8_t *dst;            // DIRECT/WINDOW: output bytes; MARKER: slab base
+    uint64_t cap;            // DIRECT/WINDOW: output capacity (bytes); MARKER: slab bytes
+    const uint8_t *window;   // WINDOW: 32768 bytes, oldest first
     uint32_t flags;
     uint32_t member;
 };
@@ -43,16 +45,22 @@ struct PieceDesc {
 struct PieceOut {
     uint64_t end_bit;
     uint64_t n16;       // u16 marker-mode entries
-    uint64_t n8;        // u8 entries (DIRECT, or CLEAN after the flip)
-    uint64_t u8_off;    // slab byte offset of the u8 region (its 32 KiB preamble first)
+    uint64_t n8;        // u8 entries (DIRECT/WINDOW, or after the flip)
+    uint64_t u8_off;    // slab byte offset of the u8 entries
     uint64_t bits_done; // bits consumed when the piece stopped (sizing on overflow)
+    uint64_t t_begin, t_end;  // wall clock (100 MHz) around the piece
     uint32_t status;
     uint32_t err;
     uint32_t nblocks;
-    uint32_t pad;
+    uint32_t mode;
 };
 
+constexpr uint32_t RING = WIN;
+constexpr uint32_t RMASK = RING - 1;
+
+template <typename RT>
 struct __align__(16) InflateShared {
+    RT ring[RING];
     uint32_t ll[1 << LL_ROOT];
     uint32_t dl[1 << D_ROOT];
     uint16_t ll_sorted[288];
@@ -68,12 +76,12 @@ struct __align__(16) InflateShared {
 };
 
 // ---- canonical table construction (whole wave) ----------------------------
-// mode 0: litlen, 1: distance, 2: precode (entry = len | sym << 16).
+// MODE 0: litlen, 1: distance, 2: precode (entry = len | sym << 16).
 // Returns 0, or -1 when the lengths over-subscribe the code space
 // (build_precode_table, marker_inflate.rs:4515-4540).  Incomplete codes are
 // accepted; their unused codewords fail when decoded.
-template <int MODE>
-__device__ int build_table(InflateShared &sh, const uint8_t *lens, int n, uint32_t *lut, int R, uint16_t *count,
+template <int MODE, typename SH>
+__device__ int build_table(SH &sh, const uint8_t *lens, int n, uint32_t *lut, int R, uint16_t *count,
                            uint16_t *sorted) {
     const uint32_t lane = lane_id();
     uint32_t l[5], rank[5];
@@ -141,7 +149,8 @@ __device__ int build_table(InflateShared &sh, const uint8_t *lens, int n, uint32
     return 0;
 }
 
-__device__ void build_fixed(InflateShared &sh) {
+template <typename SH>
+__device__ void build_fixed(SH &sh) {
     const uint32_t lane = lane_id();
     for (uint32_t s = lane; s < 288; s += WAVE) sh.lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
     __syncthreads();
@@ -154,7 +163,8 @@ __device__ void build_fixed(InflateShared &sh) {
 // read_dynamic_huffman_coding (marker_inflate.rs:1206-1245) and
 // read_literal_and_distance_code_lengths (:4447-4510).  Returns E_NONE or an
 // error code.
-__device__ uint32_t read_dynamic(InflateShared &sh, BitReader &br) {
+template <typename SH>
+__device__ __attribute__((always_inline)) uint32_t read_dynamic(SH &sh, BitReader &br) {
     const uint32_t lane = lane_id();
     br.refill();
     uint32_t hlit = br.read(5) + 257, hdist = br.read(5) + 1, hclen = br.read(4) + 4;
@@ -172,7 +182,7 @@ __device__ uint32_t read_dynamic(InflateShared &sh, BitReader &br) {
     __syncthreads();
     if (lane < hclen) sh.pre_lens[c_pre_order[lane]] = (uint8_t)pl;
     __syncthreads();
-    // precode LUT lives in the litlen LUT storage until the litlen table is built
+    // the precode LUT lives in the litlen LUT storage until the litlen table is built
     if (build_table<2>(sh, sh.pre_lens, 19, sh.ll, PRE_ROOT, sh.d_count, sh.d_sorted)) return E_PRECODE;
     const uint32_t total = hlit + hdist;
     uint32_t i = 0, prev = 0;
@@ -213,160 +223,415 @@ __device__ uint32_t read_dynamic(InflateShared &sh, BitReader &br) {
     return E_NONE;
 }
 
-// ---- LZ77 copy by the whole wave -----------------------------------------
-// Entry pos+i takes the value at pos - dist + (i mod dist): every source lies
-// before pos, so the lanes of one copy never depend on each other.  Copies
-// that read what an earlier symbol wrote rely on the wave's vector memory
-// operations completing in program order.
-template <typename T, bool MARKER, bool ZW>
-__device__ __forceinline__ void wave_copy(T *out, uint64_t pos, uint32_t len, uint32_t dist,
-                                          uint64_t &last_marker_end) {
-    const uint32_t lane = lane_id();
-    const bool overlap = dist < len;
-    for (uint32_t k = 0; k < len; k += WAVE) {
-        uint32_t i = k + lane;
-        bool act = i < len;
-        uint32_t si = overlap ? (i % dist) : i;
-        int64_t src = (int64_t)pos + (int64_t)si - (int64_t)dist;
-        T v = 0;
-        if (act) {
-            if (!ZW || src >= 0) v = out[src];
-            out[pos + i] = v;
-        }
-        if (MARKER) {
-            uint64_t m = ballot(act && v >= 256);
-            if (m) last_marker_end = pos + k + 64 - (uint64_t)__clzll(m);
-        }
-    }
-    compiler_fence();
+// ---- output segment ----------------------------------------------------
+// Entries go to `p` as u16 (wide: marker mode) or u8.  `seg0` is the piece
+// position of entry 0 of this segment (the ring index of entry i is
+// seg0 + i).  One decode loop serves both widths through a wave-uniform
+// flag, which keeps a single copy of the loop (and its registers) per kernel.
+struct OutSeg {
+    gptr<uint8_t> p;
+    uint64_t pos, cap, seg0;
+    uint32_t wide;
+};
+
+__device__ __forceinline__ void put(const OutSeg &o, uint64_t i, uint32_t v) {
+    if (o.wide) ((gptr<uint16_t>)o.p)[i] = (uint16_t)v;
+    else o.p[i] = (uint8_t)v;
 }
 
-enum : uint32_t { R_EOB = 0, R_FLIP = 1, R_OVERFLOW = 2, R_ERROR = 3 };
-
-// Decode Huffman symbols of the current block until end-of-block.
-template <typename T, bool MARKER, bool ZW>
-__device__ uint32_t decode_symbols(InflateShared &sh, BitReader &br, T *out, uint64_t &pos, uint64_t cap,
-                                   uint64_t base, uint64_t limit_bit, uint64_t &last_marker_end, uint32_t &err) {
+// ---- LZ77 copy by the whole wave -----------------------------------------
+// Entry pos+i takes the value at pos - dist + (i mod dist): every source lies
+// before pos, so the lanes of one copy never depend on each other, and a ring
+// of exactly 32768 entries suffices (each iteration reads all its sources
+// before writing; the only slot shared by a read and a write of the same
+// iteration is dist == 32768 in the same lane).  LDS operations of one wave
+// execute in order, which orders a copy after the symbols before it.
+// All index math is 32-bit (segment positions stay below 2^31).
+template <typename RT>
+__device__ __attribute__((always_inline)) void wave_copy(RT *ring, const OutSeg &o, uint32_t pos, uint32_t g0,
+                                                         uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
     const uint32_t lane = lane_id();
-    for (;;) {
-        if (br.pos() > limit_bit) {
-            err = E_TRUNC;
-            return R_ERROR;
-        }
-        br.refill();
-        uint32_t e = uni(sh.ll[br.peek(LL_ROOT)]);
-        uint32_t L = e & 15;
-        if (L == 0) {
-            uint32_t sl;
-            int s = slow_decode(br.peek(15), sh.ll_count, sh.ll_sorted, sl);
-            if (s < 0) {
-                err = E_BAD_SYM;
-                return R_ERROR;
-            }
-            e = make_ll_entry((uint32_t)s, sl);
-            L = sl;
-        }
-        uint32_t kind = (e >> 8) & 3;
-        if (kind == K_LIT) {
-            if (pos >= cap) return R_OVERFLOW;
-            br.consume(L);
-            if (lane == 0) out[pos] = (T)(e >> 16);
-            pos++;
-            continue;
-        }
-        if (kind == K_EOB) {
-            br.consume(L);
+    const uint32_t gp = g0 + pos;  // ring position of the copy's first entry
+    if (dist < len) {
+        // overlapping: source offset i mod dist, computed with a float
+        // quotient and one correction step (i, dist <= 258: exact enough)
+        const float rd = 1.0f / (float)dist;
+        for (uint32_t k = 0; k < len; k += WAVE) {
+            uint32_t i = k + lane;
+            bool act = i < len;
+            uint32_t q = (uint32_t)((float)i * rd);
+            int32_t si = (int32_t)(i - q * dist);
+            si += si < 0 ? (int32_t)dist : 0;
+            si -= si >= (int32_t)dist ? (int32_t)dist : 0;
+            uint32_t v = 0;
+            if (act) v = ring[(gp + (uint32_t)si - dist) & RMASK];
             compiler_fence();
-            return R_EOB;
+            if (act) {
+                ring[(gp + i) & RMASK] = (RT)v;
+                put(o, pos + i, v);
+            }
+            if (o.wide) {
+                uint64_t m = ballot(act && v >= 256);
+                if (m) last_marker_end = o.seg0 + pos + k + 64 - (uint64_t)__clzll(m);
+            }
+            compiler_fence();
         }
-        if (kind == K_BAD) {
+    } else {
+        for (uint32_t k = 0; k < len; k += WAVE) {
+            uint32_t i = k + lane;
+            bool act = i < len;
+            uint32_t v = 0;
+            if (act) v = ring[(gp + i - dist) & RMASK];
+            compiler_fence();
+            if (act) {
+                ring[(gp + i) & RMASK] = (RT)v;
+                put(o, pos + i, v);
+            }
+            if (o.wide) {
+                uint64_t m = ballot(act && v >= 256);
+                if (m) last_marker_end = o.seg0 + pos + k + 64 - (uint64_t)__clzll(m);
+            }
+            compiler_fence();
+        }
+    }
+}
+
+enum : uint32_t { R_EOB = 0, R_FLIP = 1, R_OVERFLOW = 2, R_ERROR = 3, R_CONT = 4 };
+
+// 128 bits of input starting at absolute bit P, as two wave-uniform words.
+// Reads six 32-bit words through the scalar data cache (SMEM loads, counted
+// by lgkmcnt — never ordered behind the wave's pending HBM stores); near the
+// end of the buffer it falls back to bounds-checked word loads.
+__device__ __attribute__((always_inline)) void window128(const uint8_t *__restrict__ in, uint64_t n, uint64_t P,
+                                                         uint64_t &W0, uint64_t &W1) {
+    const uint64_t wP = P >> 5;
+    const uint32_t sh = (uint32_t)(P & 31);
+    uint32_t a0, a1, a2, a3, a4, a5;
+    if ((wP + 6) * 4 <= n) {
+        // the input is never written while the kernel runs: read it through
+        // the constant address space so a wave-uniform load becomes SMEM
+        typedef __attribute__((address_space(4))) const uint32_t cword;
+        cword *w = (cword *)reinterpret_cast<const uint32_t *>(in) + wP;
+        a0 = w[0];
+        a1 = w[1];
+        a2 = w[2];
+        a3 = w[3];
+        a4 = w[4];
+        a5 = w[5];
+    } else {
+        a0 = load_word(in, n, wP);
+        a1 = load_word(in, n, wP + 1);
+        a2 = load_word(in, n, wP + 2);
+        a3 = load_word(in, n, wP + 3);
+        a4 = load_word(in, n, wP + 4);
+        a5 = load_word(in, n, wP + 5);
+    }
+    const uint64_t A = uni64((uint64_t)a0 | (uint64_t)a1 << 32), B = uni64((uint64_t)a2 | (uint64_t)a3 << 32),
+                   C = uni64((uint64_t)a4 | (uint64_t)a5 << 32);
+    W0 = sh ? (A >> sh) | (B << (64 - sh)) : A;
+    W1 = sh ? (B >> sh) | (C << (64 - sh)) : B;
+}
+
+// One symbol at P, the slow way (codes longer than the LUT roots, and any
+// condition the multi-symbol round leaves to a single step).
+template <typename RT, typename SH>
+__device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
+                                                               uint64_t &P, OutSeg &o, uint32_t &pos, uint32_t cap,
+                                                               uint32_t g0, uint32_t hist0, uint64_t &last_marker_end,
+                                                               uint32_t &err) {
+    const uint32_t lane = lane_id();
+    uint64_t W0, W1;
+    window128(in, n, P, W0, W1);
+    uint32_t e = uni(sh.ll[(uint32_t)W0 & ((1u << LL_ROOT) - 1)]);
+    if ((e & 15) == 0) {
+        uint32_t sl;
+        int sy = (int)uni((uint32_t)slow_decode((uint32_t)W0 & 0x7FFF, sh.ll_count, sh.ll_sorted, sl));
+        if (sy < 0) {
             err = E_BAD_SYM;
             return R_ERROR;
         }
-        uint32_t E = (e >> 4) & 15;
-        uint32_t len = (e >> 16) + ((uint32_t)(br.buf >> L) & ((1u << E) - 1));
-        if (pos + len > cap) return R_OVERFLOW;
-        br.consume(L + E);
-        br.refill();
-        uint32_t d = uni(sh.dl[br.peek(D_ROOT)]);
-        uint32_t DL = d & 15;
-        if (DL == 0) {
-            uint32_t sl;
-            int s = slow_decode(br.peek(15), sh.d_count, sh.d_sorted, sl);
-            if (s < 0) {
-                err = E_BAD_DIST_SYM;
-                return R_ERROR;
-            }
-            d = make_d_entry((uint32_t)s, sl);
-            DL = sl;
+        e = make_ll_entry((uint32_t)sy, uni(sl));
+    }
+    e = uni(e);
+    const uint32_t L = e & 15, kind = (e >> 8) & 3;
+    if (kind == K_BAD) {
+        err = E_BAD_SYM;
+        return R_ERROR;
+    }
+    if (kind == K_EOB) {
+        P += L;
+        return R_EOB;
+    }
+    if (kind == K_LIT) {
+        if (pos >= cap) return R_OVERFLOW;
+        if (lane == 0) {
+            sh.ring[(g0 + pos) & RMASK] = (RT)(e >> 16);
+            put(o, pos, e >> 16);
         }
-        if (((d >> 8) & 3) != K_LEN) {
+        compiler_fence();
+        pos++;
+        P += L;
+        return R_CONT;
+    }
+    const uint32_t E = (e >> 4) & 15;
+    const uint32_t len = (e >> 16) + ((uint32_t)(W0 >> L) & ((1u << E) - 1));
+    if (pos + len > cap) return R_OVERFLOW;
+    const uint32_t off = L + E;
+    const uint64_t wd = (W0 >> off) | (off ? W1 << (64 - off) : 0);
+    uint32_t d = uni(sh.dl[(uint32_t)wd & ((1u << D_ROOT) - 1)]);
+    if ((d & 15) == 0) {
+        uint32_t sl;
+        int sy = (int)uni((uint32_t)slow_decode((uint32_t)wd & 0x7FFF, sh.d_count, sh.d_sorted, sl));
+        if (sy < 0) {
             err = E_BAD_DIST_SYM;
             return R_ERROR;
         }
-        uint32_t DE = (d >> 4) & 15;
-        uint32_t dist = (d >> 16) + ((uint32_t)(br.buf >> DL) & ((1u << DE) - 1));
-        br.consume(DL + DE);
-        if ((uint64_t)dist > base + pos + WIN) {
-            err = E_DIST_FAR;
-            return R_ERROR;
-        }
-        wave_copy<T, MARKER, ZW>(out, pos, len, dist, last_marker_end);
-        pos += len;
-        if (MARKER && pos >= WIN && pos - last_marker_end >= WIN) return R_FLIP;
+        d = make_d_entry((uint32_t)sy, uni(sl));
     }
+    d = uni(d);
+    if (((d >> 8) & 3) != K_LEN) {
+        err = E_BAD_DIST_SYM;
+        return R_ERROR;
+    }
+    const uint32_t DL = d & 15, DE = (d >> 4) & 15;
+    const uint32_t dist = (d >> 16) + ((uint32_t)(wd >> DL) & ((1u << DE) - 1));
+    if (dist > hist0 + pos + WIN) {
+        err = E_DIST_FAR;
+        return R_ERROR;
+    }
+    wave_copy<RT>(sh.ring, o, pos, g0, len, dist, last_marker_end);
+    pos += len;
+    P += off + DL + DE;
+    return R_CONT;
+}
+
+// Decode the Huffman symbols of the current block until end-of-block, from
+// bit P.  Multi-symbol rounds: lane j looks up the litlen (and, for a length
+// code, the distance) LUT entry of a symbol starting at bit P + j, all 64 in
+// one LDS round trip.  A scalar walk from lane 0 follows the chain of real
+// symbol starts (about five per 64 bits here) and assigns output offsets;
+// then the round's literals are written in one store and its copies run in
+// order.  Anything the round cannot take (long codes, invalid codes, a full
+// output, a distance past the available history) is left to scalar_step.
+template <typename RT, typename SH>
+__device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
+                                                                  uint64_t &P, OutSeg &o, uint64_t limit_bit,
+                                                                  uint64_t &last_marker_end, uint32_t &err) {
+    const uint32_t lane = lane_id();
+    // 32-bit segment positions: the segment is rebased before 2^30 entries
+    uint32_t pos = (uint32_t)o.pos;
+    uint32_t cap = (uint32_t)(o.cap < 0x7FFFFFFFull ? o.cap : 0x7FFFFFFFull);
+    uint32_t g0 = (uint32_t)o.seg0;
+    const uint32_t hist0 = (uint32_t)(o.seg0 < WIN ? o.seg0 : WIN);  // history before the segment, capped
+    uint32_t r = R_EOB;
+    for (;;) {
+        P = uni64(P);
+        pos = uni(pos);
+        last_marker_end = uni64(last_marker_end);
+        if (P > limit_bit) {
+            err = E_TRUNC;
+            r = R_ERROR;
+            break;
+        }
+        uint64_t W0, W1;
+        window128(in, n, P, W0, W1);
+        // ---- per lane: the symbol that would start at bit P + lane
+        const uint64_t ch = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
+        const uint32_t e = sh.ll[(uint32_t)ch & ((1u << LL_ROOT) - 1)];
+        const uint32_t L = e & 15, kind = (e >> 8) & 3, E = (e >> 4) & 15;
+        uint32_t ok = (L != 0) & (kind != K_BAD);
+        uint32_t tot = L, olen = kind == K_EOB ? 0u : 1u, dist = 0;
+        if (kind == K_LEN) {
+            const uint32_t off = L + E;
+            olen = (e >> 16) + ((uint32_t)(ch >> L) & ((1u << E) - 1));
+            const uint32_t d = sh.dl[(uint32_t)(ch >> off) & ((1u << D_ROOT) - 1)];
+            const uint32_t DL = d & 15, DE = (d >> 4) & 15;
+            ok &= (DL != 0) & (((d >> 8) & 3) == K_LEN);
+            dist = (d >> 16) + ((uint32_t)(ch >> (off + DL)) & ((1u << DE) - 1));
+            tot = off + DL + DE;
+        }
+        const uint32_t packed = ok | kind << 1 | tot << 3 | olen << 9;
+        // ---- walk the chain of symbol starts: at most 8 symbols, fully
+        // unrolled with a uniform `alive` flag so the walk stays scalar
+        const uint32_t room = cap - pos;
+        uint32_t s = 0, run = 0, opos = 0;
+        uint64_t lits = 0, matches = 0;
+        bool eob = false, alive = true;
+#pragma unroll
+        for (int step = 0; step < 8; step++) {
+            if (alive) {
+                const uint32_t pk = uni((uint32_t)__builtin_amdgcn_readlane((int)packed, (int)(s & 63)));
+                const uint32_t k = (pk >> 1) & 3, t = (pk >> 3) & 63, ol = pk >> 9;
+                if (!(pk & 1)) {
+                    alive = false;
+                } else if (k == K_EOB) {
+                    eob = true;
+                    s += t;
+                    alive = false;
+                } else if (ol > room - run) {
+                    alive = false;
+                } else {
+                    opos = lane == s ? run : opos;
+                    if (k == K_LIT) lits |= 1ull << s;
+                    else matches |= 1ull << s;
+                    run += ol;
+                    s += t;
+                    alive = s < 64;
+                }
+            }
+            s = uni(s);
+            run = uni(run);
+            lits = uni64(lits);
+            matches = uni64(matches);
+        }
+        // ---- distance checks; cut the round where a copy needs it
+        for (uint64_t mm = matches; mm; mm = uni64(mm & (mm - 1))) {
+            const uint32_t mi = uni((uint32_t)__builtin_ctzll(mm));
+            const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)opos, (int)mi);
+            const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)dist, (int)mi);
+            const uint64_t below = (1ull << mi) - 1;
+            if (d > hist0 + pos + m + WIN) {
+                // past the available history: leave it to scalar_step (error)
+                lits &= below;
+                matches &= below;
+                run = m;
+                s = mi;
+                eob = false;
+                break;
+            }
+            if (d + (run - m) > RING) {
+                // a later entry of this round would overwrite the ring slot
+                // this copy still has to read: end the round after the copy
+                const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)packed, (int)mi);
+                lits &= below;
+                matches &= below | (1ull << mi);
+                run = m + (pk >> 9);
+                s = mi + ((pk >> 3) & 63);
+                eob = false;
+                break;
+            }
+        }
+        s = uni(s);
+        run = uni(run);
+        lits = uni64(lits);
+        matches = uni64(matches);
+        if (s == 0) {
+            // the round can take nothing (long code, invalid code, full
+            // output, distance past the history): one symbol the slow way,
+            // which either advances P or ends the block with a status
+            const uint32_t sr = scalar_step<RT>(sh, in, n, P, o, pos, cap, g0, hist0, last_marker_end, err);
+            if (sr != R_CONT) {
+                r = sr;
+                break;
+            }
+        } else {
+            // ---- literals of the round: one LDS write and one HBM store
+            if ((lits >> lane) & 1) {
+                const uint32_t q = pos + opos;
+                sh.ring[(g0 + q) & RMASK] = (RT)(e >> 16);
+                put(o, q, e >> 16);
+            }
+            compiler_fence();
+            // ---- copies, in stream order
+            for (uint64_t mm = matches; mm; mm = uni64(mm & (mm - 1))) {
+                const uint32_t mi = uni((uint32_t)__builtin_ctzll(mm));
+                const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)opos, (int)mi);
+                const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)dist, (int)mi);
+                const uint32_t ln = (uint32_t)__builtin_amdgcn_readlane((int)packed, (int)mi) >> 9;
+                wave_copy<RT>(sh.ring, o, pos + m, g0, ln, d, last_marker_end);
+            }
+            pos += run;
+            P += s;
+            if (eob) {
+                r = R_EOB;
+                break;
+            }
+        }
+        if (o.wide && o.seg0 + pos >= WIN && o.seg0 + pos - last_marker_end >= WIN) {
+            r = R_FLIP;
+            break;
+        }
+        if (pos >= 0x40000000u) {
+            // rebase the segment so positions stay 32-bit
+            o.p = o.p + (o.wide ? 2ull : 1ull) * pos;
+            o.seg0 += pos;
+            o.cap -= pos;
+            g0 += pos;
+            cap = (uint32_t)(o.cap < 0x7FFFFFFFull ? o.cap : 0x7FFFFFFFull);
+            pos = 0;
+        }
+    }
+    o.pos = pos;
+    return r;
 }
 
 // Stored block body: LEN bytes from the input at byte `src_byte`.
-template <typename T>
-__device__ __forceinline__ void stored_copy(const uint8_t *__restrict__ in, uint64_t src_byte, T *out, uint64_t pos,
-                                            uint32_t len) {
+template <typename RT>
+__device__ __attribute__((always_inline)) void stored_copy(gptr<const uint8_t> in, uint64_t src_byte, RT *ring,
+                                                           const OutSeg &o, uint32_t len) {
     const uint32_t lane = lane_id();
-    for (uint32_t k = 0; k < len; k += 8 * WAVE) {
+    const uint64_t gpos = o.seg0 + o.pos;
+    for (uint32_t k = 0; k < len; k += 4 * WAVE) {
+        uint32_t v[4];
 #pragma unroll
-        for (uint32_t j = 0; j < 8; j++) {
+        for (uint32_t j = 0; j < 4; j++) {
             uint32_t i = k + j * WAVE + lane;
-            if (i < len) out[pos + i] = (T)in[src_byte + i];
+            v[j] = i < len ? in[src_byte + i] : 0;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+            uint32_t i = k + j * WAVE + lane;
+            if (i < len) {
+                ring[(uint32_t)(gpos + i) & RMASK] = (RT)v[j];
+                put(o, o.pos + i, v[j]);
+            }
         }
     }
     compiler_fence();
 }
 
-__device__ void inflate_piece(InflateShared &sh, const uint8_t *__restrict__ in, uint64_t in_len,
-                              const PieceDesc &pd, PieceOut *po) {
+template <typename RT>
+__device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &sh, const uint8_t *__restrict__ in,
+                                                             uint64_t in_len, const PieceDesc &pd, PieceOut *po) {
     const uint32_t lane = lane_id();
-    enum { M_DIRECT = 0, M_MARKER = 1, M_CLEAN = 2 };
-    int mode = (pd.flags & PF_MARKER) ? M_MARKER : (pd.flags & PF_WINDOW) ? M_CLEAN : M_DIRECT;
-    uint64_t clean_base = 0;  // entries before the u8 region (marker part, once flipped)
-    uint8_t *dst = pd.dst;
-    uint16_t *out16 = nullptr;
-    uint8_t *out8 = dst;
-    uint64_t pos16 = 0, pos8 = 0, cap16 = 0, cap8 = pd.cap, u8_off = 0;
+    const bool marker = (pd.flags & PF_MARKER) != 0;
+    if (marker && sizeof(RT) == 1) {  // host error: marker pieces need the u16 ring
+        if (lane == 0) {
+            po->status = PS_ERROR;
+            po->err = E_NONE;
+            po->n16 = po->n8 = 0;
+        }
+        return;
+    }
+    gptr<uint8_t> dst = to_global(pd.dst);
+    // marker pieces start with u16 entries; everything else writes u8
+    OutSeg o;
+    o.p = dst;
+    o.pos = 0;
+    o.seg0 = 0;
+    o.wide = marker ? 1u : 0u;
+    o.cap = marker ? pd.cap / 2 : pd.cap;
+    uint64_t n16 = 0, u8_off = 0;
     uint64_t last_marker_end = 0;
     uint32_t status = PS_NONE, err = E_NONE, nblocks = 0;
     uint64_t end_bit = 0;
+    uint64_t t_begin = wall_clock64();
     sh.fixed = 0;
 
-    if (mode == M_MARKER) {
-        // u16 preamble: entry i (oldest first) holds MARKER_BASE + i
-        uint4 *p = reinterpret_cast<uint4 *>(dst);
-        for (uint32_t q = lane; q < WIN / 8; q += WAVE) {
-            uint32_t b = MARKER_BASE + q * 8;
-            uint4 v;
-            v.x = b | (b + 1) << 16;
-            v.y = (b + 2) | (b + 3) << 16;
-            v.z = (b + 4) | (b + 5) << 16;
-            v.w = (b + 6) | (b + 7) << 16;
-            p[q] = v;
-        }
-        out16 = reinterpret_cast<uint16_t *>(dst) + WIN;
-        cap16 = pd.cap >= 2 * WIN ? (pd.cap - 2 * WIN) / 2 : 0;
-        compiler_fence();
+    // ring = the window before the piece (position -32768 + i at slot i)
+    if (marker) {
+        for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[q] = (RT)(MARKER_BASE + q);
+    } else if (pd.flags & PF_WINDOW) {
+        for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[q] = to_global(pd.window)[q];
+    } else {
+        for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[q] = 0;
     }
+    __syncthreads();
 
     BitReader br;
     br.init(in, in_len, pd.start_bit);
     for (;;) {
+        br.make_uniform();
         br.refill();
         uint64_t hbit = br.pos();
         if (hbit + 3 > pd.limit_bit) {
@@ -408,19 +673,17 @@ __device__ void inflate_piece(InflateShared &sh, const uint8_t *__restrict__ in,
                 end_bit = hbit;
                 break;
             }
-            if (mode == M_MARKER) {
-                if (pos16 + len > cap16) r = R_OVERFLOW;
-                else {
-                    stored_copy<uint16_t>(in, src_byte, out16, pos16, len);
-                    pos16 += len;
-                    if (pos16 >= WIN && pos16 - last_marker_end >= WIN) r = R_FLIP;
-                }
+            if (o.pos + len > o.cap) {
+                r = R_OVERFLOW;
             } else {
-                uint64_t &p8 = pos8;
-                if (p8 + len > cap8) r = R_OVERFLOW;
-                else {
-                    stored_copy<uint8_t>(in, src_byte, out8, p8, len);
-                    p8 += len;
+                stored_copy<RT>(to_global(in), src_byte, sh.ring, o, len);
+                o.pos += len;
+                if (o.wide && o.seg0 + o.pos >= WIN && o.seg0 + o.pos - last_marker_end >= WIN) r = R_FLIP;
+                if (o.pos >= 0x40000000u) {  // keep segment positions 32-bit
+                    o.p = o.p + (o.wide ? 2ull : 1ull) * o.pos;
+                    o.seg0 += o.pos;
+                    o.cap -= o.pos;
+                    o.pos = 0;
                 }
             }
             br.init(in, in_len, (src_byte + len) * 8);
@@ -437,50 +700,31 @@ __device__ void inflate_piece(InflateShared &sh, const uint8_t *__restrict__ in,
                     break;
                 }
             }
-            for (;;) {
-                if (mode == M_MARKER)
-                    r = decode_symbols<uint16_t, true, false>(sh, br, out16, pos16, cap16, 0, pd.limit_bit,
-                                                               last_marker_end, err);
-                else if (mode == M_CLEAN)
-                    r = decode_symbols<uint8_t, false, false>(sh, br, out8, pos8, cap8, clean_base, pd.limit_bit,
-                                                              last_marker_end, err);
-                else
-                    r = decode_symbols<uint8_t, false, true>(sh, br, out8, pos8, cap8, 0, pd.limit_bit,
-                                                             last_marker_end, err);
-                if (r != R_FLIP) break;
-                // flip: narrow the last 32 KiB of u16 entries into the u8 preamble
-                u8_off = ((2 * (uint64_t)WIN + 2 * pos16) + 255) & ~255ull;
-                if (u8_off + WIN + 258 > pd.cap) {
-                    r = R_OVERFLOW;
-                    break;
-                }
-                const uint16_t *srcw = out16 + pos16 - WIN;
-                uint8_t *dw = dst + u8_off;
-                for (uint32_t q = lane; q < WIN; q += WAVE) dw[q] = (uint8_t)srcw[q];
-                compiler_fence();
-                out8 = dw + WIN;
-                cap8 = pd.cap - u8_off - WIN;
-                pos8 = 0;
-                clean_base = pos16;
-                mode = M_CLEAN;
-            }
+            uint64_t P = br.pos();
+            r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err);
+            br.init(in, in_len, P);
         }
         if (r == R_FLIP) {
-            // stored block ended a marker run: same flip as above
-            u8_off = ((2 * (uint64_t)WIN + 2 * pos16) + 255) & ~255ull;
-            if (u8_off + WIN + 258 > pd.cap) {
+            // the ring already holds the last 32768 entries, all literal:
+            // continue with u8 entries after the u16 ones
+            n16 = o.seg0 + o.pos;
+            u8_off = (2 * n16 + 255) & ~255ull;
+            if (u8_off >= pd.cap) {
                 r = R_OVERFLOW;
             } else {
-                const uint16_t *srcw = out16 + pos16 - WIN;
-                uint8_t *dw = dst + u8_off;
-                for (uint32_t q = lane; q < WIN; q += WAVE) dw[q] = (uint8_t)srcw[q];
-                compiler_fence();
-                out8 = dw + WIN;
-                cap8 = pd.cap - u8_off - WIN;
-                pos8 = 0;
-                clean_base = pos16;
-                mode = M_CLEAN;
-                r = R_EOB;
+                o.p = dst + u8_off;
+                o.seg0 = n16;
+                o.cap = pd.cap - u8_off;
+                o.pos = 0;
+                o.wide = 0;
+                // a flip inside a block resumes that block's symbols
+                if (btype != 0) {
+                    uint64_t P = br.pos();
+                    r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err);
+                    br.init(in, in_len, P);
+                } else {
+                    r = R_EOB;
+                }
             }
         }
         if (r == R_OVERFLOW) {
@@ -505,27 +749,35 @@ __device__ void inflate_piece(InflateShared &sh, const uint8_t *__restrict__ in,
             break;
         }
     }
+    __syncthreads();
     if (lane == 0) {
+        const bool flipped = marker && o.wide == 0;
         po->end_bit = end_bit;
-        po->n16 = pos16;
-        po->n8 = pos8;
-        po->u8_off = u8_off;
+        const uint64_t total = o.seg0 + o.pos;
+        po->n16 = marker ? (flipped ? n16 : total) : 0;
+        po->n8 = marker ? (flipped ? total - n16 : 0) : total;
+        po->u8_off = flipped ? u8_off : 0;
         po->bits_done = br.pos() - pd.start_bit;
+        po->t_begin = t_begin;
+        po->t_end = wall_clock64();
         po->status = status;
         po->err = err;
         po->nblocks = nblocks;
-        po->pad = (uint32_t)mode;
+        po->mode = marker ? (flipped ? 2u : 1u) : 0u;
     }
 }
 
 // Persistent inflate kernel: every workgroup (one wavefront) pulls piece
-// indices from a device queue until it is drained.  `order` lists pieces
-// largest-first so the long pieces start early.
+// indices from a device queue until it is drained; `order` lists the pieces
+// longest-first.  RT = ring entry type: uint16_t when any piece decodes with
+// markers (64 KiB ring, 2 waves per CU), uint8_t for window-known pieces only
+// (32 KiB ring, 4 waves per CU).
+template <typename RT>
 __global__ void __launch_bounds__(64) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
                                                 const PieceDesc *__restrict__ pieces,
-                                                const uint32_t *__restrict__ order, uint32_t n,
-                                                uint32_t *queue, PieceOut *out) {
-    __shared__ InflateShared sh;
+                                                const uint32_t *__restrict__ order, uint32_t n, uint32_t *queue,
+                                                PieceOut *out) {
+    __shared__ InflateShared<RT> sh;
     for (;;) {
         if (lane_id() == 0) sh.qi = atomicAdd(queue, 1u);
         __syncthreads();
@@ -534,7 +786,7 @@ __global__ void __launch_bounds__(64) k_inflate(const uint8_t *__restrict__ in, 
         if (qi >= n) break;
         uint32_t pi = order[qi];
         PieceDesc pd = pieces[pi];
-        inflate_piece(sh, in, in_len, pd, out + pi);
+        inflate_piece<RT>(sh, in, in_len, pd, out + pi);
         __syncthreads();
     }
 }

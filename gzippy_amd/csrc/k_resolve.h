@@ -33,10 +33,10 @@ struct ResolveTile {
 
 __device__ __forceinline__ uint8_t piece_entry(const ResolvePiece &r, uint64_t t, const uint8_t *win) {
     if (t < r.n16) {
-        uint32_t e = reinterpret_cast<const uint16_t *>(r.slab)[WIN + t];
+        uint32_t e = ((gptr<const uint16_t>)to_global(r.slab))[t];
         return e < 256 ? (uint8_t)e : win[e - MARKER_BASE];
     }
-    return r.slab[r.u8_off + WIN + (t - r.n16)];
+    return to_global(r.slab)[r.u8_off + (t - r.n16)];
 }
 
 __global__ void __launch_bounds__(1024) k_chain(const ResolvePiece *__restrict__ rp, const uint32_t *__restrict__ chains,
@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(1024) k_chain(const ResolvePiece *__restrict__
             const ResolvePiece f = rp[p0];
             for (uint32_t j = threadIdx.x; j < WIN; j += blockDim.x) {
                 int64_t q = (int64_t)f.out_pos - (int64_t)WIN + j;
-                win[0][j] = q >= 0 ? f.mout[q] : 0;
+                win[0][j] = q >= 0 ? to_global(f.mout)[q] : 0;
             }
         }
         __syncthreads();
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(1024) k_chain(const ResolvePiece *__restrict__
                     v = win[cur][WIN + t];
                 } else {
                     v = piece_entry(r, (uint64_t)t, win[cur]);
-                    r.mout[r.out_pos + t] = v;
+                    to_global(r.mout)[r.out_pos + t] = v;
                 }
                 win[cur ^ 1][j] = v;
             }
@@ -79,9 +79,11 @@ __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__rest
     for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
         const ResolveTile tl = tiles[b];
         const ResolvePiece r = rp[tl.piece];
-        const uint8_t *wbase = r.mout + r.out_pos;  // window = wbase[-WIN .. 0)
+        gptr<const uint8_t> wbase = to_global((const uint8_t *)r.mout) + r.out_pos;  // window = wbase[-WIN .. 0)
         const bool near_start = r.out_pos < WIN;
-        const uint16_t *e16 = reinterpret_cast<const uint16_t *>(r.slab) + WIN;
+        gptr<const uint16_t> e16 = (gptr<const uint16_t>)to_global(r.slab);
+        gptr<const uint8_t> s8 = to_global(r.slab);
+        gptr<uint8_t> mout = to_global(r.mout);
         for (uint64_t t = tl.t0 + threadIdx.x; t < tl.t1; t += blockDim.x) {
             uint8_t v;
             if (t < r.n16) {
@@ -92,9 +94,9 @@ __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__rest
                     v = (near_start && (int64_t)r.out_pos + q < 0) ? 0 : wbase[q];
                 }
             } else {
-                v = r.slab[r.u8_off + WIN + (t - r.n16)];
+                v = s8[r.u8_off + (t - r.n16)];
             }
-            r.mout[r.out_pos + t] = v;
+            mout[r.out_pos + t] = v;
         }
     }
 }
