@@ -51,6 +51,25 @@ def cpu_baseline(gz, raw_size, budget_s=12.0, max_runs=3):
                       f"(restatement of drive_thin_t1_oracle), median of {len(times)} runs"}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_inflate_hbm_traffic_1gib_text_l6.md")
+
+
+def measured_traffic(args):
+    """HBM bytes per k_inflate launch, from the committed FETCH_SIZE / WRITE_SIZE
+    summary of this exact workload (default size, level and chunk grid)."""
+    if (args.size_mib, args.level, args.chunk_kib, args.pieces_per_chunk) != (1024, 6, 512, 0):
+        return None
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return {"bytes_per_launch": int(ln.split(":", 1)[1].strip()),
+                            "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +81,9 @@ def main():
     ap.add_argument("--pieces-per-chunk", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0x675A0001)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the timed steps, run one plain device-to-device copy of this many MiB "
+                         "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -141,6 +163,18 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "algorithmic_bytes_per_step": int(C + D), "kernel_ms_per_step": round(infl_ms_step, 3)},
     }
+    traffic = measured_traffic(args)
+    if traffic is not None:
+        line["roofline"]["traffic"] = traffic["bytes_per_launch"]
+        line["roofline"]["traffic_source"] = traffic["source"]
+    if args.calibrate_copy_mib:
+        nb = args.calibrate_copy_mib << 20
+        a = torch.empty(nb, dtype=torch.uint8, device=dev).fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nb} B")
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(gz, n)
