@@ -176,6 +176,48 @@ __global__ void k_gather_envelope(const uint8_t *__restrict__ in, const uint64_t
 
 }  // namespace
 
+// Pinned host staging for the small per-call uploads and downloads (piece
+// descriptors, resolve lists, CRC segment lists, results).  A copy from
+// pageable memory makes the host wait for the stream; from pinned memory it
+// is a true async DMA, so the host keeps enqueueing while kernels run.  Bump
+// allocation; reset() only after the stream has drained.
+struct PinnedArena {
+    std::vector<std::pair<uint8_t *, size_t>> blocks;
+    size_t used = 0;
+    void *alloc(size_t n) {
+        n = (n + 63) & ~(size_t)63;
+        if (blocks.empty() || used + n > blocks.back().second) {
+            size_t cap = std::max<size_t>(n, 1 << 20);
+            if (!blocks.empty()) cap = std::max(cap, 2 * blocks.back().second);
+            void *p = nullptr;
+            if (hipHostMalloc(&p, cap, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                throw StatusError{GZGPU_GPU_UNAVAILABLE, "hipHostMalloc(" + std::to_string(cap) + ") failed"};
+            }
+            blocks.push_back({(uint8_t *)p, cap});
+            used = 0;
+        }
+        void *r = blocks.back().first + used;
+        used += n;
+        return r;
+    }
+    // caller guarantees no copy from or to the arena is in flight
+    void reset() {
+        if (blocks.size() > 1) {
+            size_t want = 0;
+            for (auto &b : blocks) want += b.second;
+            release();
+            alloc(want);
+        }
+        used = 0;
+    }
+    void release() {
+        for (auto &b : blocks) (void)hipHostFree(b.first);
+        blocks.clear();
+        used = 0;
+    }
+};
+
 // ---------------------------------------------------------------- context
 
 struct Member {
@@ -210,6 +252,7 @@ struct gzgpu_ctx {
     DevBuf d_in, d_out, d_pieces, d_pout, d_order, d_counters, d_surv, d_hits, d_slab, d_crc_tab, d_seg_off,
         d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len;
     std::vector<DevBuf> extra_slabs;
+    PinnedArena pin;
     hipEvent_t ev[10]{};
     gzgpu_stats st{};
 };
@@ -231,7 +274,18 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
 
 void upload(gzgpu_ctx *c, DevBuf &b, const void *src, size_t bytes) {
     b.ensure(bytes);
-    if (bytes) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    if (!bytes) return;
+    void *h = c->pin.alloc(bytes);
+    memcpy(h, src, bytes);
+    HIPCHK(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, c->stream));
+}
+
+// async device-to-host copy into pinned staging; valid after a stream sync
+template <typename T>
+T *download(gzgpu_ctx *c, const void *d, size_t count) {
+    T *h = (T *)c->pin.alloc(count * sizeof(T));
+    if (count) HIPCHK(hipMemcpyAsync(h, d, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    return h;
 }
 
 // ---- block finder over [start_bit, end_bit) of d_in -----------------------
@@ -339,9 +393,9 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     c->st.launches++;
-    std::vector<PieceOut> ro(descs.size());
-    HIPCHK(hipMemcpyAsync(ro.data(), c->d_pout.p, ro.size() * sizeof(PieceOut), hipMemcpyDeviceToHost, c->stream));
+    const PieceOut *hro = download<PieceOut>(c, c->d_pout.p, descs.size());
     HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<PieceOut> ro(hro, hro + descs.size());
     float ms = ev_ms(c->ev[5], c->ev[6]);
     c->st.ms_inflate_kernel += ms;
     c->st.inflate_launches++;
@@ -591,7 +645,12 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
                     M.msg = "output capacity exceeded";
                     break;
                 }
-                if (!chain_open) {
+                // a piece that flipped to u8 holds at least 32768 entries and
+                // its last 32768 are marker-free: its end window needs no
+                // predecessor, so a new chain starts there (chains run in
+                // parallel, one workgroup each)
+                const bool self_windowed = r.u8_off != 0 && L >= WIN;
+                if (!chain_open || self_windowed) {
                     chains.push_back((uint32_t)rps.size());
                     chain_open = true;
                 }
@@ -699,8 +758,7 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
                            c->d_mlen.as<uint64_t>(), nm, c->d_mcrc.as<uint32_t>());
         HIPCHK(hipGetLastError());
         c->st.launches++;
-        std::vector<uint32_t> crcs(nm);
-        HIPCHK(hipMemcpyAsync(crcs.data(), c->d_mcrc.p, nm * 4, hipMemcpyDeviceToHost, c->stream));
+        const uint32_t *crcs = download<uint32_t>(c, c->d_mcrc.p, nm);
         HIPCHK(hipEventRecord(c->ev[4], c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (uint32_t i = 0; i < nm; i++) {
@@ -760,6 +818,11 @@ int finish(gzgpu_ctx *c, int st, const std::string &msg) {
 template <typename F>
 int guarded(gzgpu_ctx *c, F &&f) {
     try {
+        // every call starts with an idle stream, so staging memory is free
+        if (c && c->stream) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            c->pin.reset();
+        }
         return f();
     } catch (const HipError &e) {
         set_err(c, std::string("HIP error: ") + hipGetErrorString(e.e) + " in " + e.what);
@@ -908,6 +971,7 @@ void gzgpu_ctx_destroy(gzgpu_ctx *c) {
                       &c->d_mcrc, &c->d_rp, &c->d_tiles, &c->d_chains, &c->d_env, &c->d_env_off, &c->d_env_len})
         b->release();
     for (auto &b : c->extra_slabs) b.release();
+    c->pin.release();
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -8,11 +8,14 @@
 // with window = the 32768 final output bytes before the piece (zero-padded
 // at the member start, WindowMap{0: zeros}).
 //
-// k_chain walks one member's marker pieces in order, keeping the rolling
-// 32 KiB window in LDS, and writes each piece's last 32 KiB (the end window
-// its successor needs: get_last_window, chunk_data.rs:1295).  k_resolve_body
-// then resolves every other entry in parallel, reading windows from the now
-// final output.
+// k_chain walks a chain of marker pieces in order, keeping the rolling 32 KiB
+// window in LDS, and writes each piece's last 32 KiB (the end window its
+// successor needs: get_last_window, chunk_data.rs:1295).  A chain starts at
+// the member's first marker piece and at every piece that flipped to u8
+// entries (its last 32 KiB are literal, so its end window depends on nothing
+// before it); chains are independent workgroups.  k_resolve_body then
+// resolves every other entry in parallel, reading windows from the now final
+// output.
 #pragma once
 #include "gz_common.h"
 
@@ -46,6 +49,10 @@ __global__ void __launch_bounds__(1024) k_chain(const ResolvePiece *__restrict__
         const uint32_t p0 = chains[c], p1 = chains[c + 1];
         {
             const ResolvePiece f = rp[p0];
+            // a flipped first piece (u8 entries present, >= 32768 entries)
+            // never reads the window before it: leave it unloaded, since that
+            // output may still be being written by another chain
+            if (!(f.u8_off != 0 && f.n16 + f.n8 >= WIN))
             for (uint32_t j = threadIdx.x; j < WIN; j += blockDim.x) {
                 int64_t q = (int64_t)f.out_pos - (int64_t)WIN + j;
                 win[0][j] = q >= 0 ? to_global(f.mout)[q] : 0;
