@@ -36,7 +36,9 @@ class Stats(ctypes.Structure):
                 ("compressed_bytes", ctypes.c_uint64), ("decompressed_bytes", ctypes.c_uint64),
                 ("pieces", ctypes.c_uint32), ("redecoded", ctypes.c_uint32), ("candidates", ctypes.c_uint32),
                 ("launches", ctypes.c_uint32), ("ms_inflate_kernel", ctypes.c_double),
-                ("inflate_launches", ctypes.c_uint32), ("pad_", ctypes.c_uint32)]
+                ("inflate_launches", ctypes.c_uint32), ("pad_", ctypes.c_uint32),
+                ("ms_inflate_marker", ctypes.c_double), ("ms_inflate_clean", ctypes.c_double),
+                ("clean_in_bytes", ctypes.c_uint64), ("clean_out_bytes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

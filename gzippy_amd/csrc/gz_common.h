@@ -18,8 +18,6 @@ namespace gz {
 constexpr int WAVE = 64;
 constexpr uint32_t WIN = 32768;            // DEFLATE window (RFC 1951 3.2)
 constexpr uint32_t MARKER_BASE = 32768;     // replace_markers.rs:23
-constexpr int LL_ROOT = 11;                 // litlen LUT root bits
-constexpr int D_ROOT = 10;                  // distance LUT root bits
 constexpr int PRE_ROOT = 7;                 // precode LUT covers every precode length
 
 // Piece status (device → host)
@@ -29,6 +27,7 @@ enum : uint32_t {
     PS_FINAL = 2,     // decoded the final (BFINAL) block
     PS_OVERFLOW = 3,  // slab / output capacity exhausted
     PS_ERROR = 4,     // invalid stream from this start
+    PS_FLIPPED = 5,   // marker piece handed over to the u8 kernel at its flip
 };
 // error sub-codes (PieceOut::err)
 enum : uint32_t {

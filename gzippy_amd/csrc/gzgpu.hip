@@ -237,7 +237,7 @@ struct Member {
 };
 
 struct PieceHost {
-    PieceDesc d;
+    PieceDesc d{};
     uint64_t acc_lo = 0, acc_hi = 0;  // accepted predecessor end range
     uint32_t member = 0;
     bool confirmed = false, dropped = false;
@@ -355,67 +355,128 @@ bool debug_enabled() {
     return v == 1;
 }
 
-// One persistent inflate launch over `todo`; results land in R.
-void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<PieceHost> &P,
-                    const std::vector<Member> &mem, std::vector<uint32_t> &todo, std::vector<PieceOut> &R) {
-    // longest first
-    std::sort(todo.begin(), todo.end(), [&](uint32_t a, uint32_t b) {
-        uint64_t ea = (P[a].d.stop_bit == NO_STOP ? mem[P[a].member].de : P[a].d.stop_bit) - P[a].d.start_bit;
-        uint64_t eb = (P[b].d.stop_bit == NO_STOP ? mem[P[b].member].de : P[b].d.stop_bit) - P[b].d.start_bit;
-        return ea > eb;
-    });
-    bool any_marker = false;
-    std::vector<PieceDesc> descs(todo.size());
-    for (size_t i = 0; i < todo.size(); i++) {
-        descs[i] = P[todo[i]].d;
-        any_marker |= (descs[i].flags & PF_MARKER) != 0;
-    }
-    std::vector<uint32_t> order(todo.size());
-    for (uint32_t i = 0; i < todo.size(); i++) order[i] = i;
+// One launch of the persistent inflate kernel over `descs` (pulled
+// longest-first by `cost`); returns one PieceOut per desc.
+template <typename RT>
+std::vector<PieceOut> run_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len,
+                                  const std::vector<PieceDesc> &descs, const std::vector<uint64_t> &cost, float &ms) {
+    const uint32_t n = (uint32_t)descs.size();
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
     upload(c, c->d_pieces, descs.data(), descs.size() * sizeof(PieceDesc));
     upload(c, c->d_order, order.data(), order.size() * 4);
     c->d_pout.ensure(descs.size() * sizeof(PieceOut));
     c->d_counters.ensure(64);
     HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
-    // LDS bounds residency (160 KiB per CU): u16 ring -> 2 waves/CU, u8 ring -> 3
-    uint32_t per_cu = any_marker ? 2 : 3;
+    // LDS bounds residency (160 KiB per CU): u16 ring -> 2 waves/CU, u8 ring -> 4
+    uint32_t per_cu = sizeof(RT) == 2 ? 2 : 4;
     if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
-    uint32_t grid = (uint32_t)std::min<uint64_t>(descs.size(), (uint64_t)c->num_cus * per_cu);
+    uint32_t grid = (uint32_t)std::min<uint64_t>(n, (uint64_t)c->num_cus * per_cu);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    if (any_marker)
-        hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(grid), dim3(64), 0, c->stream, d_in, in_len,
-                           c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), (uint32_t)descs.size(),
-                           c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
-    else
-        hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(grid), dim3(64), 0, c->stream, d_in, in_len,
-                           c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), (uint32_t)descs.size(),
-                           c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
+    hipLaunchKernelGGL(k_inflate<RT>, dim3(grid), dim3(64), 0, c->stream, d_in, in_len, c->d_pieces.as<PieceDesc>(),
+                       c->d_order.as<uint32_t>(), n, c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     c->st.launches++;
-    const PieceOut *hro = download<PieceOut>(c, c->d_pout.p, descs.size());
+    const PieceOut *hro = download<PieceOut>(c, c->d_pout.p, n);
     HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<PieceOut> ro(hro, hro + descs.size());
-    float ms = ev_ms(c->ev[5], c->ev[6]);
+    std::vector<PieceOut> ro(hro, hro + n);
+    ms = ev_ms(c->ev[5], c->ev[6]);
     c->st.ms_inflate_kernel += ms;
     c->st.inflate_launches++;
-    for (size_t i = 0; i < todo.size(); i++) R[todo[i]] = ro[i];
     if (debug_enabled()) {
         std::vector<double> us;
-        uint64_t t0 = ~0ull, t1 = 0, bytes = 0;
+        uint64_t t0 = ~0ull, t1 = 0, bytes = 0, flipped = 0, n16 = 0;
         for (auto &r : ro) {
             us.push_back((double)(r.t_end - r.t_begin) / 100.0);
             t0 = std::min(t0, r.t_begin);
             t1 = std::max(t1, r.t_end);
             bytes += r.n16 + r.n8;
+            n16 += r.n16;
+            flipped += r.status == PS_FLIPPED;
         }
         std::sort(us.begin(), us.end());
         fprintf(stderr,
-                "[gzgpu] inflate<%s> pieces=%zu grid=%u kernel=%.3f ms span=%.3f ms piece_us min=%.1f p50=%.1f "
-                "max=%.1f entries=%llu per-wave MB/s p50=%.1f\n",
-                any_marker ? "u16" : "u8", ro.size(), grid, ms, (double)(t1 - t0) / 1e5, us.front(),
-                us[us.size() / 2], us.back(), (unsigned long long)bytes,
-                us[us.size() / 2] > 0 ? (double)bytes / ro.size() / us[us.size() / 2] : 0.0);
+                "[gzgpu] inflate<%s> pieces=%u grid=%u kernel=%.3f ms span=%.3f ms piece_us min=%.1f p50=%.1f "
+                "max=%.1f entries=%llu (u16 %llu) flipped=%llu per-wave MB/s p50=%.1f\n",
+                sizeof(RT) == 2 ? "u16" : "u8", n, grid, ms, (double)(t1 - t0) / 1e5, us.front(), us[us.size() / 2],
+                us.back(), (unsigned long long)bytes, (unsigned long long)n16, (unsigned long long)flipped,
+                us[us.size() / 2] > 0 ? (double)bytes / n / us[us.size() / 2] : 0.0);
+    }
+    return ro;
+}
+
+// Inflate the pieces in `todo`; results land in R.  Marker pieces run in the
+// u16-ring kernel until their flip; the rest of every flipped piece and all
+// window-known pieces then run in the u8-ring kernel at twice the occupancy.
+void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<PieceHost> &P,
+                    const std::vector<Member> &mem, std::vector<uint32_t> &todo, std::vector<PieceOut> &R) {
+    auto stop_of = [&](const PieceDesc &d) { return d.stop_bit == NO_STOP ? mem[d.member].de : d.stop_bit; };
+    std::vector<uint32_t> marker_idx, clean_idx;
+    for (uint32_t pi : todo) ((P[pi].d.flags & PF_MARKER) ? marker_idx : clean_idx).push_back(pi);
+    if (!marker_idx.empty()) {
+        std::vector<PieceDesc> d1;
+        std::vector<uint64_t> cost;
+        for (uint32_t pi : marker_idx) {
+            PieceDesc d = P[pi].d;
+            d.flags |= PF_STOP_AT_FLIP;
+            d1.push_back(d);
+            cost.push_back(stop_of(d) - d.start_bit);
+        }
+        float ms = 0;
+        std::vector<PieceOut> r1 = run_inflate<uint16_t>(c, d_in, in_len, d1, cost, ms);
+        c->st.ms_inflate_marker += ms;
+        for (size_t i = 0; i < marker_idx.size(); i++) R[marker_idx[i]] = r1[i];
+    }
+    std::vector<PieceDesc> d2;
+    std::vector<uint32_t> who;
+    std::vector<uint64_t> cost, from;
+    for (uint32_t pi : clean_idx) {
+        d2.push_back(P[pi].d);
+        who.push_back(pi);
+        cost.push_back(stop_of(P[pi].d) - P[pi].d.start_bit);
+        from.push_back(P[pi].d.start_bit);
+    }
+    for (uint32_t pi : marker_idx) {
+        const PieceOut &r = R[pi];
+        if (r.status != PS_FLIPPED) continue;
+        const PieceDesc &m = P[pi].d;
+        PieceDesc d = m;
+        d.flags = PF_RESUME;
+        d.start_bit = r.in_block ? r.hdr_bit : r.end_bit;
+        d.resume_bit = r.in_block ? r.end_bit : 0;
+        d.seg0 = r.n16;
+        d.window = m.dst + 2 * (r.n16 - WIN);  // the last 32768 u16 entries, all literal
+        d.dst = m.dst + r.u8_off;
+        d.cap = m.cap - r.u8_off;
+        d2.push_back(d);
+        who.push_back(pi);
+        cost.push_back(stop_of(m) - r.end_bit);
+        from.push_back(r.end_bit);
+    }
+    if (d2.empty()) return;
+    float ms = 0;
+    std::vector<PieceOut> r2 = run_inflate<uint8_t>(c, d_in, in_len, d2, cost, ms);
+    c->st.ms_inflate_clean += ms;
+    for (size_t i = 0; i < d2.size(); i++) {
+        const uint32_t pi = who[i];
+        const PieceOut &q = r2[i];
+        c->st.clean_in_bytes += (q.end_bit > from[i] ? q.end_bit - from[i] : 0) / 8;
+        c->st.clean_out_bytes += q.n8;
+        if (!(P[pi].d.flags & PF_MARKER)) {
+            R[pi] = q;
+            continue;
+        }
+        PieceOut &r = R[pi];
+        r.end_bit = q.end_bit;
+        r.n8 = q.n8;
+        r.bits_done = q.bits_done + (d2[i].start_bit - P[pi].d.start_bit);
+        r.t_end = q.t_end;
+        r.status = q.status;
+        r.err = q.err;
+        r.nblocks += q.nblocks;
+        r.mode = 2;
     }
 }
 
@@ -423,11 +484,11 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
 void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
     const uint64_t chunk_bits = (c->tune.chunk_bytes ? c->tune.chunk_bytes : 512 * KIB) * 8;
     // wavefronts per chunk: explicit, or enough pieces to fill every resident
-    // inflate slot (2 per CU with the u16 ring) across the split members
+    // slot of the u8-ring inflate kernel (4 per CU) across the split members
     uint64_t split_bits = 0;
     for (auto &M : mem)
         if (M.status == GZGPU_OK && M.split) split_bits += M.de - M.ds;
-    uint64_t slots = (uint64_t)c->num_cus * 2;
+    uint64_t slots = (uint64_t)c->num_cus * 4;
     uint64_t ppc = c->tune.pieces_per_chunk;
     if (!ppc) {
         uint64_t chunks = std::max<uint64_t>(1, (split_bits + chunk_bits - 1) / chunk_bits);

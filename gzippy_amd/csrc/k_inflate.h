@@ -22,6 +22,18 @@
 //            32768 entries hold no marker (WidthRing::should_flip,
 //            width_ring.rs:310); from then on the output is u8.
 //
+//   RESUME : the rest of a MARKER piece after its flip.  The window is the
+//            piece's last 32768 u16 entries (all literal), the ring is
+//            aligned to the entries already produced (seg0), and decoding
+//            restarts at the saved block header (tables rebuilt) and jumps to
+//            the saved symbol bit.
+//
+// Two kernels split a marker piece at its flip: the u16-ring kernel
+// (64 KiB ring, 2 wavefronts per CU) decodes until the flip (STOP_AT_FLIP)
+// and the u8-ring kernel (32 KiB ring, 4 wavefronts per CU) decodes the rest
+// together with the DIRECT pieces, so most of the output is produced at the
+// higher occupancy.
+//
 // Slab layout of a MARKER piece (bytes): [u16 entries: 2*n16] (256-aligned)
 // [u8 entries: n8].
 #pragma once
@@ -29,15 +41,17 @@
 
 namespace gz {
 
-enum : uint32_t { PF_DIRECT = 1, PF_MARKER = 2, PF_WINDOW = 4 };
+enum : uint32_t { PF_DIRECT = 1, PF_MARKER = 2, PF_WINDOW = 4, PF_RESUME = 8, PF_STOP_AT_FLIP = 16 };
 
 struct PieceDesc {
-    uint64_t start_bit;      // first block header to decode
+    uint64_t start_bit;      // first block header to decode (RESUME: header of the resumed block)
     uint64_t stop_bit;       // stop before a non-final block header at >= stop_bit
     uint64_t limit_bit;      // end of the member's readable deflate bits
-    uint8_t *dst;            // DIRECT/WINDOW: output bytes; MARKER: slab base
-    uint64_t cap;            // DIRECT/WINDOW: output capacity (bytes); MARKER: slab bytes
-    const uint8_t *window;   // WINDOW: 32768 bytes, oldest first
+    uint8_t *dst;            // DIRECT/WINDOW/RESUME: output bytes; MARKER: slab base
+    uint64_t cap;            // DIRECT/WINDOW/RESUME: output capacity (bytes); MARKER: slab bytes
+    const uint8_t *window;   // WINDOW: 32768 bytes; RESUME: 32768 u16 entries; oldest first
+    uint64_t resume_bit;     // RESUME inside a block: bit of the next symbol (0: start at a header)
+    uint64_t seg0;           // RESUME: entries the piece produced before dst
     uint32_t flags;
     uint32_t member;
 };
@@ -49,10 +63,13 @@ struct PieceOut {
     uint64_t u8_off;    // slab byte offset of the u8 entries
     uint64_t bits_done; // bits consumed when the piece stopped (sizing on overflow)
     uint64_t t_begin, t_end;  // wall clock (100 MHz) around the piece
+    uint64_t hdr_bit;   // PS_FLIPPED inside a block: that block's header bit
     uint32_t status;
     uint32_t err;
     uint32_t nblocks;
     uint32_t mode;
+    uint32_t in_block;  // PS_FLIPPED: 1 = resume at end_bit inside the block at hdr_bit
+    uint32_t pad;
 };
 
 constexpr uint32_t RING = WIN;
@@ -60,9 +77,14 @@ constexpr uint32_t RMASK = RING - 1;
 
 template <typename RT>
 struct __align__(16) InflateShared {
+    // LUT root bits: the u16-ring kernel runs 2 wavefronts per CU and has
+    // room for 11/10; the u8-ring kernel keeps 10/9 so that four fit in a
+    // CU's 160 KiB of LDS (about 40 KB each)
+    static constexpr int LLR = sizeof(RT) == 2 ? 11 : 10;
+    static constexpr int DR = sizeof(RT) == 2 ? 10 : 9;
     RT ring[RING];
-    uint32_t ll[1 << LL_ROOT];
-    uint32_t dl[1 << D_ROOT];
+    uint32_t ll[1 << LLR];
+    uint32_t dl[1 << DR];
     uint16_t ll_sorted[288];
     uint16_t d_sorted[32];
     uint16_t ll_count[16];
@@ -154,10 +176,10 @@ __device__ void build_fixed(SH &sh) {
     const uint32_t lane = lane_id();
     for (uint32_t s = lane; s < 288; s += WAVE) sh.lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
     __syncthreads();
-    build_table<0>(sh, sh.lens, 288, sh.ll, LL_ROOT, sh.ll_count, sh.ll_sorted);
+    build_table<0>(sh, sh.lens, 288, sh.ll, SH::LLR, sh.ll_count, sh.ll_sorted);
     for (uint32_t s = lane; s < 32; s += WAVE) sh.lens[s] = 5;
     __syncthreads();
-    build_table<1>(sh, sh.lens, 32, sh.dl, D_ROOT, sh.d_count, sh.d_sorted);
+    build_table<1>(sh, sh.lens, 32, sh.dl, SH::DR, sh.d_count, sh.d_sorted);
 }
 
 // read_dynamic_huffman_coding (marker_inflate.rs:1206-1245) and
@@ -218,8 +240,8 @@ __device__ __attribute__((always_inline)) uint32_t read_dynamic(SH &sh, BitReade
     }
     __syncthreads();
     if (sh.lens[256] == 0) return E_EOB_ZERO;
-    if (build_table<0>(sh, sh.lens, (int)hlit, sh.ll, LL_ROOT, sh.ll_count, sh.ll_sorted)) return E_LL_TABLE;
-    if (build_table<1>(sh, sh.lens + hlit, (int)hdist, sh.dl, D_ROOT, sh.d_count, sh.d_sorted)) return E_D_TABLE;
+    if (build_table<0>(sh, sh.lens, (int)hlit, sh.ll, SH::LLR, sh.ll_count, sh.ll_sorted)) return E_LL_TABLE;
+    if (build_table<1>(sh, sh.lens + hlit, (int)hdist, sh.dl, SH::DR, sh.d_count, sh.d_sorted)) return E_D_TABLE;
     return E_NONE;
 }
 
@@ -342,7 +364,7 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
     const uint32_t lane = lane_id();
     uint64_t W0, W1;
     window128(in, n, P, W0, W1);
-    uint32_t e = uni(sh.ll[(uint32_t)W0 & ((1u << LL_ROOT) - 1)]);
+    uint32_t e = uni(sh.ll[(uint32_t)W0 & ((1u << SH::LLR) - 1)]);
     if ((e & 15) == 0) {
         uint32_t sl;
         int sy = (int)uni((uint32_t)slow_decode((uint32_t)W0 & 0x7FFF, sh.ll_count, sh.ll_sorted, sl));
@@ -378,7 +400,7 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
     if (pos + len > cap) return R_OVERFLOW;
     const uint32_t off = L + E;
     const uint64_t wd = (W0 >> off) | (off ? W1 << (64 - off) : 0);
-    uint32_t d = uni(sh.dl[(uint32_t)wd & ((1u << D_ROOT) - 1)]);
+    uint32_t d = uni(sh.dl[(uint32_t)wd & ((1u << SH::DR) - 1)]);
     if ((d & 15) == 0) {
         uint32_t sl;
         int sy = (int)uni((uint32_t)slow_decode((uint32_t)wd & 0x7FFF, sh.d_count, sh.d_sorted, sl));
@@ -437,14 +459,14 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
         window128(in, n, P, W0, W1);
         // ---- per lane: the symbol that would start at bit P + lane
         const uint64_t ch = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
-        const uint32_t e = sh.ll[(uint32_t)ch & ((1u << LL_ROOT) - 1)];
+        const uint32_t e = sh.ll[(uint32_t)ch & ((1u << SH::LLR) - 1)];
         const uint32_t L = e & 15, kind = (e >> 8) & 3, E = (e >> 4) & 15;
         uint32_t ok = (L != 0) & (kind != K_BAD);
         uint32_t tot = L, olen = kind == K_EOB ? 0u : 1u, dist = 0;
         if (kind == K_LEN) {
             const uint32_t off = L + E;
             olen = (e >> 16) + ((uint32_t)(ch >> L) & ((1u << E) - 1));
-            const uint32_t d = sh.dl[(uint32_t)(ch >> off) & ((1u << D_ROOT) - 1)];
+            const uint32_t d = sh.dl[(uint32_t)(ch >> off) & ((1u << SH::DR) - 1)];
             const uint32_t DL = d & 15, DE = (d >> 4) & 15;
             ok &= (DL != 0) & (((d >> 8) & 3) == K_LEN);
             dist = (d >> 16) + ((uint32_t)(ch >> (off + DL)) & ((1u << DE) - 1));
@@ -604,23 +626,30 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
         return;
     }
     gptr<uint8_t> dst = to_global(pd.dst);
+    const bool resume = (pd.flags & PF_RESUME) != 0;
+    const uint64_t base0 = resume ? pd.seg0 : 0;  // entries produced before dst
+    uint64_t resume_bit = resume ? pd.resume_bit : 0;
     // marker pieces start with u16 entries; everything else writes u8
     OutSeg o;
     o.p = dst;
     o.pos = 0;
-    o.seg0 = 0;
+    o.seg0 = base0;
     o.wide = marker ? 1u : 0u;
     o.cap = marker ? pd.cap / 2 : pd.cap;
     uint64_t n16 = 0, u8_off = 0;
     uint64_t last_marker_end = 0;
-    uint32_t status = PS_NONE, err = E_NONE, nblocks = 0;
-    uint64_t end_bit = 0;
+    uint32_t status = PS_NONE, err = E_NONE, nblocks = 0, in_block = 0;
+    uint64_t end_bit = 0, hdr_bit = 0;
     uint64_t t_begin = wall_clock64();
     sh.fixed = 0;
 
-    // ring = the window before the piece (position -32768 + i at slot i)
+    // ring = the window before the piece (position base0 - 32768 + i at
+    // slot (base0 + i) mod 32768)
     if (marker) {
         for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[q] = (RT)(MARKER_BASE + q);
+    } else if (resume) {
+        gptr<const uint16_t> w16 = (gptr<const uint16_t>)to_global(pd.window);
+        for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[(uint32_t)(base0 + q) & RMASK] = (RT)w16[q];
     } else if (pd.flags & PF_WINDOW) {
         for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[q] = to_global(pd.window)[q];
     } else {
@@ -700,7 +729,10 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
                     break;
                 }
             }
-            uint64_t P = br.pos();
+            // a resumed piece re-read its block's header for the tables and
+            // continues at the saved symbol
+            uint64_t P = resume_bit ? resume_bit : br.pos();
+            resume_bit = 0;
             r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err);
             br.init(in, in_len, P);
         }
@@ -711,6 +743,17 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
             u8_off = (2 * n16 + 255) & ~255ull;
             if (u8_off >= pd.cap) {
                 r = R_OVERFLOW;
+            } else if ((pd.flags & PF_STOP_AT_FLIP) && !(btype == 0 && bfinal)) {
+                // hand the rest to the u8-ring kernel: resume at the next
+                // header (after a stored block) or inside this block
+                o.seg0 = n16;
+                o.pos = 0;
+                o.wide = 0;
+                status = PS_FLIPPED;
+                in_block = btype != 0;
+                hdr_bit = hbit;
+                end_bit = br.pos();
+                break;
             } else {
                 o.p = dst + u8_off;
                 o.seg0 = n16;
@@ -755,8 +798,10 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
         po->end_bit = end_bit;
         const uint64_t total = o.seg0 + o.pos;
         po->n16 = marker ? (flipped ? n16 : total) : 0;
-        po->n8 = marker ? (flipped ? total - n16 : 0) : total;
+        po->n8 = marker ? (flipped ? total - n16 : 0) : total - base0;
         po->u8_off = flipped ? u8_off : 0;
+        po->hdr_bit = hdr_bit;
+        po->in_block = in_block;
         po->bits_done = br.pos() - pd.start_bit;
         po->t_begin = t_begin;
         po->t_end = wall_clock64();
