@@ -94,7 +94,6 @@ struct __align__(16) InflateShared {
     uint8_t lens[320 + 2 * WAVE];
     uint8_t pre_lens[32];
     uint32_t qi;
-    int32_t fixed;
 };
 
 // ---- canonical table construction (whole wave) ----------------------------
@@ -239,7 +238,9 @@ __device__ __attribute__((always_inline)) uint32_t read_dynamic(SH &sh, BitReade
         }
     }
     __syncthreads();
-    if (sh.lens[256] == 0) return E_EOB_ZERO;
+    // LDS values that steer control flow are read back through readfirstlane
+    // so the compiler keeps the branch (and everything after it) uniform
+    if (uni(sh.lens[256]) == 0) return E_EOB_ZERO;
     if (build_table<0>(sh, sh.lens, (int)hlit, sh.ll, SH::LLR, sh.ll_count, sh.ll_sorted)) return E_LL_TABLE;
     if (build_table<1>(sh, sh.lens + hlit, (int)hdist, sh.dl, SH::DR, sh.d_count, sh.d_sorted)) return E_D_TABLE;
     return E_NONE;
@@ -440,11 +441,21 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
                                                                   uint64_t &P, OutSeg &o, uint64_t limit_bit,
                                                                   uint64_t &last_marker_end, uint32_t &err) {
     const uint32_t lane = lane_id();
+    // every control value is re-asserted wave-uniform (readfirstlane): the
+    // compiler's divergence analysis cannot prove it for values that came
+    // through memory, and a "divergent" loop control turns the whole decode
+    // loop into exec-masked code with VGPR copies of scalar state
+    o.p = (gptr<uint8_t>)uni64((uint64_t)o.p);
+    o.pos = uni64(o.pos);
+    o.cap = uni64(o.cap);
+    o.seg0 = uni64(o.seg0);
+    o.wide = uni(o.wide);
+    limit_bit = uni64(limit_bit);
     // 32-bit segment positions: the segment is rebased before 2^30 entries
     uint32_t pos = (uint32_t)o.pos;
-    uint32_t cap = (uint32_t)(o.cap < 0x7FFFFFFFull ? o.cap : 0x7FFFFFFFull);
-    uint32_t g0 = (uint32_t)o.seg0;
-    const uint32_t hist0 = (uint32_t)(o.seg0 < WIN ? o.seg0 : WIN);  // history before the segment, capped
+    uint32_t cap = uni((uint32_t)(o.cap < 0x7FFFFFFFull ? o.cap : 0x7FFFFFFFull));
+    uint32_t g0 = uni((uint32_t)o.seg0);
+    const uint32_t hist0 = uni((uint32_t)(o.seg0 < WIN ? o.seg0 : WIN));  // history before the segment, capped
     uint32_t r = R_EOB;
     for (;;) {
         P = uni64(P);
@@ -475,23 +486,23 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
         const uint32_t packed = ok | kind << 1 | tot << 3 | olen << 9;
         // ---- walk the chain of symbol starts: at most 8 symbols, fully
         // unrolled with a uniform `alive` flag so the walk stays scalar
-        const uint32_t room = cap - pos;
+        const uint32_t room = uni(cap - pos);
         uint32_t s = 0, run = 0, opos = 0;
         uint64_t lits = 0, matches = 0;
-        bool eob = false, alive = true;
+        uint32_t eob = 0, alive = 1;
 #pragma unroll
         for (int step = 0; step < 8; step++) {
             if (alive) {
                 const uint32_t pk = uni((uint32_t)__builtin_amdgcn_readlane((int)packed, (int)(s & 63)));
                 const uint32_t k = (pk >> 1) & 3, t = (pk >> 3) & 63, ol = pk >> 9;
                 if (!(pk & 1)) {
-                    alive = false;
+                    alive = 0;
                 } else if (k == K_EOB) {
-                    eob = true;
+                    eob = 1;
                     s += t;
-                    alive = false;
+                    alive = 0;
                 } else if (ol > room - run) {
-                    alive = false;
+                    alive = 0;
                 } else {
                     opos = lane == s ? run : opos;
                     if (k == K_LIT) lits |= 1ull << s;
@@ -505,6 +516,8 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             run = uni(run);
             lits = uni64(lits);
             matches = uni64(matches);
+            alive = uni(alive);
+            eob = uni(eob);
         }
         // ---- distance checks; cut the round where a copy needs it
         for (uint64_t mm = matches; mm; mm = uni64(mm & (mm - 1))) {
@@ -518,7 +531,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
                 matches &= below;
                 run = m;
                 s = mi;
-                eob = false;
+                eob = 0;
                 break;
             }
             if (d + (run - m) > RING) {
@@ -529,7 +542,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
                 matches &= below | (1ull << mi);
                 run = m + (pk >> 9);
                 s = mi + ((pk >> 3) & 63);
-                eob = false;
+                eob = 0;
                 break;
             }
         }
@@ -641,7 +654,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
     uint32_t status = PS_NONE, err = E_NONE, nblocks = 0, in_block = 0;
     uint64_t end_bit = 0, hdr_bit = 0;
     uint64_t t_begin = wall_clock64();
-    sh.fixed = 0;
+    uint32_t fixed_built = 0;  // the fixed tables are in the LUTs
 
     // ring = the window before the piece (position base0 - 32768 + i at
     // slot (base0 + i) mod 32768)
@@ -718,10 +731,10 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
             br.init(in, in_len, (src_byte + len) * 8);
         } else {
             if (btype == 1) {
-                if (!sh.fixed) build_fixed(sh);
-                sh.fixed = 1;
+                if (!fixed_built) build_fixed(sh);
+                fixed_built = 1;
             } else {
-                sh.fixed = 0;
+                fixed_built = 0;
                 err = read_dynamic(sh, br);
                 if (err) {
                     status = PS_ERROR;
