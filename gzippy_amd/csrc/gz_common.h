@@ -65,6 +65,21 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
 }
 __device__ __forceinline__ void compiler_fence() { __asm__ __volatile__("" ::: "memory"); }
 
+// Inclusive prefix sum over the 64 lanes: Hillis-Steele inside each row of
+// 16 lanes (DPP row_shr 1, 2, 4, 8; lanes shifted in from outside the row
+// read 0), then the row totals are carried forward with the GFX9 row
+// broadcasts (row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2
+// and 3).
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return v;
+}
+
 // Pointers that reach a kernel through a struct are generic; accesses through
 // them compile to FLAT instructions, which count against lgkmcnt as well as
 // vmcnt, so every later LDS wait would also wait for outstanding HBM stores.

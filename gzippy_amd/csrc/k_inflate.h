@@ -342,12 +342,13 @@ __device__ __attribute__((always_inline)) void window128(const uint8_t *__restri
         a4 = w[4];
         a5 = w[5];
     } else {
-        a0 = load_word(in, n, wP);
-        a1 = load_word(in, n, wP + 1);
-        a2 = load_word(in, n, wP + 2);
-        a3 = load_word(in, n, wP + 3);
-        a4 = load_word(in, n, wP + 4);
-        a5 = load_word(in, n, wP + 5);
+        // uniform too, so both paths merge in scalar registers
+        a0 = uni(load_word(in, n, wP));
+        a1 = uni(load_word(in, n, wP + 1));
+        a2 = uni(load_word(in, n, wP + 2));
+        a3 = uni(load_word(in, n, wP + 3));
+        a4 = uni(load_word(in, n, wP + 4));
+        a5 = uni(load_word(in, n, wP + 5));
     }
     const uint64_t A = uni64((uint64_t)a0 | (uint64_t)a1 << 32), B = uni64((uint64_t)a2 | (uint64_t)a3 << 32),
                    C = uni64((uint64_t)a4 | (uint64_t)a5 << 32);
@@ -431,11 +432,13 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
 // Decode the Huffman symbols of the current block until end-of-block, from
 // bit P.  Multi-symbol rounds: lane j looks up the litlen (and, for a length
 // code, the distance) LUT entry of a symbol starting at bit P + j, all 64 in
-// one LDS round trip.  A scalar walk from lane 0 follows the chain of real
-// symbol starts (about five per 64 bits here) and assigns output offsets;
-// then the round's literals are written in one store and its copies run in
-// order.  Anything the round cannot take (long codes, invalid codes, a full
-// output, a distance past the available history) is left to scalar_step.
+// one LDS round trip, and knows where the next symbol would start
+// (nxt = j + code bits).  A scalar walk from lane 0 follows nxt through the
+// lanes that hold complete codes (about six per 64 input bits here), a wave
+// prefix sum gives every taken symbol its output offset, the round's
+// literals go out in one LDS write and one store, and its copies run in
+// order.  A round that can take nothing (a code longer than the LUT root, an
+// invalid code, a full output) is left to scalar_step.
 template <typename RT, typename SH>
 __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
                                                                   uint64_t &P, OutSeg &o, uint64_t limit_bit,
@@ -472,8 +475,8 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
         const uint64_t ch = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
         const uint32_t e = sh.ll[(uint32_t)ch & ((1u << SH::LLR) - 1)];
         const uint32_t L = e & 15, kind = (e >> 8) & 3, E = (e >> 4) & 15;
-        uint32_t ok = (L != 0) & (kind != K_BAD);
-        uint32_t tot = L, olen = kind == K_EOB ? 0u : 1u, dist = 0;
+        bool ok = (L != 0) & (kind != K_BAD);
+        uint32_t tot = L, olen = kind == K_LIT ? 1u : 0u, dist = 0;
         if (kind == K_LEN) {
             const uint32_t off = L + E;
             olen = (e >> 16) + ((uint32_t)(ch >> L) & ((1u << E) - 1));
@@ -483,83 +486,50 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             dist = (d >> 16) + ((uint32_t)(ch >> (off + DL)) & ((1u << DE) - 1));
             tot = off + DL + DE;
         }
-        const uint32_t packed = ok | kind << 1 | tot << 3 | olen << 9;
-        // ---- walk the chain of symbol starts: at most 8 symbols, fully
-        // unrolled with a uniform `alive` flag so the walk stays scalar
-        const uint32_t room = uni(cap - pos);
-        uint32_t s = 0, run = 0, opos = 0;
-        uint64_t lits = 0, matches = 0;
-        uint32_t eob = 0, alive = 1;
-#pragma unroll
-        for (int step = 0; step < 8; step++) {
-            if (alive) {
-                const uint32_t pk = uni((uint32_t)__builtin_amdgcn_readlane((int)packed, (int)(s & 63)));
-                const uint32_t k = (pk >> 1) & 3, t = (pk >> 3) & 63, ol = pk >> 9;
-                if (!(pk & 1)) {
-                    alive = 0;
-                } else if (k == K_EOB) {
-                    eob = 1;
-                    s += t;
-                    alive = 0;
-                } else if (ol > room - run) {
-                    alive = 0;
-                } else {
-                    opos = lane == s ? run : opos;
-                    if (k == K_LIT) lits |= 1ull << s;
-                    else matches |= 1ull << s;
-                    run += ol;
-                    s += t;
-                    alive = s < 64;
-                }
-            }
-            s = uni(s);
-            run = uni(run);
-            lits = uni64(lits);
-            matches = uni64(matches);
-            alive = uni(alive);
-            eob = uni(eob);
+        const uint32_t nxt = lane + tot;  // <= 63 + 48
+        const uint64_t CONT = ballot(ok && kind != K_EOB);
+        const uint64_t EOBM = ballot(ok && kind == K_EOB);
+        const uint64_t LITM = ballot(ok && kind == K_LIT);
+        // ---- walk the chain of symbol starts from lane 0
+        uint64_t M = 0;
+        uint32_t s = 0, eob = 0;
+        for (;;) {
+            if (!((CONT >> s) & 1)) break;  // s < 64 here
+            M |= 1ull << s;
+            s = uni((uint32_t)__builtin_amdgcn_readlane((int)nxt, (int)s));
+            if (s >= 64) break;
         }
-        // ---- distance checks; cut the round where a copy needs it
-        for (uint64_t mm = matches; mm; mm = uni64(mm & (mm - 1))) {
-            const uint32_t mi = uni((uint32_t)__builtin_ctzll(mm));
-            const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)opos, (int)mi);
-            const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)dist, (int)mi);
-            const uint64_t below = (1ull << mi) - 1;
-            if (d > hist0 + pos + m + WIN) {
-                // past the available history: leave it to scalar_step (error)
-                lits &= below;
-                matches &= below;
-                run = m;
-                s = mi;
-                eob = 0;
-                break;
-            }
-            if (d + (run - m) > RING) {
-                // a later entry of this round would overwrite the ring slot
-                // this copy still has to read: end the round after the copy
-                const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)packed, (int)mi);
-                lits &= below;
-                matches &= below | (1ull << mi);
-                run = m + (pk >> 9);
-                s = mi + ((pk >> 3) & 63);
-                eob = 0;
-                break;
-            }
+        if (s < 64 && ((EOBM >> s) & 1)) {
+            M |= 1ull << s;
+            s = uni((uint32_t)__builtin_amdgcn_readlane((int)nxt, (int)s));
+            eob = 1;
         }
-        s = uni(s);
-        run = uni(run);
-        lits = uni64(lits);
-        matches = uni64(matches);
-        if (s == 0) {
-            // the round can take nothing (long code, invalid code, full
-            // output, distance past the history): one symbol the slow way,
-            // which either advances P or ends the block with a status
+        M = uni64(M);
+        // ---- output offsets of the taken symbols
+        const uint32_t v = ((M >> lane) & 1) ? olen : 0u;
+        const uint32_t incl = wave_scan_incl(v);
+        const uint32_t opos = incl - v;
+        uint32_t run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, 63));
+        if (M == 0 || run > cap - pos) {
+            // nothing the round can take: one symbol the slow way, which
+            // either advances P or ends the block with a status
             const uint32_t sr = scalar_step<RT>(sh, in, n, P, o, pos, cap, g0, hist0, last_marker_end, err);
             if (sr != R_CONT) {
                 r = sr;
                 break;
             }
         } else {
+            // a copy whose source slot a later entry of this round would
+            // overwrite first (distance near 32768): end the round after it
+            const uint64_t BAD = ballot(((M & ~LITM & ~EOBM) >> lane) & 1 && dist + run - opos > RING);
+            if (BAD) {
+                const uint32_t fb = uni((uint32_t)__builtin_ctzll(BAD));
+                M &= fb == 63 ? ~0ull : (2ull << fb) - 1;
+                run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, (int)fb));
+                s = uni((uint32_t)__builtin_amdgcn_readlane((int)nxt, (int)fb));
+                eob = 0;
+            }
+            const uint64_t lits = uni64(M & LITM), matches = uni64(M & ~LITM & ~EOBM);
             // ---- literals of the round: one LDS write and one HBM store
             if ((lits >> lane) & 1) {
                 const uint32_t q = pos + opos;
@@ -572,12 +542,12 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
                 const uint32_t mi = uni((uint32_t)__builtin_ctzll(mm));
                 const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)opos, (int)mi);
                 const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)dist, (int)mi);
-                const uint32_t ln = (uint32_t)__builtin_amdgcn_readlane((int)packed, (int)mi) >> 9;
+                const uint32_t ln = (uint32_t)__builtin_amdgcn_readlane((int)olen, (int)mi);
                 wave_copy<RT>(sh.ring, o, pos + m, g0, ln, d, last_marker_end);
             }
             pos += run;
             P += s;
-            if (eob) {
+            if (uni(eob)) {
                 r = R_EOB;
                 break;
             }
