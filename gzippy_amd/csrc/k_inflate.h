@@ -257,66 +257,89 @@ struct OutSeg {
     uint32_t wide;
 };
 
+// Entries are u8 in the u8-ring kernel at compile time; the u16-ring kernel
+// writes u16 until its flip.
+template <typename RT>
 __device__ __forceinline__ void put(const OutSeg &o, uint64_t i, uint32_t v) {
-    if (o.wide) ((gptr<uint16_t>)o.p)[i] = (uint16_t)v;
+    if (sizeof(RT) == 2 && o.wide) ((gptr<uint16_t>)o.p)[i] = (uint16_t)v;
     else o.p[i] = (uint8_t)v;
 }
 
 // ---- LZ77 copy by the whole wave -----------------------------------------
 // Entry pos+i takes the value at pos - dist + (i mod dist): every source lies
-// before pos, so the lanes of one copy never depend on each other, and a ring
-// of exactly 32768 entries suffices (each iteration reads all its sources
-// before writing; the only slot shared by a read and a write of the same
-// iteration is dist == 32768 in the same lane).  LDS operations of one wave
-// execute in order, which orders a copy after the symbols before it.
-// All index math is 32-bit (segment positions stay below 2^31).
+// before pos, so no entry of a copy depends on another entry of the same
+// copy.  All reads (at most five chunks of 64 for a 258-byte copy) are issued
+// before any write, with one wait in between; the only ring slot both read
+// and written is dist == 32768 for the same entry, read first.  LDS
+// operations of one wave execute in order, which orders a copy after the
+// symbols before it.  All index math is 32-bit (segment positions stay
+// below 2^31).
+template <typename RT>
+__device__ __forceinline__ void copy_chunk_out(RT *ring, const OutSeg &o, uint32_t gpos, uint32_t pos, uint32_t i,
+                                               uint32_t len, uint32_t v, uint64_t &last_marker_end) {
+    const bool act = i < len;
+    if (act) {
+        ring[(gpos + i) & RMASK] = (RT)v;
+        put<RT>(o, pos + i, v);
+    }
+    if (sizeof(RT) == 2 && o.wide) {
+        const uint64_t m = ballot(act && v >= 256);
+        if (m) last_marker_end = o.seg0 + pos + (i - lane_id()) + 64 - (uint64_t)__clzll(m);
+    }
+}
+
 template <typename RT>
 __device__ __attribute__((always_inline)) void wave_copy(RT *ring, const OutSeg &o, uint32_t pos, uint32_t g0,
                                                          uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
     const uint32_t lane = lane_id();
-    const uint32_t gp = g0 + pos;  // ring position of the copy's first entry
+    const uint32_t gpos = g0 + pos;
+    const uint32_t src0 = gpos - dist;  // ring position of the source's first entry
+    // source offsets of the (up to) five chunks: i, or i mod dist when the
+    // copy overlaps itself (quotient from the hardware reciprocal, corrected
+    // by one step either way; i < 320 and dist <= 258)
+    uint32_t s0 = lane, s1 = lane + 64, s2 = lane + 128, s3 = lane + 192, s4 = lane + 256;
     if (dist < len) {
-        // overlapping: source offset i mod dist, computed with a float
-        // quotient and one correction step (i, dist <= 258: exact enough)
-        const float rd = 1.0f / (float)dist;
-        for (uint32_t k = 0; k < len; k += WAVE) {
-            uint32_t i = k + lane;
-            bool act = i < len;
-            uint32_t q = (uint32_t)((float)i * rd);
-            int32_t si = (int32_t)(i - q * dist);
-            si += si < 0 ? (int32_t)dist : 0;
-            si -= si >= (int32_t)dist ? (int32_t)dist : 0;
-            uint32_t v = 0;
-            if (act) v = ring[(gp + (uint32_t)si - dist) & RMASK];
-            compiler_fence();
-            if (act) {
-                ring[(gp + i) & RMASK] = (RT)v;
-                put(o, pos + i, v);
+        const float rd = __builtin_amdgcn_rcpf((float)dist);
+        auto mod = [&](uint32_t i) {
+            const uint32_t q = (uint32_t)((float)i * rd);
+            int32_t r = (int32_t)(i - q * dist);
+            r += r < 0 ? (int32_t)dist : 0;
+            r -= r >= (int32_t)dist ? (int32_t)dist : 0;
+            return (uint32_t)r;
+        };
+        s0 = mod(s0);
+        s1 = mod(s1);
+        s2 = mod(s2);
+        s3 = mod(s3);
+        s4 = mod(s4);
+    }
+    // every read before any write (ring indices are masked, so lanes past
+    // the copy's end read harmlessly)
+    uint32_t v0, v1, v2, v3, v4;
+    v0 = ring[(src0 + s0) & RMASK];
+    if (len > 64) {
+        v1 = ring[(src0 + s1) & RMASK];
+        if (len > 128) {
+            v2 = ring[(src0 + s2) & RMASK];
+            if (len > 192) {
+                v3 = ring[(src0 + s3) & RMASK];
+                if (len > 256) v4 = ring[(src0 + s4) & RMASK];
             }
-            if (o.wide) {
-                uint64_t m = ballot(act && v >= 256);
-                if (m) last_marker_end = o.seg0 + pos + k + 64 - (uint64_t)__clzll(m);
-            }
-            compiler_fence();
-        }
-    } else {
-        for (uint32_t k = 0; k < len; k += WAVE) {
-            uint32_t i = k + lane;
-            bool act = i < len;
-            uint32_t v = 0;
-            if (act) v = ring[(gp + i - dist) & RMASK];
-            compiler_fence();
-            if (act) {
-                ring[(gp + i) & RMASK] = (RT)v;
-                put(o, pos + i, v);
-            }
-            if (o.wide) {
-                uint64_t m = ballot(act && v >= 256);
-                if (m) last_marker_end = o.seg0 + pos + k + 64 - (uint64_t)__clzll(m);
-            }
-            compiler_fence();
         }
     }
+    compiler_fence();
+    copy_chunk_out<RT>(ring, o, gpos, pos, lane, len, v0, last_marker_end);
+    if (len > 64) {
+        copy_chunk_out<RT>(ring, o, gpos, pos, lane + 64, len, v1, last_marker_end);
+        if (len > 128) {
+            copy_chunk_out<RT>(ring, o, gpos, pos, lane + 128, len, v2, last_marker_end);
+            if (len > 192) {
+                copy_chunk_out<RT>(ring, o, gpos, pos, lane + 192, len, v3, last_marker_end);
+                if (len > 256) copy_chunk_out<RT>(ring, o, gpos, pos, lane + 256, len, v4, last_marker_end);
+            }
+        }
+    }
+    compiler_fence();
 }
 
 enum : uint32_t { R_EOB = 0, R_FLIP = 1, R_OVERFLOW = 2, R_ERROR = 3, R_CONT = 4 };
@@ -390,7 +413,7 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
         if (pos >= cap) return R_OVERFLOW;
         if (lane == 0) {
             sh.ring[(g0 + pos) & RMASK] = (RT)(e >> 16);
-            put(o, pos, e >> 16);
+            put<RT>(o, pos, e >> 16);
         }
         compiler_fence();
         pos++;
@@ -429,6 +452,47 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
     return R_CONT;
 }
 
+// Per-lane decode of the symbol that would start at bit P + lane, and the
+// wave masks the walk needs.
+struct Lookahead {
+    uint32_t e;      // litlen LUT entry (literal value in bits 16..23)
+    uint32_t olen;   // entries the symbol produces (0 for end-of-block)
+    uint32_t dist;   // copy distance (length codes)
+    uint32_t nxt;    // lane of the following symbol (>= 64: past this window)
+    uint64_t CONT;   // lanes holding a complete literal or length+distance code
+    uint64_t EOBM;   // lanes holding end-of-block
+    uint64_t LITM;   // lanes holding a literal
+};
+
+template <typename SH>
+__device__ __attribute__((always_inline)) void lookahead(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
+                                                         uint64_t P, Lookahead &la) {
+    const uint32_t lane = lane_id();
+    uint64_t W0, W1;
+    window128(in, n, P, W0, W1);
+    const uint64_t ch = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
+    const uint32_t e = sh.ll[(uint32_t)ch & ((1u << SH::LLR) - 1)];
+    const uint32_t L = e & 15, kind = (e >> 8) & 3, E = (e >> 4) & 15;
+    bool ok = (L != 0) & (kind != K_BAD);
+    uint32_t tot = L, olen = kind == K_LIT ? 1u : 0u, dist = 0;
+    if (kind == K_LEN) {
+        const uint32_t off = L + E;
+        olen = (e >> 16) + ((uint32_t)(ch >> L) & ((1u << E) - 1));
+        const uint32_t d = sh.dl[(uint32_t)(ch >> off) & ((1u << SH::DR) - 1)];
+        const uint32_t DL = d & 15, DE = (d >> 4) & 15;
+        ok &= (DL != 0) & (((d >> 8) & 3) == K_LEN);
+        dist = (d >> 16) + ((uint32_t)(ch >> (off + DL)) & ((1u << DE) - 1));
+        tot = off + DL + DE;
+    }
+    la.e = e;
+    la.olen = olen;
+    la.dist = dist;
+    la.nxt = lane + tot;  // <= 63 + 48
+    la.CONT = ballot(ok && kind != K_EOB);
+    la.EOBM = ballot(ok && kind == K_EOB);
+    la.LITM = ballot(ok && kind == K_LIT);
+}
+
 // Decode the Huffman symbols of the current block until end-of-block, from
 // bit P.  Multi-symbol rounds: lane j looks up the litlen (and, for a length
 // code, the distance) LUT entry of a symbol starting at bit P + j, all 64 in
@@ -437,8 +501,10 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
 // lanes that hold complete codes (about six per 64 input bits here), a wave
 // prefix sum gives every taken symbol its output offset, the round's
 // literals go out in one LDS write and one store, and its copies run in
-// order.  A round that can take nothing (a code longer than the LUT root, an
-// invalid code, a full output) is left to scalar_step.
+// order.  The next round's input window and LUT lookups are issued before
+// this round's stores and copies, so their latency overlaps the copies.  A
+// round that can take nothing (a code longer than the LUT root, an invalid
+// code, a full output) is left to scalar_step.
 template <typename RT, typename SH>
 __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
                                                                   uint64_t &P, OutSeg &o, uint64_t limit_bit,
@@ -460,6 +526,9 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
     uint32_t g0 = uni((uint32_t)o.seg0);
     const uint32_t hist0 = uni((uint32_t)(o.seg0 < WIN ? o.seg0 : WIN));  // history before the segment, capped
     uint32_t r = R_EOB;
+    P = uni64(P);
+    Lookahead la;
+    lookahead(sh, in, n, P, la);
     for (;;) {
         P = uni64(P);
         pos = uni(pos);
@@ -469,44 +538,24 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             r = R_ERROR;
             break;
         }
-        uint64_t W0, W1;
-        window128(in, n, P, W0, W1);
-        // ---- per lane: the symbol that would start at bit P + lane
-        const uint64_t ch = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
-        const uint32_t e = sh.ll[(uint32_t)ch & ((1u << SH::LLR) - 1)];
-        const uint32_t L = e & 15, kind = (e >> 8) & 3, E = (e >> 4) & 15;
-        bool ok = (L != 0) & (kind != K_BAD);
-        uint32_t tot = L, olen = kind == K_LIT ? 1u : 0u, dist = 0;
-        if (kind == K_LEN) {
-            const uint32_t off = L + E;
-            olen = (e >> 16) + ((uint32_t)(ch >> L) & ((1u << E) - 1));
-            const uint32_t d = sh.dl[(uint32_t)(ch >> off) & ((1u << SH::DR) - 1)];
-            const uint32_t DL = d & 15, DE = (d >> 4) & 15;
-            ok &= (DL != 0) & (((d >> 8) & 3) == K_LEN);
-            dist = (d >> 16) + ((uint32_t)(ch >> (off + DL)) & ((1u << DE) - 1));
-            tot = off + DL + DE;
-        }
-        const uint32_t nxt = lane + tot;  // <= 63 + 48
-        const uint64_t CONT = ballot(ok && kind != K_EOB);
-        const uint64_t EOBM = ballot(ok && kind == K_EOB);
-        const uint64_t LITM = ballot(ok && kind == K_LIT);
+        const uint64_t CONT = uni64(la.CONT), EOBM = uni64(la.EOBM), LITM = uni64(la.LITM);
         // ---- walk the chain of symbol starts from lane 0
         uint64_t M = 0;
         uint32_t s = 0, eob = 0;
         for (;;) {
             if (!((CONT >> s) & 1)) break;  // s < 64 here
             M |= 1ull << s;
-            s = uni((uint32_t)__builtin_amdgcn_readlane((int)nxt, (int)s));
+            s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)s));
             if (s >= 64) break;
         }
         if (s < 64 && ((EOBM >> s) & 1)) {
             M |= 1ull << s;
-            s = uni((uint32_t)__builtin_amdgcn_readlane((int)nxt, (int)s));
+            s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)s));
             eob = 1;
         }
         M = uni64(M);
         // ---- output offsets of the taken symbols
-        const uint32_t v = ((M >> lane) & 1) ? olen : 0u;
+        const uint32_t v = ((M >> lane) & 1) ? la.olen : 0u;
         const uint32_t incl = wave_scan_incl(v);
         const uint32_t opos = incl - v;
         uint32_t run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, 63));
@@ -518,36 +567,42 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
                 r = sr;
                 break;
             }
+            P = uni64(P);
+            lookahead(sh, in, n, P, la);
         } else {
             // a copy whose source slot a later entry of this round would
             // overwrite first (distance near 32768): end the round after it
-            const uint64_t BAD = ballot(((M & ~LITM & ~EOBM) >> lane) & 1 && dist + run - opos > RING);
+            const uint64_t BAD = ballot(((M & ~LITM & ~EOBM) >> lane) & 1 && la.dist + run - opos > RING);
             if (BAD) {
                 const uint32_t fb = uni((uint32_t)__builtin_ctzll(BAD));
                 M &= fb == 63 ? ~0ull : (2ull << fb) - 1;
                 run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, (int)fb));
-                s = uni((uint32_t)__builtin_amdgcn_readlane((int)nxt, (int)fb));
+                s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)fb));
                 eob = 0;
             }
+            eob = uni(eob);
             const uint64_t lits = uni64(M & LITM), matches = uni64(M & ~LITM & ~EOBM);
+            // this round's symbols, kept while the next round is looked up
+            const uint32_t c_e = la.e, c_dist = la.dist, c_olen = la.olen;
+            P += s;
+            if (!eob) lookahead(sh, in, n, P, la);
             // ---- literals of the round: one LDS write and one HBM store
             if ((lits >> lane) & 1) {
                 const uint32_t q = pos + opos;
-                sh.ring[(g0 + q) & RMASK] = (RT)(e >> 16);
-                put(o, q, e >> 16);
+                sh.ring[(g0 + q) & RMASK] = (RT)(c_e >> 16);
+                put<RT>(o, q, c_e >> 16);
             }
             compiler_fence();
             // ---- copies, in stream order
             for (uint64_t mm = matches; mm; mm = uni64(mm & (mm - 1))) {
                 const uint32_t mi = uni((uint32_t)__builtin_ctzll(mm));
                 const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)opos, (int)mi);
-                const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)dist, (int)mi);
-                const uint32_t ln = (uint32_t)__builtin_amdgcn_readlane((int)olen, (int)mi);
+                const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)c_dist, (int)mi);
+                const uint32_t ln = (uint32_t)__builtin_amdgcn_readlane((int)c_olen, (int)mi);
                 wave_copy<RT>(sh.ring, o, pos + m, g0, ln, d, last_marker_end);
             }
             pos += run;
-            P += s;
-            if (uni(eob)) {
+            if (eob) {
                 r = R_EOB;
                 break;
             }
@@ -588,7 +643,7 @@ __device__ __attribute__((always_inline)) void stored_copy(gptr<const uint8_t> i
             uint32_t i = k + j * WAVE + lane;
             if (i < len) {
                 ring[(uint32_t)(gpos + i) & RMASK] = (RT)v[j];
-                put(o, o.pos + i, v[j]);
+                put<RT>(o, o.pos + i, v[j]);
             }
         }
     }
