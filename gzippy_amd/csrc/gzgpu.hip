@@ -302,13 +302,14 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
         c->d_hits.ensure((size_t)hit_cap * sizeof(gzgpu_boundary));
         HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
         uint32_t *cnt = c->d_counters.as<uint32_t>();
-        uint64_t blocks = std::min<uint64_t>((nbits + 255) / 256, (uint64_t)c->num_cus * 64);
+        // stage A: one lane per 32 bit offsets (1 KiB of input per workgroup pass)
+        uint64_t chunks = (end_bit - (start_bit & ~31ull) + 31) / 32;
+        uint64_t blocks = std::min<uint64_t>((chunks + 255) / 256, (uint64_t)c->num_cus * 16);
         hipLaunchKernelGGL(k_bf_stage_a, dim3((uint32_t)blocks), dim3(256), 0, c->stream, d_in, in_len, start_bit, end_bit,
                            c->d_surv.as<uint64_t>(), surv_cap, cnt + 0, c->d_hits.as<gzgpu_boundary>(), hit_cap,
                            cnt + 1);
         HIPCHK(hipGetLastError());
-        uint32_t h_cnt[2];
-        HIPCHK(hipMemcpyAsync(h_cnt, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+        uint32_t *h_cnt = download<uint32_t>(c, cnt, 2);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->st.launches++;
         if (h_cnt[0] > surv_cap) {
@@ -323,7 +324,7 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
             HIPCHK(hipGetLastError());
             c->st.launches++;
         }
-        HIPCHK(hipMemcpyAsync(h_cnt, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+        h_cnt = download<uint32_t>(c, cnt, 2);
         HIPCHK(hipStreamSynchronize(c->stream));
         if (h_cnt[1] > hit_cap) {
             hit_cap = h_cnt[1] + 1024;

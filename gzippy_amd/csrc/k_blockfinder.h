@@ -54,6 +54,67 @@ __device__ __forceinline__ bool dyn_stage_a(const uint8_t *__restrict__ in, uint
     return leaves == 128 || (leaves == 64 && gt1 == 0);
 }
 
+// Precode condition of the dynamic predicate, from the 57 bits after the
+// 17 header bits (`p`) and HCLEN.
+__device__ __forceinline__ bool precode_leaves_ok(uint64_t p, uint32_t hclen) {
+    uint32_t leaves = 0, gt1 = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 19; i++) {
+        uint32_t l = (uint32_t)(p >> (3 * i)) & 7;
+        if (i >= hclen + 4) l = 0;
+        leaves += l ? (1u << (7 - l)) : 0u;
+        gt1 += l > 1;
+    }
+    return leaves == 128 || (leaves == 64 && gt1 == 0);
+}
+
+// The stored-block conditions at byte-aligned offset o whose LEN/NLEN word
+// matched; fills the hit fields (find_uncompressed_blocks,
+// blockfinder_validation.rs:370-470).
+__device__ __forceinline__ bool stored_hit(const uint8_t *__restrict__ in, uint64_t n, uint64_t o, uint64_t start_bit,
+                                           uint64_t end_bit, uint64_t sb, uint64_t until, uint32_t size,
+                                           uint64_t &earliest, uint64_t &latest, bool &succ) {
+    const uint64_t by = o >> 3;
+    if (!(by >= sb && by < until && o >= 10)) return false;
+    const uint64_t first = o - 10;
+    const uint32_t raw = (uint32_t)in[first >> 3] | (uint32_t)in[(first >> 3) + 1] << 8;
+    const uint32_t pre = (raw >> (first & 7)) & 0x3FF;
+    if ((pre & (7u << 7)) != 0) return false;
+    uint32_t tz = 3;
+    for (uint32_t j = 4; j <= 10; j++) {
+        if (pre & (1u << (10 - j))) break;
+        tz = j;
+    }
+    earliest = o - tz;
+    latest = o - 3;
+    const uint32_t len = size & 0xFFFF;
+    if (!(earliest >= start_bit && latest < end_bit && len > 0 && by + 4 + len <= n)) return false;
+    // successor header after the payload (planning hint only)
+    succ = false;
+    const uint64_t nb = by + 4 + len;
+    if (nb < n) {
+        const uint32_t h3 = in[nb] & 7, bt = h3 >> 1;
+        if (bt == 0) {
+            if (nb + 5 <= n) {
+                const uint32_t s2 = (uint32_t)in[nb + 1] | (uint32_t)in[nb + 2] << 8 | (uint32_t)in[nb + 3] << 16 |
+                                    (uint32_t)in[nb + 4] << 24;
+                succ = ((s2 ^ (s2 >> 16)) & 0xFFFF) == 0xFFFF;
+            }
+        } else if (bt == 2) {
+            // a final dynamic block would fail BFINAL=0: test the header
+            // with BFINAL masked off
+            succ = (in[nb] & 1) ? ((peek64(in, n, nb * 8) >> 3 & 31) <= 29) : dyn_stage_a(in, n, nb * 8);
+        }
+    }
+    return true;
+}
+
+// Stage A.  Each lane owns 32 consecutive bit offsets [o0, o0 + 32) and
+// reads their 128 bits once into registers.  The 13-bit header test runs on
+// all 32 offsets from the registers (v_alignbit); the precode sum only on the
+// offsets that pass it (about one in ten); the stored-block LEN/NLEN test on
+// the 4 byte-aligned offsets, the rest of the stored conditions only on a
+// match.  Survivors and hits are appended with one atomic per wave.
 __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ in, uint64_t n, uint64_t start_bit,
                                                     uint64_t end_bit, uint64_t *__restrict__ surv, uint32_t surv_cap,
                                                     uint32_t *surv_count, gzgpu_boundary *__restrict__ hits,
@@ -63,66 +124,80 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
     uint64_t until = (end_bit + 7) / 8;
     if (n < 4) until = 0;
     else if (until > n - 4) until = n - 4;
+    // dynamic candidates need o + 17 < total
+    const uint64_t dyn_end = total > 17 ? (end_bit < total - 17 ? end_bit : total - 17) : 0;
+    const uint64_t first = start_bit & ~31ull;
+    const uint64_t nchunk = end_bit > first ? (end_bit - first + 31) / 32 : 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    // iterate in wave-aligned steps so every lane reaches the ballots
-    for (uint64_t base = start_bit + (uint64_t)blockIdx.x * blockDim.x; base < end_bit; base += stride) {
-        const uint64_t o = base + threadIdx.x;
-        bool in_range = o < end_bit;
-        // ---- dynamic stage A
-        bool dyn = false;
-        if (in_range && o + 17 < total) dyn = dyn_stage_a(in, n, o);
-        uint32_t slot;
-        wave_append(dyn, surv_count, slot);
-        if (dyn && slot < surv_cap) surv[slot] = o;
-        // ---- stored
-        bool st = false, succ = false;
-        uint64_t earliest = 0, latest = 0;
-        if (in_range && (o & 7) == 0) {
-            uint64_t by = o >> 3;
-            if (by >= sb && by < until && o >= 10) {
-                uint32_t size = (uint32_t)in[by] | (uint32_t)in[by + 1] << 8 | (uint32_t)in[by + 2] << 16 |
-                                (uint32_t)in[by + 3] << 24;
-                if (((size ^ (size >> 16)) & 0xFFFF) == 0xFFFF) {
-                    uint64_t first = o - 10;
-                    uint32_t raw = (uint32_t)in[first >> 3] | (uint32_t)in[(first >> 3) + 1] << 8;
-                    uint32_t pre = (raw >> (first & 7)) & 0x3FF;
-                    if ((pre & (7u << 7)) == 0) {
-                        uint32_t tz = 3;
-                        for (uint32_t j = 4; j <= 10; j++) {
-                            if (pre & (1u << (10 - j))) break;
-                            tz = j;
-                        }
-                        earliest = o - tz;
-                        latest = o - 3;
-                        uint32_t len = size & 0xFFFF;
-                        st = earliest >= start_bit && latest < end_bit && len > 0 && by + 4 + len <= n;
-                        // successor header after the payload (planning hint only)
-                        uint64_t nb = by + 4 + len;
-                        if (st && nb < n) {
-                            uint32_t h3 = in[nb] & 7, bt = h3 >> 1;
-                            if (bt == 0) {
-                                if (nb + 5 <= n) {
-                                    uint32_t s2 = (uint32_t)in[nb + 1] | (uint32_t)in[nb + 2] << 8 |
-                                                  (uint32_t)in[nb + 3] << 16 | (uint32_t)in[nb + 4] << 24;
-                                    succ = ((s2 ^ (s2 >> 16)) & 0xFFFF) == 0xFFFF;
-                                }
-                            } else if (bt == 2) {
-                                // a final dynamic block would fail BFINAL=0: test the
-                                // header with BFINAL masked off
-                                succ = (in[nb] & 1) ? ((peek64(in, n, nb * 8) >> 3 & 31) <= 29)
-                                                    : dyn_stage_a(in, n, nb * 8);
-                            }
-                        }
-                    }
-                }
-            }
+    // wave-aligned iterations so every lane reaches the ballots
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nchunk; base += stride) {
+        const uint64_t c = base + threadIdx.x;
+        const uint64_t o0 = first + 32 * c;
+        const bool live = c < nchunk;
+        uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+        if (live) {
+            const uint64_t wi = o0 >> 5;
+            w0 = load_word(in, n, wi);
+            w1 = load_word(in, n, wi + 1);
+            w2 = load_word(in, n, wi + 2);
+            w3 = load_word(in, n, wi + 3);
         }
-        wave_append(st, hit_count, slot);
-        if (st && slot < hit_cap) {
-            hits[slot].bit_offset = earliest;
-            hits[slot].seek_bit = latest;
-            hits[slot].kind = 0;
-            hits[slot].info = succ ? 1u : 0u;
+        // ---- header test on all 32 offsets
+        uint32_t hm = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 32; j++) {
+            const uint32_t h = __builtin_amdgcn_alignbit(w1, w0, j);
+            const bool pass = (h & 7) == 4 && ((h >> 3) & 31) <= 29 && ((h >> 8) & 31) <= 29;
+            hm |= (uint32_t)pass << j;
+        }
+        // offsets inside [start_bit, dyn_end)
+        {
+            const uint64_t lo = start_bit > o0 ? start_bit - o0 : 0;
+            const uint64_t hi = dyn_end > o0 ? dyn_end - o0 : 0;
+            const uint32_t lo32 = lo > 32 ? 32u : (uint32_t)lo, hi32 = hi > 32 ? 32u : (uint32_t)hi;
+            const uint32_t below_hi = hi32 >= 32 ? 0xFFFFFFFFu : ((1u << hi32) - 1);
+            const uint32_t below_lo = lo32 >= 32 ? 0xFFFFFFFFu : ((1u << lo32) - 1);
+            hm &= below_hi & ~below_lo;
+            if (!live) hm = 0;
+        }
+        // ---- precode test on the header survivors
+        for (;;) {
+            if (!ballot(hm != 0)) break;
+            bool ok = false;
+            uint64_t o = 0;
+            if (hm) {
+                const uint32_t j = (uint32_t)__builtin_ctz(hm);
+                hm &= hm - 1;
+                o = o0 + j;
+                const uint32_t h = __builtin_amdgcn_alignbit(w1, w0, j);
+                const uint32_t q = j + 17, k = q >> 5, r = q & 31;
+                const uint32_t a = k ? w1 : w0, b = k ? w2 : w1, d = k ? w3 : w2;
+                const uint64_t p = (uint64_t)__builtin_amdgcn_alignbit(b, a, r) |
+                                   (uint64_t)__builtin_amdgcn_alignbit(d, b, r) << 32;
+                ok = precode_leaves_ok(p, (h >> 13) & 15);
+            }
+            uint32_t slot;
+            wave_append(ok, surv_count, slot);
+            if (ok && slot < surv_cap) surv[slot] = o;
+        }
+        // ---- stored blocks at the 4 byte-aligned offsets
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint64_t o = o0 + 8 * k;
+            const uint32_t size = __builtin_amdgcn_alignbit(w1, w0, 8 * k);
+            bool cand = live && o < end_bit && ((size ^ (size >> 16)) & 0xFFFF) == 0xFFFF;
+            if (!ballot(cand)) continue;
+            uint64_t earliest = 0, latest = 0;
+            bool succ = false, st = false;
+            if (cand) st = stored_hit(in, n, o, start_bit, end_bit, sb, until, size, earliest, latest, succ);
+            uint32_t slot;
+            wave_append(st, hit_count, slot);
+            if (st && slot < hit_cap) {
+                hits[slot].bit_offset = earliest;
+                hits[slot].seek_bit = latest;
+                hits[slot].kind = 0;
+                hits[slot].info = succ ? 1u : 0u;
+            }
         }
     }
 }
