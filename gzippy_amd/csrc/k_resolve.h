@@ -81,29 +81,95 @@ __global__ void __launch_bounds__(1024) k_chain(const ResolvePiece *__restrict__
     }
 }
 
+// Value of entry t of a piece for the body pass: window bytes come from the
+// final output just before the piece (zeros before the member start).
+__device__ __forceinline__ uint8_t body_entry(const ResolvePiece &r, uint64_t t) {
+    gptr<const uint8_t> mout = to_global((const uint8_t *)r.mout);
+    if (t < r.n16) {
+        const uint32_t e = ((gptr<const uint16_t>)to_global(r.slab))[t];
+        if (e < 256) return (uint8_t)e;
+        const int64_t q = (int64_t)r.out_pos + (int64_t)(e - MARKER_BASE) - (int64_t)WIN;
+        return q < 0 ? 0 : mout[q];
+    }
+    return to_global(r.slab)[r.u8_off + (t - r.n16)];
+}
+
+__device__ __forceinline__ uint32_t resolve16(uint32_t e, gptr<const uint8_t> win, bool near_start, int64_t out_pos) {
+    if (e < 256) return e;
+    const int64_t q = (int64_t)(e - MARKER_BASE) - (int64_t)WIN;
+    return (near_start && out_pos + q < 0) ? 0u : (uint32_t)win[q];
+}
+
+// Body pass: every entry of a marker piece except its last 32768 (written
+// by k_chain).  One thread per 16-byte-aligned block of the output, written
+// with one 16-byte store when the block lies inside the tile:
+//   * all u8 entries: five aligned dword loads from the slab, realigned with
+//     v_alignbyte (the slab's u8 part is 256-byte aligned, the output is not);
+//   * all u16 entries: nine aligned dword loads, markers resolved from the
+//     window just before the piece;
+//   * tile edges and the u16/u8 seam: entry by entry.
 __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__restrict__ rp,
                                                       const ResolveTile *__restrict__ tiles, uint32_t ntiles) {
     for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
         const ResolveTile tl = tiles[b];
         const ResolvePiece r = rp[tl.piece];
-        gptr<const uint8_t> wbase = to_global((const uint8_t *)r.mout) + r.out_pos;  // window = wbase[-WIN .. 0)
-        const bool near_start = r.out_pos < WIN;
-        gptr<const uint16_t> e16 = (gptr<const uint16_t>)to_global(r.slab);
-        gptr<const uint8_t> s8 = to_global(r.slab);
         gptr<uint8_t> mout = to_global(r.mout);
-        for (uint64_t t = tl.t0 + threadIdx.x; t < tl.t1; t += blockDim.x) {
-            uint8_t v;
-            if (t < r.n16) {
-                uint32_t e = e16[t];
-                if (e < 256) v = (uint8_t)e;
-                else {
-                    int64_t q = (int64_t)(e - MARKER_BASE) - (int64_t)WIN;
-                    v = (near_start && (int64_t)r.out_pos + q < 0) ? 0 : wbase[q];
+        gptr<const uint8_t> win = to_global((const uint8_t *)r.mout) + r.out_pos;  // window = win[-WIN .. 0)
+        const bool near_start = r.out_pos < WIN;
+        const int64_t d0 = (int64_t)(r.out_pos + tl.t0), d1 = (int64_t)(r.out_pos + tl.t1);  // output range
+        // blocks are 16-byte aligned in the address space (the member's
+        // output base need not be aligned); offsets relative to mout
+        const int64_t ba = (int64_t)(uintptr_t)r.mout;
+        const int64_t first = ((ba + d0) & ~(int64_t)15) - ba;
+        for (int64_t blk = first + 16 * (int64_t)threadIdx.x; blk < d1; blk += 16 * (int64_t)blockDim.x) {
+            const bool full = blk >= d0 && blk + 16 <= d1;
+            const uint64_t t = (uint64_t)(blk - (int64_t)r.out_pos);  // entry of the block's first byte (full blocks)
+            if (full && t >= r.n16) {
+                // u8 entries [t, t + 16) at slab offset u8_off + t - n16
+                const uint64_t so = r.u8_off + (t - r.n16);
+                gptr<const uint32_t> sw = (gptr<const uint32_t>)(to_global(r.slab) + (so & ~3ull));
+                const uint32_t sh = (uint32_t)(so & 3) * 8;
+                uint32_t x0 = sw[0], x1 = sw[1], x2 = sw[2], x3 = sw[3], x4 = sh ? sw[4] : 0;
+                typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 o;
+                o.x = __builtin_amdgcn_alignbit(x1, x0, sh);
+                o.y = __builtin_amdgcn_alignbit(x2, x1, sh);
+                o.z = __builtin_amdgcn_alignbit(x3, x2, sh);
+                o.w = __builtin_amdgcn_alignbit(x4, x3, sh);
+                *(gptr<u32x4>)(mout + blk) = o;
+            } else if (full && t + 16 <= r.n16) {
+                // u16 entries [t, t + 16): 32 bytes at slab offset 2t
+                const uint64_t so = 2 * t;
+                gptr<const uint32_t> sw = (gptr<const uint32_t>)(to_global(r.slab) + (so & ~3ull));
+                const uint32_t sh = (uint32_t)(so & 3) * 8;
+                uint32_t x[9];
+#pragma unroll
+                for (int k = 0; k < 8; k++) x[k] = sw[k];
+                x[8] = sh ? sw[8] : 0;
+                uint32_t ob[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t lo = __builtin_amdgcn_alignbit(x[2 * k + 1], x[2 * k], sh);
+                    const uint32_t hi = __builtin_amdgcn_alignbit(x[2 * k + 2], x[2 * k + 1], sh);
+                    const uint32_t b0 = resolve16(lo & 0xFFFF, win, near_start, (int64_t)r.out_pos);
+                    const uint32_t b1 = resolve16(lo >> 16, win, near_start, (int64_t)r.out_pos);
+                    const uint32_t b2 = resolve16(hi & 0xFFFF, win, near_start, (int64_t)r.out_pos);
+                    const uint32_t b3 = resolve16(hi >> 16, win, near_start, (int64_t)r.out_pos);
+                    ob[k] = b0 | b1 << 8 | b2 << 16 | b3 << 24;
                 }
+                typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 o;
+                o.x = ob[0];
+                o.y = ob[1];
+                o.z = ob[2];
+                o.w = ob[3];
+                *(gptr<u32x4>)(mout + blk) = o;
             } else {
-                v = s8[r.u8_off + (t - r.n16)];
+                for (int64_t k = 0; k < 16; k++) {
+                    const int64_t d = blk + k;
+                    if (d >= d0 && d < d1) mout[d] = body_entry(r, (uint64_t)(d - (int64_t)r.out_pos));
+                }
             }
-            mout[r.out_pos + t] = v;
         }
     }
 }

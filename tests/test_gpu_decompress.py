@@ -141,3 +141,41 @@ def test_batch_matches_per_member(gpu):
         out = torch.zeros(max(raws[i].size, 1), dtype=torch.uint8, device="cuda")
         n, c, _ = gpu.decompress_device(dev(ins[i]), out)
         assert n == raws[i].size and np.array_equal(out[:n].cpu().numpy(), raws[i])
+
+
+def test_batch_split_members_unaligned_outputs(gpu):
+    """Members large enough to be split into speculative pieces, written at
+    odd output offsets: the marker resolution writes 16-byte blocks aligned
+    in the address space, whatever the member's output base."""
+    import torch
+    ctx = gpu.context()
+    old = ctx.tuning()
+    ctx.set_tuning(chunk_bytes=64 << 10, pieces_per_chunk=4, batch_split_bytes=32 << 10)
+    try:
+        kinds = ["text", "runs", "u32", "text", "random", "text"]
+        raws, ins = [], []
+        for i, k in enumerate(kinds):
+            raw = W.generate(k, (1 << 20) + 4099 * i + 13, 700 + i)
+            raws.append(raw)
+            ins.append(W.gzip_compress(raw, 6 if i % 2 else 1))
+        in_off = np.cumsum([0] + [x.size for x in ins[:-1]])
+        in_len = np.array([x.size for x in ins])
+        out_cap = np.array([r.size for r in raws])
+        out_off = np.cumsum([3] + [c + 7 for c in out_cap[:-1]])
+        d_in = dev(np.concatenate(ins))
+        d_out = torch.zeros(int(out_off[-1] + out_cap[-1]) + 5, dtype=torch.uint8, device="cuda")
+        say("split batch", len(raws), [x.size for x in ins])
+        st = {}
+        status, olen, crc, st = gpu.decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap)
+        assert (status == 0).all(), status
+        say("pieces", st["pieces"])
+        assert st["pieces"] > len(raws)
+        host = d_out.cpu().numpy()
+        for i, r in enumerate(raws):
+            assert olen[i] == r.size
+            assert np.array_equal(host[out_off[i]: out_off[i] + r.size], r), i
+            assert crc[i] == zlib.crc32(r.tobytes())
+            # the gaps between members stay untouched
+            assert not host[out_off[i] + r.size: out_off[i] + r.size + 7].any()
+    finally:
+        ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
