@@ -137,10 +137,18 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
         uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
         if (live) {
             const uint64_t wi = o0 >> 5;
-            w0 = load_word(in, n, wi);
-            w1 = load_word(in, n, wi + 1);
-            w2 = load_word(in, n, wi + 2);
-            w3 = load_word(in, n, wi + 3);
+            if ((wi + 4) * 4 <= n) {
+                gptr<const uint32_t> w = (gptr<const uint32_t>)to_global(in) + wi;
+                w0 = w[0];
+                w1 = w[1];
+                w2 = w[2];
+                w3 = w[3];
+            } else {
+                w0 = load_word(in, n, wi);
+                w1 = load_word(in, n, wi + 1);
+                w2 = load_word(in, n, wi + 2);
+                w3 = load_word(in, n, wi + 3);
+            }
         }
         // ---- header test on all 32 offsets
         uint32_t hm = 0;
@@ -160,25 +168,30 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
             hm &= below_hi & ~below_lo;
             if (!live) hm = 0;
         }
-        // ---- precode test on the header survivors
-        for (;;) {
-            if (!ballot(hm != 0)) break;
-            bool ok = false;
-            uint64_t o = 0;
-            if (hm) {
-                const uint32_t j = (uint32_t)__builtin_ctz(hm);
-                hm &= hm - 1;
-                o = o0 + j;
-                const uint32_t h = __builtin_amdgcn_alignbit(w1, w0, j);
-                const uint32_t q = j + 17, k = q >> 5, r = q & 31;
-                const uint32_t a = k ? w1 : w0, b = k ? w2 : w1, d = k ? w3 : w2;
-                const uint64_t p = (uint64_t)__builtin_amdgcn_alignbit(b, a, r) |
-                                   (uint64_t)__builtin_amdgcn_alignbit(d, b, r) << 32;
-                ok = precode_leaves_ok(p, (h >> 13) & 15);
+        // ---- precode test on the header survivors -> survivor mask
+        uint32_t sm = 0;
+        for (uint32_t hh = hm; hh; hh &= hh - 1) {
+            const uint32_t j = (uint32_t)__builtin_ctz(hh);
+            const uint32_t h = __builtin_amdgcn_alignbit(w1, w0, j);
+            const uint32_t q = j + 17, k = q >> 5, r = q & 31;
+            const uint32_t a = k ? w1 : w0, b = k ? w2 : w1, d = k ? w3 : w2;
+            const uint64_t p =
+                (uint64_t)__builtin_amdgcn_alignbit(b, a, r) | (uint64_t)__builtin_amdgcn_alignbit(d, b, r) << 32;
+            if (precode_leaves_ok(p, (h >> 13) & 15)) sm |= 1u << j;
+        }
+        // one atomic per wave: lanes write their survivors after a prefix sum
+        {
+            const uint32_t cnt = (uint32_t)__builtin_popcount(sm);
+            const uint32_t incl = wave_scan_incl(cnt);
+            const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            uint32_t base_slot = 0;
+            if (tot) {
+                if (lane_id() == 0) base_slot = atomicAdd(surv_count, tot);
+                base_slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)base_slot);
             }
-            uint32_t slot;
-            wave_append(ok, surv_count, slot);
-            if (ok && slot < surv_cap) surv[slot] = o;
+            uint32_t slot = base_slot + incl - cnt;
+            for (uint32_t ss = sm; ss; ss &= ss - 1, slot++)
+                if (slot < surv_cap) surv[slot] = o0 + (uint32_t)__builtin_ctz(ss);
         }
         // ---- stored blocks at the 4 byte-aligned offsets
 #pragma unroll

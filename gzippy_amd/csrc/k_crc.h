@@ -2,9 +2,14 @@
 //
 // Stage 1 (k_crc_segments): every workgroup folds one segment of <= 64 KiB;
 // each of its 256 lanes runs a slicing-by-8 table CRC over 256 contiguous
-// bytes from a zero register, then the lanes are combined in a tree with the
-// GF(2) shift crc(A||B) = crc(A) * x^(8|B|) mod P ^ crc(B) (combine_crc32,
-// crc32.rs:127-133; polynomial_multiply_modulo :64; x_power_modulo :104).
+// bytes from a zero register.  The lane CRCs combine with the GF(2) shift
+// crc(A||B) = crc(A) * x^(8|B|) mod P ^ crc(B) (combine_crc32,
+// crc32.rs:127-133; polynomial_multiply_modulo :64; x_power_modulo :104):
+// for a full, 16-byte-aligned segment lane l's CRC is multiplied by the
+// constant x^(8 * 256 * (255 - l)) and the 256 products are XORed; the
+// segment is first staged through LDS with coalesced 16-byte loads.  A short
+// or unaligned segment (a member's last, or an unaligned member) folds the
+// lanes in a tree with computed shifts.
 // Stage 2 (k_crc_members): per member, fold its segment CRCs the same way
 // and apply the init/final inversion: crc = ~(x^(8D) * 0xFFFFFFFF ^ raw).
 #pragma once
@@ -37,6 +42,10 @@ __device__ __forceinline__ uint32_t xpow_bytes(uint64_t n) {
     return p;
 }
 
+// c_kseg[l] = x^(8 * 256 * (255 - l)) mod P: lane l's shift to the end of a
+// full 64 KiB segment, filled by the host
+__constant__ uint32_t c_kseg[256];
+
 // shift a raw CRC register over `len` zero bytes
 __device__ __forceinline__ uint32_t crc_shift(uint32_t c, uint64_t len) { return len ? gf2_mulmod(xpow_bytes(len), c) : c; }
 
@@ -48,11 +57,43 @@ __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict_
                                                       const uint32_t *__restrict__ tables, uint32_t *__restrict__ seg_crc) {
     __shared__ uint32_t t[8][256];
     __shared__ uint32_t part[256];
+    __shared__ uint32_t buf[256 * 65];  // lane l's 256 bytes at dword 65 * l (conflict-free reads)
+    __shared__ uint32_t red;
     for (uint32_t i = threadIdx.x; i < 8 * 256; i += blockDim.x) t[i >> 8][i & 255] = tables[i];
     __syncthreads();
     for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
         const uint8_t *p = base + seg_off[s];
         const uint32_t len = seg_len[s];
+        if (len == CRC_SEG && (((uintptr_t)p) & 15) == 0) {
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            gptr<const u32x4> src = (gptr<const u32x4>)to_global(p);
+#pragma unroll 4
+            for (uint32_t k = 0; k < 16; k++) {
+                const uint32_t o = k * 4096 + threadIdx.x * 16;  // byte offset in the segment
+                const u32x4 v = src[o >> 4];
+                uint32_t *d = &buf[65 * (o >> 8) + ((o & 255) >> 2)];
+                d[0] = v.x;
+                d[1] = v.y;
+                d[2] = v.z;
+                d[3] = v.w;
+            }
+            if (threadIdx.x == 0) red = 0;
+            __syncthreads();
+            const uint32_t *my = &buf[65 * threadIdx.x];
+            uint32_t c = 0;
+#pragma unroll 4
+            for (uint32_t j = 0; j < 32; j++) {
+                const uint32_t lo = c ^ my[2 * j], hi = my[2 * j + 1];
+                c = t[7][lo & 0xFF] ^ t[6][(lo >> 8) & 0xFF] ^ t[5][(lo >> 16) & 0xFF] ^ t[4][lo >> 24] ^
+                    t[3][hi & 0xFF] ^ t[2][(hi >> 8) & 0xFF] ^ t[1][(hi >> 16) & 0xFF] ^ t[0][hi >> 24];
+            }
+            c = gf2_mulmod(c_kseg[threadIdx.x], c);
+            atomicXor(&red, c);
+            __syncthreads();
+            if (threadIdx.x == 0) seg_crc[s] = red;
+            __syncthreads();
+            continue;
+        }
         const uint32_t b0 = threadIdx.x * 256;
         uint32_t mine = b0 < len ? (len - b0 < 256 ? len - b0 : 256) : 0;
         const uint8_t *q = p + b0;
