@@ -71,6 +71,83 @@ def measured_traffic(args):
     return None
 
 
+def batch_leg(args, dev, ctx):
+    """10,000 independent members (lognormal sizes around 64 KiB, four content
+    kinds, levels 1/6/9): one batched call versus one call per member."""
+    import ctypes
+    import numpy as np
+    import torch
+
+    import gzippy_amd
+    from gzippy_amd import workloads as W
+
+    t0 = time.time()
+    mem = W.batch_members(args.batch_members, seed=args.seed + 3)
+    raws = [m[0] for m in mem]
+    ins = [m[1] for m in mem]
+    kinds = [m[2] for m in mem]
+    levels = [m[3] for m in mem]
+    in_len = np.array([x.size for x in ins], dtype=np.uint64)
+    in_off = np.concatenate([[0], np.cumsum(in_len)[:-1]]).astype(np.uint64)
+    out_cap = np.array([r.size for r in raws], dtype=np.uint64)
+    out_off = np.concatenate([[0], np.cumsum(out_cap)[:-1]]).astype(np.uint64)
+    C, D = int(in_len.sum()), int(out_cap.sum())
+    log(f"batch workload: {len(mem)} members, {D} B -> {C} B in {time.time() - t0:.1f}s")
+    d_in = torch.from_numpy(np.concatenate(ins)).to(dev)
+    d_out = torch.empty(D + 1, dtype=torch.uint8, device=dev)
+    ref = torch.from_numpy(np.concatenate(raws)).to(dev)
+    del raws, ins, mem
+    torch.cuda.synchronize()
+
+    # batched: every member in one pipeline (one persistent inflate launch)
+    d_out.zero_()
+    status, olen, crcs, st = gzippy_amd.decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=ctx)
+    torch.cuda.synchronize()
+    ok = bool((status == 0).all()) and bool((olen == out_cap).all()) and torch.equal(d_out[:D], ref)
+    if not ok:
+        log("FAILED: batched output differs")
+        sys.exit(1)
+    bt = []
+    for _ in range(args.batch_steps):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        gzippy_amd.decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=ctx)
+        torch.cuda.synchronize()
+        bt.append(time.perf_counter() - ts)
+    b_s = sorted(bt)[len(bt) // 2]
+
+    # plain: one library call (one pipeline, one inflate launch) per member,
+    # straight through the C ABI
+    lib = ctx.lib
+    n = ctypes.c_size_t()
+    crc = ctypes.c_uint32()
+    base_in, base_out = d_in.data_ptr(), d_out.data_ptr()
+    args_list = [(ctypes.c_void_p(base_in + int(in_off[i])), int(in_len[i]), ctypes.c_void_p(base_out + int(out_off[i])),
+                  int(out_cap[i])) for i in range(len(in_off))]
+    d_out.zero_()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for i, (a_in, l_in, a_out, cap) in enumerate(args_list):
+        st_p = lib.gzgpu_decompress_device(ctx.h, a_in, l_in, a_out, cap, ctypes.byref(n), ctypes.byref(crc), None)
+        if st_p != 0:
+            log(f"FAILED: per-member decode of member {i} ({kinds[i]}, level {levels[i]}, {int(in_len[i])} B -> "
+                f"{int(out_cap[i])} B): status {st_p}: {ctx.error(st_p)}")
+            sys.exit(1)
+    torch.cuda.synchronize()
+    p_s = time.perf_counter() - ts
+    if not torch.equal(d_out[:D], ref):
+        log("FAILED: per-member output differs")
+        sys.exit(1)
+    return {"members": len(in_off), "compressed_bytes": C, "decompressed_bytes": D, "seed": args.seed + 3,
+            "batched_gbs": round(D / b_s / 1e9, 3), "batched_ms": round(b_s * 1e3, 3),
+            "plain_per_member_gbs": round(D / p_s / 1e9, 3), "plain_per_member_ms": round(p_s * 1e3, 3),
+            "batched_over_plain": round(p_s / b_s, 2), "pieces": st["pieces"],
+            "batched_phase_ms": {k: round(st[k], 3) for k in ("ms_find", "ms_inflate", "ms_resolve", "ms_crc",
+                                                              "ms_total", "ms_inflate_marker", "ms_inflate_clean")},
+            "batched_launches": st["launches"],
+            "check": "every member's bytes equal the generator's; CRC32 and ISIZE verified per member"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +159,8 @@ def main():
     ap.add_argument("--pieces-per-chunk", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0x675A0001)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-members", type=int, default=10000, help="batched-members leg (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, run one plain device-to-device copy of this many MiB "
                          "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -178,6 +257,10 @@ def main():
                                        "marker_ms": round(ms_marker / args.steps, 3),
                                        "achieved_c_plus_d_gbs": round(stage_gbs, 2)}},
     }
+    if args.batch_members:
+        del d_in, d_out
+        torch.cuda.empty_cache()
+        line["batch"] = batch_leg(args, dev, ctx)
     traffic = measured_traffic(args)
     if traffic is not None:
         line["roofline"]["traffic"] = traffic["bytes_per_launch"]

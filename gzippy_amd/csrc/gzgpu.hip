@@ -405,13 +405,19 @@ std::vector<PieceOut> run_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in
             n16 += r.n16;
             flipped += r.status == PS_FLIPPED;
         }
+        size_t slow = 0;
+        for (size_t i = 1; i < ro.size(); i++)
+            if (ro[i].t_end - ro[i].t_begin > ro[slow].t_end - ro[slow].t_begin) slow = i;
         std::sort(us.begin(), us.end());
         fprintf(stderr,
                 "[gzgpu] inflate<%s> pieces=%u grid=%u kernel=%.3f ms span=%.3f ms piece_us min=%.1f p50=%.1f "
-                "max=%.1f entries=%llu (u16 %llu) flipped=%llu per-wave MB/s p50=%.1f\n",
+                "max=%.1f entries=%llu (u16 %llu) flipped=%llu per-wave MB/s p50=%.1f | slowest: %llu entries, "
+                "%llu bits, %u blocks, flags %u, status %u\n",
                 sizeof(RT) == 2 ? "u16" : "u8", n, grid, ms, (double)(t1 - t0) / 1e5, us.front(), us[us.size() / 2],
                 us.back(), (unsigned long long)bytes, (unsigned long long)n16, (unsigned long long)flipped,
-                us[us.size() / 2] > 0 ? (double)bytes / n / us[us.size() / 2] : 0.0);
+                us[us.size() / 2] > 0 ? (double)bytes / n / us[us.size() / 2] : 0.0,
+                (unsigned long long)(ro[slow].n16 + ro[slow].n8), (unsigned long long)ro[slow].bits_done,
+                ro[slow].nblocks, descs[slow].flags, ro[slow].status);
     }
     return ro;
 }
@@ -489,21 +495,62 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
     }
 }
 
+// Every kernel reads the compressed input from a 16-byte-aligned base: the
+// inflate kernels fetch it with scalar loads, which ignore the low address
+// bits.  An unaligned caller pointer becomes (base, shift bytes) and every
+// bit position is moved by the shift.
+struct AlignedIn {
+    const uint8_t *base;
+    uint64_t shift;  // bytes
+};
+AlignedIn align_input(const void *p) {
+    const uintptr_t a = (uintptr_t)p;
+    return {(const uint8_t *)(a & ~(uintptr_t)15), (uint64_t)(a & 15)};
+}
+
+void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem);
+
 // ---- the member pipeline ---------------------------------------------------
 void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
-    const uint64_t chunk_bits = (c->tune.chunk_bytes ? c->tune.chunk_bytes : 512 * KIB) * 8;
-    // wavefronts per chunk: explicit, or enough pieces to fill every resident
-    // slot of the u8-ring inflate kernel (4 per CU) across the split members
-    uint64_t split_bits = 0;
-    for (auto &M : mem)
-        if (M.status == GZGPU_OK && M.split) split_bits += M.de - M.ds;
-    uint64_t slots = (uint64_t)c->num_cus * 4;
-    uint64_t ppc = c->tune.pieces_per_chunk;
-    if (!ppc) {
-        uint64_t chunks = std::max<uint64_t>(1, (split_bits + chunk_bits - 1) / chunk_bits);
-        ppc = std::max<uint64_t>(1, slots / chunks);
+    const AlignedIn ai = align_input(d_in);
+    const uint64_t sb = ai.shift * 8;
+    for (auto &M : mem) {
+        M.ds += sb;
+        M.de += sb;
+        M.limit += sb;
     }
-    const uint64_t spacing = std::max<uint64_t>(chunk_bits / ppc, 8 * 4 * KIB);
+    run_members_aligned(c, ai.base, in_len + ai.shift, mem);
+    for (auto &M : mem) {
+        M.ds -= sb;
+        M.de -= sb;
+        M.limit -= sb;
+        if (M.end_bit >= sb) M.end_bit -= sb;
+    }
+}
+
+void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
+    const uint64_t chunk_bits = (c->tune.chunk_bytes ? c->tune.chunk_bytes : 512 * KIB) * 8;
+    const uint64_t min_spacing = 8 * 32 * KIB;  // gzip_block_finder.rs:109 (spacing >= 32 KiB)
+    const uint64_t slots = (uint64_t)c->num_cus * 4;  // resident u8-ring inflate wavefronts
+    // Piece grid.  Explicit pieces_per_chunk: chunk_bits / ppc compressed bits.
+    // Default: aim every piece at about one slot's share of the call's
+    // expected output (ISIZE), so a highly compressible member is cut as
+    // finely as an incompressible one and no wavefront is left with a tail.
+    auto expect_out = [](const Member &M) {
+        const uint64_t cb = (M.de - M.ds) / 8;
+        return (uint64_t)M.exp_isize >= cb / 2 ? (uint64_t)M.exp_isize : 4 * cb;  // ISIZE is mod 2^32
+    };
+    uint64_t total_out = 0;
+    for (auto &M : mem)
+        if (M.status == GZGPU_OK) total_out += expect_out(M);
+    const uint64_t target_out = std::max<uint64_t>(256 * KIB, total_out / (slots - slots / 16));
+    const uint64_t ppc = c->tune.pieces_per_chunk;
+    auto spacing_of = [&](const Member &M) -> uint64_t {
+        if (ppc) return std::max<uint64_t>(chunk_bits / ppc, min_spacing);
+        const uint64_t npieces = (expect_out(M) + target_out - 1) / target_out;
+        if (npieces <= 1) return ~0ull;
+        return std::max<uint64_t>((M.de - M.ds) / npieces, min_spacing);
+    };
 
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     // ---- 2/3: find + plan
@@ -526,7 +573,8 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
         M.pieces.clear();
         M.pieces.push_back((uint32_t)P.size());
         P.push_back(p0);
-        if (!M.split || M.de <= M.ds + spacing) continue;
+        const uint64_t spacing = spacing_of(M);
+        if (!M.split || spacing == ~0ull || M.de <= M.ds + spacing) continue;
         std::vector<gzgpu_boundary> hits = find_hits(c, d_in, in_len, M.ds + 1, M.de);
         c->st.candidates += (uint32_t)hits.size();
         // piece starts: per grid point the first hit past it; a stored hit
@@ -1061,7 +1109,7 @@ int gzgpu_get_tuning(gzgpu_ctx *c, gzgpu_tuning *t) {
     if (!t->chunk_bytes) t->chunk_bytes = 512 * KIB;
     if (!t->pieces_per_chunk) t->pieces_per_chunk = 8;
     if (!t->waves_per_cu) t->waves_per_cu = 16;
-    if (!t->batch_split_bytes) t->batch_split_bytes = 4 * MIB;
+
     return GZGPU_OK;
 }
 
@@ -1217,7 +1265,8 @@ int gzgpu_decompress_batch(gzgpu_ctx *c, const void *d_in, const uint64_t *in_of
             M.exp_isize = le32(e + ENVELOPE_HEAD + 4);
             M.out = (uint8_t *)d_out + out_off[i];
             M.out_cap = out_cap[i];
-            M.split = in_len[i] > tn.batch_split_bytes;
+            // 0 (default): the planner cuts members by expected output
+            M.split = tn.batch_split_bytes == 0 || in_len[i] > tn.batch_split_bytes;
         }
         run_members(c, (const uint8_t *)d_in, in_total, mem);
         int first_bad = GZGPU_OK;
@@ -1244,7 +1293,15 @@ int gzgpu_find_blocks(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t st
     return guarded(c, [&]() {
         HIPCHK(hipSetDevice(c->device));
         uint64_t eb = std::min<uint64_t>(end_bit, (uint64_t)in_len * 8);
-        std::vector<gzgpu_boundary> h = find_hits(c, (const uint8_t *)d_in, in_len, start_bit, eb);
+        const AlignedIn ai = align_input(d_in);
+        const uint64_t sb = ai.shift * 8;
+        std::vector<gzgpu_boundary> h =
+            eb > start_bit ? find_hits(c, ai.base, in_len + ai.shift, start_bit + sb, eb + sb)
+                           : std::vector<gzgpu_boundary>();
+        for (auto &b : h) {
+            b.bit_offset -= sb;
+            b.seek_bit -= sb;
+        }
         *count = h.size();
         if (out) std::memcpy(out, h.data(), std::min(cap, h.size()) * sizeof(gzgpu_boundary));
         return (int)GZGPU_OK;
@@ -1297,10 +1354,12 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
     if (!c || !d_in || !d_out || !res) return GZGPU_INVALID_ARGUMENT;
     return guarded(c, [&]() {
         HIPCHK(hipSetDevice(c->device));
+        const AlignedIn ai = align_input(d_in);
+        const uint64_t sb = ai.shift * 8;
         PieceDesc pd{};
-        pd.start_bit = start_bit;
-        pd.stop_bit = stop_bit;
-        pd.limit_bit = (uint64_t)in_len * 8;
+        pd.start_bit = start_bit + sb;
+        pd.stop_bit = stop_bit >= NO_STOP - sb ? NO_STOP : stop_bit + sb;
+        pd.limit_bit = ((uint64_t)in_len + ai.shift) * 8;
         pd.member = 0;
         DevBuf slab, win;
         if (window) {
@@ -1325,18 +1384,18 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
         c->d_counters.ensure(64);
         HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
         if (window)
-            hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(1), dim3(64), 0, c->stream, (const uint8_t *)d_in,
-                               (uint64_t)in_len, c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u,
+            hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(1), dim3(64), 0, c->stream, ai.base,
+                               (uint64_t)in_len + ai.shift, c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u,
                                c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
         else
-            hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(1), dim3(64), 0, c->stream, (const uint8_t *)d_in,
-                               (uint64_t)in_len, c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u,
+            hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(1), dim3(64), 0, c->stream, ai.base,
+                               (uint64_t)in_len + ai.shift, c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u,
                                c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
         HIPCHK(hipGetLastError());
         PieceOut po;
         HIPCHK(hipMemcpyAsync(&po, c->d_pout.p, sizeof po, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        res->end_bit = po.end_bit;
+        res->end_bit = po.end_bit >= sb ? po.end_bit - sb : 0;
         res->bfinal_hit = po.status == PS_FINAL;
         res->status = po.status == PS_FINAL || po.status == PS_STOPPED ? GZGPU_OK
                       : po.status == PS_OVERFLOW                      ? GZGPU_OUTPUT_TOO_SMALL

@@ -77,23 +77,29 @@ constexpr uint32_t RMASK = RING - 1;
 
 template <typename RT>
 struct __align__(16) InflateShared {
-    // LUT root bits: the u16-ring kernel runs 2 wavefronts per CU and has
-    // room for 11/10; the u8-ring kernel keeps 10/9 so that four fit in a
-    // CU's 160 KiB of LDS (about 40 KB each)
+    // Two-level Huffman LUTs: a root of LLR / DR bits, then subtables for
+    // the codes longer than the root (zlib's inflate_table layout), in the
+    // same array after the root.  The u16-ring kernel runs 2 wavefronts per
+    // CU and has room for 11/10-bit roots; the u8-ring kernel keeps 10/8 so
+    // that four fit in a CU's 160 KiB of LDS (40 KB each).  A subtable that
+    // does not fit its reserve leaves its codes to the canonical slow path.
     static constexpr int LLR = sizeof(RT) == 2 ? 11 : 10;
-    static constexpr int DR = sizeof(RT) == 2 ? 10 : 9;
+    static constexpr int DR = sizeof(RT) == 2 ? 10 : 8;
+    static constexpr int LL_SUB = sizeof(RT) == 2 ? 512 : 384;
+    static constexpr int D_SUB = sizeof(RT) == 2 ? 128 : 64;
     RT ring[RING];
-    uint32_t ll[1 << LLR];
-    uint32_t dl[1 << DR];
+    uint32_t ll[(1 << LLR) + LL_SUB];
+    uint32_t dl[(1 << DR) + D_SUB];
     uint16_t ll_sorted[288];
     uint16_t d_sorted[32];
     uint16_t ll_count[16];
     uint16_t d_count[16];
     uint16_t offs[16];
     uint16_t firstc[16];
-    uint8_t lens[320 + 2 * WAVE];
+    uint8_t lens[320];
     uint8_t pre_lens[32];
     uint32_t qi;
+    uint32_t sub_next;
 };
 
 // ---- canonical table construction (whole wave) ----------------------------
@@ -102,7 +108,7 @@ struct __align__(16) InflateShared {
 // (build_precode_table, marker_inflate.rs:4515-4540).  Incomplete codes are
 // accepted; their unused codewords fail when decoded.
 template <int MODE, typename SH>
-__device__ int build_table(SH &sh, const uint8_t *lens, int n, uint32_t *lut, int R, uint16_t *count,
+__device__ int build_table(SH &sh, const uint8_t *lens, int n, uint32_t *lut, int R, int sub_cap, uint16_t *count,
                            uint16_t *sorted) {
     const uint32_t lane = lane_id();
     uint32_t l[5], rank[5];
@@ -167,7 +173,68 @@ __device__ int build_table(SH &sh, const uint8_t *lens, int n, uint32_t *lut, in
         }
     }
     __syncthreads();
+    // ---- second level for codes longer than the root.  Canonical codes
+    // sharing their first R bits are consecutive in canonical order; each
+    // such group gets a subtable of 2^(longest - R) entries, indexed by the
+    // next bits of the stream.  Root entry of a group: length 0, extra field
+    // = subtable bits, value = subtable index.
+    uint32_t nlong = 0;
+#pragma unroll
+    for (int L = 1; L <= 15; L++) nlong += L > R ? run[L] : 0u;
+    if (sub_cap > 0 && nlong > 0) {
+        if (lane == 0) {
+            uint32_t next = 1u << R, gprefix = ~0u, glen = 0;
+            auto close_group = [&]() {
+                const uint32_t bits = glen - (uint32_t)R;
+                if (next + (1u << bits) <= (1u << R) + (uint32_t)sub_cap) {
+                    lut[__brev(gprefix) >> (32 - R)] = (next << 16) | (bits << 4);
+                    next += 1u << bits;
+                }
+            };
+            for (uint32_t L = (uint32_t)R + 1; L <= 15; L++) {
+                const uint32_t cntL = count[L], fc = sh.firstc[L];
+                for (uint32_t i = 0; i < cntL; i++) {
+                    const uint32_t prefix = (fc + i) >> (L - (uint32_t)R);
+                    if (prefix != gprefix) {
+                        if (gprefix != ~0u) close_group();
+                        gprefix = prefix;
+                    }
+                    glen = L;
+                }
+            }
+            close_group();
+            sh.sub_next = next;
+        }
+        __syncthreads();
+        if (!complete)  // unused codewords of an incomplete code must read as invalid
+            for (uint32_t i = (1u << R) + lane; i < sh.sub_next; i += WAVE) lut[i] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            uint32_t s = c * WAVE + lane;
+            uint32_t L = l[c];
+            if ((int)s < n && (int)L > R) {
+                const uint32_t code = (uint32_t)sh.firstc[L] + rank[c];
+                const uint32_t k = L - (uint32_t)R;
+                const uint32_t pe = lut[__brev(code >> k) >> (32 - R)];
+                if ((pe & 15) == 0 && (pe >> 16) != 0) {
+                    const uint32_t sb = (pe >> 4) & 15, off = pe >> 16;
+                    const uint32_t sidx = __brev(code & ((1u << k) - 1)) >> (32 - k);
+                    const uint32_t e = MODE == 0 ? make_ll_entry(s, L) : make_d_entry(s, L);
+                    for (uint32_t f = sidx; f < (1u << sb); f += 1u << k) lut[off + f] = e;
+                }
+            }
+        }
+        __syncthreads();
+    }
     return 0;
+}
+
+// Two-level lookup: the root entry, or the subtable entry it points to.
+__device__ __forceinline__ uint32_t sub_lookup(const uint32_t *lut, uint32_t e, uint64_t bits_after_root) {
+    if ((e & 15) == 0 && (e >> 16) != 0)
+        e = lut[(e >> 16) + ((uint32_t)bits_after_root & ((1u << ((e >> 4) & 15)) - 1))];
+    return e;
 }
 
 template <typename SH>
@@ -175,10 +242,10 @@ __device__ void build_fixed(SH &sh) {
     const uint32_t lane = lane_id();
     for (uint32_t s = lane; s < 288; s += WAVE) sh.lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
     __syncthreads();
-    build_table<0>(sh, sh.lens, 288, sh.ll, SH::LLR, sh.ll_count, sh.ll_sorted);
+    build_table<0>(sh, sh.lens, 288, sh.ll, SH::LLR, SH::LL_SUB, sh.ll_count, sh.ll_sorted);
     for (uint32_t s = lane; s < 32; s += WAVE) sh.lens[s] = 5;
     __syncthreads();
-    build_table<1>(sh, sh.lens, 32, sh.dl, SH::DR, sh.d_count, sh.d_sorted);
+    build_table<1>(sh, sh.lens, 32, sh.dl, SH::DR, SH::D_SUB, sh.d_count, sh.d_sorted);
 }
 
 // read_dynamic_huffman_coding (marker_inflate.rs:1206-1245) and
@@ -204,7 +271,7 @@ __device__ __attribute__((always_inline)) uint32_t read_dynamic(SH &sh, BitReade
     if (lane < hclen) sh.pre_lens[c_pre_order[lane]] = (uint8_t)pl;
     __syncthreads();
     // the precode LUT lives in the litlen LUT storage until the litlen table is built
-    if (build_table<2>(sh, sh.pre_lens, 19, sh.ll, PRE_ROOT, sh.d_count, sh.d_sorted)) return E_PRECODE;
+    if (build_table<2>(sh, sh.pre_lens, 19, sh.ll, PRE_ROOT, 0, sh.d_count, sh.d_sorted)) return E_PRECODE;
     const uint32_t total = hlit + hdist;
     uint32_t i = 0, prev = 0;
     while (i < total) {
@@ -241,8 +308,8 @@ __device__ __attribute__((always_inline)) uint32_t read_dynamic(SH &sh, BitReade
     // LDS values that steer control flow are read back through readfirstlane
     // so the compiler keeps the branch (and everything after it) uniform
     if (uni(sh.lens[256]) == 0) return E_EOB_ZERO;
-    if (build_table<0>(sh, sh.lens, (int)hlit, sh.ll, SH::LLR, sh.ll_count, sh.ll_sorted)) return E_LL_TABLE;
-    if (build_table<1>(sh, sh.lens + hlit, (int)hdist, sh.dl, SH::DR, sh.d_count, sh.d_sorted)) return E_D_TABLE;
+    if (build_table<0>(sh, sh.lens, (int)hlit, sh.ll, SH::LLR, SH::LL_SUB, sh.ll_count, sh.ll_sorted)) return E_LL_TABLE;
+    if (build_table<1>(sh, sh.lens + hlit, (int)hdist, sh.dl, SH::DR, SH::D_SUB, sh.d_count, sh.d_sorted)) return E_D_TABLE;
     return E_NONE;
 }
 
@@ -390,6 +457,7 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
     uint64_t W0, W1;
     window128(in, n, P, W0, W1);
     uint32_t e = uni(sh.ll[(uint32_t)W0 & ((1u << SH::LLR) - 1)]);
+    e = uni(sub_lookup(sh.ll, e, W0 >> SH::LLR));
     if ((e & 15) == 0) {
         uint32_t sl;
         int sy = (int)uni((uint32_t)slow_decode((uint32_t)W0 & 0x7FFF, sh.ll_count, sh.ll_sorted, sl));
@@ -426,6 +494,7 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
     const uint32_t off = L + E;
     const uint64_t wd = (W0 >> off) | (off ? W1 << (64 - off) : 0);
     uint32_t d = uni(sh.dl[(uint32_t)wd & ((1u << SH::DR) - 1)]);
+    d = uni(sub_lookup(sh.dl, d, wd >> SH::DR));
     if ((d & 15) == 0) {
         uint32_t sl;
         int sy = (int)uni((uint32_t)slow_decode((uint32_t)wd & 0x7FFF, sh.d_count, sh.d_sorted, sl));
@@ -471,18 +540,26 @@ __device__ __attribute__((always_inline)) void lookahead(SH &sh, const uint8_t *
     uint64_t W0, W1;
     window128(in, n, P, W0, W1);
     const uint64_t ch = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
-    const uint32_t e = sh.ll[(uint32_t)ch & ((1u << SH::LLR) - 1)];
+    uint32_t e = sh.ll[(uint32_t)ch & ((1u << SH::LLR) - 1)];
+    // codes longer than the root: one more LDS round trip, only when a lane needs it
+    if (ballot((e & 15) == 0 && (e >> 16) != 0)) e = sub_lookup(sh.ll, e, ch >> SH::LLR);
     const uint32_t L = e & 15, kind = (e >> 8) & 3, E = (e >> 4) & 15;
     bool ok = (L != 0) & (kind != K_BAD);
-    uint32_t tot = L, olen = kind == K_LIT ? 1u : 0u, dist = 0;
+    uint32_t tot = L, olen = kind == K_LIT ? 1u : 0u, dist = 0, dcode = 0, doff = 0;
     if (kind == K_LEN) {
         const uint32_t off = L + E;
         olen = (e >> 16) + ((uint32_t)(ch >> L) & ((1u << E) - 1));
-        const uint32_t d = sh.dl[(uint32_t)(ch >> off) & ((1u << SH::DR) - 1)];
+        dcode = sh.dl[(uint32_t)(ch >> off) & ((1u << SH::DR) - 1)];
+        doff = off;
+    }
+    if (ballot(kind == K_LEN && (dcode & 15) == 0 && (dcode >> 16) != 0))
+        if (kind == K_LEN) dcode = sub_lookup(sh.dl, dcode, ch >> (doff + SH::DR));
+    if (kind == K_LEN) {
+        const uint32_t d = dcode;
         const uint32_t DL = d & 15, DE = (d >> 4) & 15;
         ok &= (DL != 0) & (((d >> 8) & 3) == K_LEN);
-        dist = (d >> 16) + ((uint32_t)(ch >> (off + DL)) & ((1u << DE) - 1));
-        tot = off + DL + DE;
+        dist = (d >> 16) + ((uint32_t)(ch >> (doff + DL)) & ((1u << DE) - 1));
+        tot = doff + DL + DE;
     }
     la.e = e;
     la.olen = olen;
@@ -625,25 +702,57 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
     return r;
 }
 
-// Stored block body: LEN bytes from the input at byte `src_byte`.
+// Stored block body: LEN bytes from the input at byte `src_byte`.  The wave
+// moves 4 KiB per step: every lane issues its aligned dword loads for 16
+// four-byte groups before the one wait, realigns them with v_alignbit and
+// writes the output (one dword store per group when the output is dword
+// aligned and the entries are bytes).  Only the block's last 32768 entries
+// go to the ring; earlier ones would be overwritten there.
 template <typename RT>
-__device__ __attribute__((always_inline)) void stored_copy(gptr<const uint8_t> in, uint64_t src_byte, RT *ring,
+__device__ __attribute__((always_inline)) void stored_copy(const uint8_t *in_, uint64_t n, uint64_t src_byte, RT *ring,
                                                            const OutSeg &o, uint32_t len) {
+    constexpr uint32_t G = 16;  // groups per lane per step
     const uint32_t lane = lane_id();
+    gptr<const uint32_t> inw = (gptr<const uint32_t>)to_global(in_);
     const uint64_t gpos = o.seg0 + o.pos;
-    for (uint32_t k = 0; k < len; k += 4 * WAVE) {
-        uint32_t v[4];
+    const uint32_t sh = (uint32_t)(src_byte & 3) * 8;
+    const uint64_t w0 = src_byte >> 2;
+    const uint64_t nwords = n >> 2;  // whole words inside the buffer
+    const uint32_t ring_from = len > RING ? len - RING : 0;
+    const bool dst_aligned = sizeof(RT) == 1 || !o.wide ? ((((uint64_t)o.p + o.pos) & 3) == 0) : false;
+    for (uint32_t k = 0; k < len; k += G * WAVE * 4) {
+        uint32_t lo[G], hi[G];
+        const uint64_t qk = (uint64_t)(k >> 2);
+        if (w0 + qk + G * WAVE + 1 < nwords) {
 #pragma unroll
-        for (uint32_t j = 0; j < 4; j++) {
-            uint32_t i = k + j * WAVE + lane;
-            v[j] = i < len ? in[src_byte + i] : 0;
+            for (uint32_t g = 0; g < G; g++) {
+                const uint64_t wi = w0 + qk + g * WAVE + lane;
+                lo[g] = inw[wi];
+                hi[g] = inw[wi + 1];
+            }
+        } else {
+#pragma unroll
+            for (uint32_t g = 0; g < G; g++) {
+                const uint64_t wi = w0 + qk + g * WAVE + lane;
+                lo[g] = load_word(in_, n, wi);
+                hi[g] = load_word(in_, n, wi + 1);
+            }
         }
 #pragma unroll
-        for (uint32_t j = 0; j < 4; j++) {
-            uint32_t i = k + j * WAVE + lane;
-            if (i < len) {
-                ring[(uint32_t)(gpos + i) & RMASK] = (RT)v[j];
-                put<RT>(o, o.pos + i, v[j]);
+        for (uint32_t g = 0; g < G; g++) {
+            const uint32_t t = k + (g * WAVE + lane) * 4;  // first entry of the group
+            if (t < len) {
+                const uint32_t v = __builtin_amdgcn_alignbit(hi[g], lo[g], sh);
+                const uint32_t nb = len - t < 4 ? len - t : 4;
+                if (dst_aligned && nb == 4 && (sizeof(RT) == 1 || !o.wide)) {
+                    *(gptr<uint32_t>)(o.p + o.pos + t) = v;
+                } else {
+                    for (uint32_t j = 0; j < nb; j++) put<RT>(o, o.pos + t + j, (v >> (8 * j)) & 0xFF);
+                }
+                if (t + 4 > ring_from) {
+                    for (uint32_t j = 0; j < nb; j++)
+                        if (t + j >= ring_from) ring[(uint32_t)(gpos + t + j) & RMASK] = (RT)((v >> (8 * j)) & 0xFF);
+                }
             }
         }
     }
@@ -743,7 +852,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
             if (o.pos + len > o.cap) {
                 r = R_OVERFLOW;
             } else {
-                stored_copy<RT>(to_global(in), src_byte, sh.ring, o, len);
+                stored_copy<RT>(in, in_len, src_byte, sh.ring, o, len);
                 o.pos += len;
                 if (o.wide && o.seg0 + o.pos >= WIN && o.seg0 + o.pos - last_marker_end >= WIN) r = R_FLIP;
                 if (o.pos >= 0x40000000u) {  // keep segment positions 32-bit

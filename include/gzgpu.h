@@ -53,7 +53,7 @@ typedef struct {
     uint64_t chunk_bytes;       /* speculation grid, compressed bytes per chunk (default 512 KiB) */
     uint32_t pieces_per_chunk;  /* wavefronts per chunk: sub-splits at found block boundaries */
     uint32_t waves_per_cu;      /* persistent-grid occupancy target */
-    uint64_t batch_split_bytes; /* batch: members above this compressed size are split speculatively */
+    uint64_t batch_split_bytes; /* batch: split only members above this compressed size (0: by expected output) */
 } gzgpu_tuning;
 
 /* Per-call timing and shape, filled when a non-NULL pointer is passed. */

@@ -179,3 +179,27 @@ def test_batch_split_members_unaligned_outputs(gpu):
             assert not host[out_off[i] + r.size: out_off[i] + r.size + 7].any()
     finally:
         ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
+
+
+def test_unaligned_input_pointers(gpu, fine_grid):
+    """Members whose compressed bytes start at every offset mod 16: decoded
+    (single piece and split), and block-finder hits shifted exactly."""
+    import torch
+    cases = [W.text_member(300_000, seed=41), W.text_member(3 << 20, seed=42)]
+    for raw, gz in cases:
+        ref_hits = None
+        for shift in (0, 1, 2, 3, 5, 8, 13, 15):
+            buf = np.zeros(gz.size + 32, dtype=np.uint8)
+            buf[shift: shift + gz.size] = gz
+            d_buf = dev(buf)
+            d_in = d_buf[shift: shift + gz.size]
+            assert d_in.data_ptr() % 16 == (d_buf.data_ptr() + shift) % 16
+            out = torch.zeros(raw.size, dtype=torch.uint8, device="cuda")
+            say("unaligned", gz.size, "shift", shift)
+            n, crc, st = gpu.decompress_device(d_in, out)
+            assert n == raw.size and crc == zlib.crc32(raw.tobytes())
+            assert np.array_equal(out.cpu().numpy(), raw)
+            hits = gpu.find_blocks(d_in, 0, gz.size * 8, in_len=gz.size)
+            if ref_hits is None:
+                ref_hits = hits
+            assert hits == ref_hits, shift
