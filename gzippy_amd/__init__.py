@@ -30,7 +30,7 @@ from ._native import GzgpuError, Stats, Tuning  # noqa: F401
 __all__ = [
     "GzippyError", "decompress", "decompress_with_threads", "decompress_to_writer",
     "decompress_to_writer_with_threads", "decompress_device", "decompress_batch", "find_blocks", "crc32",
-    "inflate_piece", "context",
+    "inflate_piece", "context", "Server",
 ]
 
 
@@ -234,3 +234,77 @@ def inflate_piece(d_in, start_bit, stop_bit, d_out, window=None, in_len=None, ct
     if st != _native.OK:
         _raise(ctx, st)
     return res
+
+
+class Server:
+    """Continuous-batching decompression server on one GPU (the C ABI's
+    gzgpu_server).  submit() copies the request and returns a ticket at once;
+    requests that arrive while others run join the next device batch.  Each
+    result equals decompress() of that request alone."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_requests=0, workers=0):
+        self.lib = _native.load_library()
+        cfg = _native.ServerConfig(hbm_budget_bytes, max_batch_requests, workers)
+        h = ctypes.c_void_p()
+        st = self.lib.gzgpu_server_create(device, ctypes.byref(cfg), ctypes.byref(h))
+        if st != _native.OK:
+            raise GzippyError("cannot start the server", kind=_native.STATUS.get(st, str(st)), status=st)
+        self.h = h
+
+    def close(self):
+        """Drain queued requests and stop the workers."""
+        if getattr(self, "h", None):
+            self.lib.gzgpu_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, data):
+        a = _as_bytes_view(data)
+        t = ctypes.c_uint64()
+        st = self.lib.gzgpu_submit(self.h, a.ctypes.data if a.size else None, a.size, ctypes.byref(t))
+        if st != _native.OK:
+            raise GzippyError("submit failed", kind=_native.STATUS.get(st, str(st)), status=st)
+        return t.value
+
+    def poll(self, ticket, timeout_ms=-1):
+        """Request info dict once finished (waits up to timeout_ms; < 0: wait),
+        else None."""
+        info = _native.RequestInfo()
+        st = self.lib.gzgpu_poll(self.h, ticket, int(timeout_ms), ctypes.byref(info))
+        if st == _native.PENDING:
+            return None
+        if st != _native.OK:
+            raise GzippyError("unknown ticket", kind=_native.STATUS.get(st, str(st)), status=st)
+        return {"status": info.status, "members": info.members, "out_len": info.out_len,
+                "queue_ms": info.queue_ms, "latency_ms": info.latency_ms, "batch": info.batch}
+
+    def fetch(self, ticket, info=None):
+        """The decoded bytes of a finished request (releases the ticket);
+        raises GzippyError when the request failed."""
+        info = info or self.poll(ticket, -1)
+        out = np.empty(max(int(info["out_len"]), 1), dtype=np.uint8)
+        n = ctypes.c_size_t()
+        st = self.lib.gzgpu_fetch(self.h, ticket, out.ctypes.data, out.size, ctypes.byref(n))
+        if st != _native.OK:
+            raise GzippyError(f"request failed: {_native.STATUS.get(st, st)}", kind=_native.STATUS.get(st, str(st)),
+                              status=st)
+        return out[: n.value].tobytes()
+
+    def result(self, ticket):
+        return self.fetch(ticket, self.poll(ticket, -1))
+
+    def stats(self):
+        s = _native.ServerStats()
+        self.lib.gzgpu_server_get_stats(self.h, ctypes.byref(s))
+        return s.as_dict()

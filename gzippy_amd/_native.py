@@ -13,7 +13,7 @@ from . import build as _build
 STATUS = {
     0: "OK", 1: "InvalidHeader", 2: "InvalidGzipFormat", 3: "DecodeFailed", 4: "SizeMismatch",
     5: "CrcMismatch", 6: "Io", 7: "GpuUnavailable", 8: "OutOfDeviceMemory", 9: "OutputTooSmall",
-    10: "InvalidArgument",
+    10: "InvalidArgument", 11: "Pending",
 }
 OK, INVALID_HEADER, INVALID_FORMAT, DECODE_FAILED, SIZE_MISMATCH, CRC_MISMATCH, IO, GPU_UNAVAILABLE, \
     OUT_OF_DEVICE_MEMORY, OUTPUT_TOO_SMALL, INVALID_ARGUMENT = range(11)
@@ -22,7 +22,28 @@ EXPORTS = [
     "gzgpu_ctx_create", "gzgpu_ctx_destroy", "gzgpu_set_tuning", "gzgpu_get_tuning", "gzgpu_last_error",
     "gzgpu_status_string", "gzgpu_decompress", "gzgpu_decompress_to_writer", "gzgpu_decompress_device",
     "gzgpu_decompress_batch", "gzgpu_find_blocks", "gzgpu_crc32", "gzgpu_inflate_piece",
+    "gzgpu_server_create", "gzgpu_server_destroy", "gzgpu_submit", "gzgpu_poll", "gzgpu_fetch",
+    "gzgpu_server_get_stats",
 ]
+PENDING = 11
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_batch_requests", ctypes.c_uint32),
+                ("workers", ctypes.c_uint32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("members", ctypes.c_uint32), ("out_len", ctypes.c_uint64),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("batch", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests_done", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("bytes_in", ctypes.c_uint64),
+                ("bytes_out", ctypes.c_uint64), ("fallbacks", ctypes.c_uint64), ("hbm_peak_bytes", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Tuning(ctypes.Structure):
@@ -99,6 +120,13 @@ def load_library():
         L.gzgpu_find_blocks.argtypes = [vp, vp, sz, u64, u64, P(Boundary), sz, P(sz)]
         L.gzgpu_crc32.argtypes = [vp, vp, sz, P(u32)]
         L.gzgpu_inflate_piece.argtypes = [vp, vp, sz, u64, u64, vp, vp, sz, P(PieceResult)]
+        L.gzgpu_server_create.argtypes = [ctypes.c_int, P(ServerConfig), P(vp)]
+        L.gzgpu_server_destroy.argtypes = [vp]
+        L.gzgpu_server_destroy.restype = None
+        L.gzgpu_submit.argtypes = [vp, vp, sz, P(u64)]
+        L.gzgpu_poll.argtypes = [vp, u64, ctypes.c_int32, P(RequestInfo)]
+        L.gzgpu_fetch.argtypes = [vp, u64, vp, sz, P(sz)]
+        L.gzgpu_server_get_stats.argtypes = [vp, P(ServerStats)]
         for name in EXPORTS:
             getattr(L, name)  # every declared symbol must resolve
         _lib = L

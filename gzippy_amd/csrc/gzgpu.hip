@@ -25,8 +25,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "gzgpu.h"
@@ -1128,6 +1133,7 @@ const char *gzgpu_status_string(int s) {
     case GZGPU_OUT_OF_DEVICE_MEMORY: return "out of device memory";
     case GZGPU_OUTPUT_TOO_SMALL: return "output buffer too small";
     case GZGPU_INVALID_ARGUMENT: return "invalid argument";
+    case GZGPU_PENDING: return "request pending";
     default: return "unknown status";
     }
 }
@@ -1423,3 +1429,5 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
 }
 
 }  // extern "C"
+
+#include "serve.h"

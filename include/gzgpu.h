@@ -44,6 +44,7 @@ typedef enum {
     GZGPU_OUT_OF_DEVICE_MEMORY = 8,
     GZGPU_OUTPUT_TOO_SMALL = 9,    /* out_cap < decoded size; *out_len holds the size needed */
     GZGPU_INVALID_ARGUMENT = 10,
+    GZGPU_PENDING = 11,            /* gzgpu_poll: the request is not finished yet */
 } gzgpu_status;
 
 typedef struct gzgpu_ctx gzgpu_ctx;
@@ -152,6 +153,53 @@ typedef struct {
 } gzgpu_piece_result;
 int gzgpu_inflate_piece(gzgpu_ctx *ctx, const void *d_in, size_t in_len, uint64_t start_bit, uint64_t stop_bit,
                         const uint8_t *window, void *d_out, size_t out_cap_elems, gzgpu_piece_result *res);
+
+/* ---- serving: continuous batching --------------------------------------- */
+/* A request is one gzip stream (one member or several concatenated, like
+ * gzippy::decompress on that buffer: src/lib.rs:176, src/decompress/mod.rs:725-741).
+ * gzgpu_submit copies the input and returns at once.  Worker threads (one HIP
+ * stream each) keep admitting every pending request that fits the HBM budget
+ * into the next device batch, so requests that arrive while others run on the
+ * GPU join the following batch and batches overlap on the device.  Each
+ * result is byte-identical to decoding that request alone.  No counterpart in
+ * the reference, which has no serving loop; it replaces a host-side request
+ * loop calling decompress() per request. */
+typedef struct gzgpu_server gzgpu_server;
+
+typedef struct {
+    uint64_t hbm_budget_bytes;   /* device bytes held for admitted requests (default 200 GB) */
+    uint32_t max_batch_requests; /* requests per device batch (default 4096) */
+    uint32_t workers;            /* batches in flight, one stream each (default 2) */
+} gzgpu_server_config;
+
+typedef struct {
+    int32_t status;              /* gzgpu_status of the request's decode */
+    uint32_t members;            /* gzip members in the request */
+    uint64_t out_len;            /* decoded bytes */
+    double queue_ms;             /* submit -> its batch started */
+    double latency_ms;           /* submit -> result ready */
+    uint64_t batch;              /* sequence number of the batch that ran it */
+} gzgpu_request_info;
+
+typedef struct {
+    uint64_t requests_done;
+    uint64_t batches;
+    uint64_t bytes_in;
+    uint64_t bytes_out;
+    uint64_t fallbacks;          /* requests re-decoded member by member after a split miss */
+    uint64_t hbm_peak_bytes;
+} gzgpu_server_stats;
+
+int gzgpu_server_create(int device, const gzgpu_server_config *cfg, gzgpu_server **out);
+void gzgpu_server_destroy(gzgpu_server *s);
+int gzgpu_submit(gzgpu_server *s, const uint8_t *in, size_t in_len, uint64_t *ticket);
+/* GZGPU_OK and *info when the request finished (its own status in info->status),
+ * GZGPU_PENDING when it did not within timeout_ms (0: do not wait, < 0: wait). */
+int gzgpu_poll(gzgpu_server *s, uint64_t ticket, int32_t timeout_ms, gzgpu_request_info *info);
+/* Copy a finished request's output and release the ticket; GZGPU_OUTPUT_TOO_SMALL
+ * with *out_len set (ticket kept) when cap is short. */
+int gzgpu_fetch(gzgpu_server *s, uint64_t ticket, uint8_t *out, size_t cap, size_t *out_len);
+int gzgpu_server_get_stats(gzgpu_server *s, gzgpu_server_stats *st);
 
 #ifdef __cplusplus
 }
