@@ -148,6 +148,79 @@ def batch_leg(args, dev, ctx):
             "check": "every member's bytes equal the generator's; CRC32 and ISIZE verified per member"}
 
 
+def serve_leg(args, rates):
+    """Open-loop Poisson arrivals at each rate for args.serve_seconds; requests
+    follow the batch mix, 10 % of them 4-64-member concatenations.  Latency is
+    the server's own submit-to-result clock; every result is CRC-checked."""
+    import threading
+    import zlib
+
+    import numpy as np
+
+    import gzippy_amd
+    from gzippy_amd import workloads as W
+
+    rng = np.random.default_rng(args.seed + 4)
+    base = W.batch_members(args.serve_pool, seed=args.seed + 4)
+    pool = []
+    for i in range(args.serve_pool):
+        raw, g, _, level = base[i]
+        if rng.random() < 0.10:
+            # pigz-style: the request's bytes cut into k members, each its own gzip stream
+            k = int(rng.integers(4, 65))
+            cuts = np.linspace(0, raw.size, k + 1).astype(np.int64)
+            gz = b"".join(W.gzip_compress(raw[cuts[j]:cuts[j + 1]], level).tobytes() for j in range(k))
+            pool.append((gz, int(raw.size), zlib.crc32(raw), k))
+        else:
+            pool.append((g.tobytes(), int(raw.size), zlib.crc32(raw), 1))
+    del base
+    buf = np.empty(max(p[1] for p in pool) + 1, dtype=np.uint8)
+    out = {"pool_requests": len(pool), "multi_member_share": round(sum(p[3] > 1 for p in pool) / len(pool), 3),
+           "seconds_per_rate": args.serve_seconds, "rates": []}
+    with gzippy_amd.Server(workers=2) as srv:
+        for rate in rates:
+            n = max(1, int(rate * args.serve_seconds))
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+            tickets = [None] * n
+            ready = threading.Semaphore(0)
+            lat, bad, bytes_out = [], [0], [0]
+            t_start = time.perf_counter()
+
+            def submitter():
+                for i in range(n):
+                    delay = t_start + arrivals[i] - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                    tickets[i] = srv.submit(pool[i % len(pool)][0])
+                    ready.release()
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            for i in range(n):
+                ready.acquire()
+                info = srv.poll(tickets[i], -1)
+                got = srv.fetch_into(tickets[i], buf)
+                exp = pool[i % len(pool)]
+                if info["status"] != 0 or got != exp[1] or zlib.crc32(memoryview(buf)[:got]) != exp[2]:
+                    bad[0] += 1
+                lat.append(info["latency_ms"])
+                bytes_out[0] += got
+            th.join()
+            wall = time.perf_counter() - t_start
+            lat.sort()
+            rec = {"offered_req_per_s": rate, "requests": n, "wall_s": round(wall, 3),
+                   "throughput_gbs": round(bytes_out[0] / wall / 1e9, 3),
+                   "p50_ms": round(lat[len(lat) // 2], 3), "p99_ms": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
+                   "max_ms": round(lat[-1], 3), "mismatches": bad[0]}
+            log(f"serve {rate}/s: {rec}")
+            if bad[0]:
+                log("FAILED: served output differs")
+                sys.exit(1)
+            out["rates"].append(rec)
+        out["server"] = srv.stats()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +234,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-members", type=int, default=10000, help="batched-members leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="serving leg, seconds per rate (0: skip)")
+    ap.add_argument("--serve-pool", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, run one plain device-to-device copy of this many MiB "
                          "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -261,6 +336,9 @@ def main():
         del d_in, d_out
         torch.cuda.empty_cache()
         line["batch"] = batch_leg(args, dev, ctx)
+    if args.serve_seconds > 0:
+        torch.cuda.empty_cache()
+        line["serving"] = serve_leg(args, (200, 1000, 5000))
     traffic = measured_traffic(args)
     if traffic is not None:
         line["roofline"]["traffic"] = traffic["bytes_per_launch"]

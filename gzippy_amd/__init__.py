@@ -301,6 +301,16 @@ class Server:
                               status=st)
         return out[: n.value].tobytes()
 
+    def fetch_into(self, ticket, out):
+        """Copy a finished request's bytes into the uint8 numpy array `out`;
+        returns the byte count (releases the ticket)."""
+        n = ctypes.c_size_t()
+        st = self.lib.gzgpu_fetch(self.h, ticket, out.ctypes.data, out.size, ctypes.byref(n))
+        if st != _native.OK:
+            raise GzippyError(f"request failed: {_native.STATUS.get(st, st)}", kind=_native.STATUS.get(st, str(st)),
+                              status=st)
+        return n.value
+
     def result(self, ticket):
         return self.fetch(ticket, self.poll(ticket, -1))
 
