@@ -538,21 +538,24 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     const uint64_t min_spacing = 8 * 32 * KIB;  // gzip_block_finder.rs:109 (spacing >= 32 KiB)
     const uint64_t slots = (uint64_t)c->num_cus * 4;  // resident u8-ring inflate wavefronts
     // Piece grid.  Explicit pieces_per_chunk: chunk_bits / ppc compressed bits.
-    // Default: aim every piece at about one slot's share of the call's
-    // expected output (ISIZE), so a highly compressible member is cut as
-    // finely as an incompressible one and no wavefront is left with a tail.
-    auto expect_out = [](const Member &M) {
-        const uint64_t cb = (M.de - M.ds) / 8;
-        return (uint64_t)M.exp_isize >= cb / 2 ? (uint64_t)M.exp_isize : 4 * cb;  // ISIZE is mod 2^32
+    // Default: balance decode time.  A wavefront decodes Huffman-coded input
+    // at a roughly constant rate of compressed bits, whatever the ratio;
+    // stored data (ratio below 1.15, the reference's StoredParallel cut,
+    // decompress/mod.rs:143-151) moves at copy speed, weighted 1/10.  Every
+    // piece aims at one resident slot's share of the call's total cost.
+    auto cost_of = [](const Member &M) -> uint64_t {
+        const uint64_t cbits = M.de - M.ds;
+        const bool stored_like = (uint64_t)M.exp_isize * 100 < (cbits / 8) * 115;  // ISIZE mod 2^32: low only if so
+        return stored_like ? cbits / 10 : cbits;
     };
-    uint64_t total_out = 0;
+    uint64_t total_cost = 0;
     for (auto &M : mem)
-        if (M.status == GZGPU_OK) total_out += expect_out(M);
-    const uint64_t target_out = std::max<uint64_t>(256 * KIB, total_out / (slots - slots / 16));
+        if (M.status == GZGPU_OK) total_cost += cost_of(M);
+    const uint64_t target_cost = std::max<uint64_t>(min_spacing, total_cost / (slots - slots / 16));
     const uint64_t ppc = c->tune.pieces_per_chunk;
     auto spacing_of = [&](const Member &M) -> uint64_t {
         if (ppc) return std::max<uint64_t>(chunk_bits / ppc, min_spacing);
-        const uint64_t npieces = (expect_out(M) + target_out - 1) / target_out;
+        const uint64_t npieces = (cost_of(M) + target_cost - 1) / target_cost;
         if (npieces <= 1) return ~0ull;
         return std::max<uint64_t>((M.de - M.ds) / npieces, min_spacing);
     };

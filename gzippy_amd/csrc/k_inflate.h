@@ -528,6 +528,7 @@ struct Lookahead {
     uint32_t olen;   // entries the symbol produces (0 for end-of-block)
     uint32_t dist;   // copy distance (length codes)
     uint32_t nxt;    // lane of the following symbol (>= 64: past this window)
+    uint32_t step;   // nxt, plus bit 8 when the walk stops there (past the window, or not a complete code)
     uint64_t CONT;   // lanes holding a complete literal or length+distance code
     uint64_t EOBM;   // lanes holding end-of-block
     uint64_t LITM;   // lanes holding a literal
@@ -568,6 +569,9 @@ __device__ __attribute__((always_inline)) void lookahead(SH &sh, const uint8_t *
     la.CONT = ballot(ok && kind != K_EOB);
     la.EOBM = ballot(ok && kind == K_EOB);
     la.LITM = ballot(ok && kind == K_LIT);
+    const uint32_t nx = la.nxt;
+    const bool go_on = nx < 64 && ((la.CONT >> (nx & 63)) & 1);
+    la.step = nx | (go_on ? 0u : 0x100u);
 }
 
 // Decode the Huffman symbols of the current block until end-of-block, from
@@ -616,14 +620,17 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             break;
         }
         const uint64_t CONT = uni64(la.CONT), EOBM = uni64(la.EOBM), LITM = uni64(la.LITM);
-        // ---- walk the chain of symbol starts from lane 0
+        // ---- walk the chain of symbol starts from lane 0: each lane's
+        // `step` names the next start and whether the walk continues there
         uint64_t M = 0;
         uint32_t s = 0, eob = 0;
-        for (;;) {
-            if (!((CONT >> s) & 1)) break;  // s < 64 here
-            M |= 1ull << s;
-            s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)s));
-            if (s >= 64) break;
+        if (CONT & 1) {
+            for (;;) {
+                M |= 1ull << s;
+                const uint32_t v = uni((uint32_t)__builtin_amdgcn_readlane((int)la.step, (int)s));
+                s = v & 0xFF;
+                if (v & 0x100) break;
+            }
         }
         if (s < 64 && ((EOBM >> s) & 1)) {
             M |= 1ull << s;
