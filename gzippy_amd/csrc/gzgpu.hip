@@ -585,26 +585,37 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         if (!M.split || spacing == ~0ull || M.de <= M.ds + spacing) continue;
         std::vector<gzgpu_boundary> hits = find_hits(c, d_in, in_len, M.ds + 1, M.de);
         c->st.candidates += (uint32_t)hits.size();
-        // piece starts: per grid point the first hit past it; a stored hit
-        // only when the block after its payload also parses as a header
-        // (info bit 0), which keeps LEN/~NLEN coincidences inside compressed
-        // data from becoming starts.  A scheduling choice only: any start is
-        // verified by the predecessor's end.
+        // piece starts: per grid point the hit nearest to it within half a
+        // spacing either way (pieces come out near equal, and the last one
+        // to finish sets the launch time); a dynamic-block hit beats a
+        // stored one, and a stored hit counts only when the block after its
+        // payload also parses as a header (info bit 0), which keeps LEN/~NLEN
+        // coincidences inside compressed data from becoming starts.  A
+        // scheduling choice only: any start is verified by the predecessor's
+        // end.
         size_t h = 0;
         uint64_t last = M.ds;
         std::vector<size_t> chosen;
         auto eligible = [&](size_t i) { return hits[i].kind == 2 || (hits[i].info & 1); };
         for (uint64_t g = M.ds + spacing; g < M.de; g += spacing) {
-            while (h < hits.size() && (hits[h].bit_offset < g || hits[h].bit_offset <= last || !eligible(h))) h++;
-            if (h >= hits.size()) break;
-            // within this grid cell a dynamic hit beats a stored one
-            size_t pick = h;
-            for (size_t j = h; j < hits.size() && hits[j].bit_offset < g + spacing && hits[pick].kind != 2; j++)
-                if (hits[j].kind == 2) pick = j;
-            if (hits[pick].seek_bit >= M.de) break;
-            chosen.push_back(pick);
-            last = hits[pick].bit_offset;
-            h = pick + 1;
+            const uint64_t lo = std::max<uint64_t>(last + 1, g - spacing / 2), hi = g + spacing / 2;
+            while (h < hits.size() && hits[h].bit_offset < lo) h++;
+            size_t best = hits.size();
+            uint64_t bestd = ~0ull;
+            for (int pass = 0; pass < 2 && best == hits.size(); pass++)
+                for (size_t j = h; j < hits.size() && hits[j].bit_offset < hi; j++) {
+                    if (pass == 0 ? hits[j].kind != 2 : !eligible(j)) continue;
+                    const uint64_t d = hits[j].bit_offset > g ? hits[j].bit_offset - g : g - hits[j].bit_offset;
+                    if (d < bestd) {
+                        bestd = d;
+                        best = j;
+                    }
+                }
+            if (best == hits.size()) continue;  // no boundary near this grid point
+            if (hits[best].seek_bit >= M.de) break;
+            chosen.push_back(best);
+            last = hits[best].bit_offset;
+            h = best + 1;
         }
         const uint64_t total_bits = M.de - M.ds;
         for (size_t k = 0; k < chosen.size(); k++) {
