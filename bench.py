@@ -99,6 +99,10 @@ def batch_leg(args, dev, ctx):
     del raws, ins, mem
     torch.cuda.synchronize()
 
+    # autotuned tuning for this mix (device time, about 3 s budget)
+    tuned, tuned_ms = gzippy_amd.autotune(d_in, in_off, in_len, d_out, out_off, out_cap, budget_ms=3000, ctx=ctx)
+    log(f"batch autotune: chunk {tuned.chunk_bytes} ppc {tuned.pieces_per_chunk} waves/CU {tuned.waves_per_cu} "
+        f"split {tuned.batch_split_bytes} -> {tuned_ms:.1f} ms")
     # batched: every member in one pipeline (one persistent inflate launch)
     d_out.zero_()
     status, olen, crcs, st = gzippy_amd.decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=ctx)
@@ -142,6 +146,8 @@ def batch_leg(args, dev, ctx):
             "batched_gbs": round(D / b_s / 1e9, 3), "batched_ms": round(b_s * 1e3, 3),
             "plain_per_member_gbs": round(D / p_s / 1e9, 3), "plain_per_member_ms": round(p_s * 1e3, 3),
             "batched_over_plain": round(p_s / b_s, 2), "pieces": st["pieces"],
+            "autotuned": {"chunk_bytes": tuned.chunk_bytes, "pieces_per_chunk": tuned.pieces_per_chunk,
+                          "waves_per_cu": tuned.waves_per_cu, "batch_split_bytes": tuned.batch_split_bytes},
             "batched_phase_ms": {k: round(st[k], 3) for k in ("ms_find", "ms_inflate", "ms_resolve", "ms_crc",
                                                               "ms_total", "ms_inflate_marker", "ms_inflate_clean")},
             "batched_launches": st["launches"],
@@ -232,6 +238,7 @@ def main():
     ap.add_argument("--pieces-per-chunk", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0x675A0001)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", type=int, default=1, help="pick the tuning by measurement before timing")
     ap.add_argument("--batch-members", type=int, default=10000, help="batched-members leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="serving leg, seconds per rate (0: skip)")
@@ -264,6 +271,12 @@ def main():
     ctx = gzippy_amd.context(0)
     ctx.set_tuning(chunk_bytes=args.chunk_kib << 10, pieces_per_chunk=args.pieces_per_chunk)
     torch.cuda.synchronize()
+    tuning = None
+    if args.autotune:
+        t, tms = gzippy_amd.autotune(d_in, [0], [gz.size], d_out, [0], [n], budget_ms=1500, ctx=ctx)
+        tuning = {"chunk_bytes": t.chunk_bytes, "pieces_per_chunk": t.pieces_per_chunk, "waves_per_cu": t.waves_per_cu,
+                  "autotune_ms": round(tms, 3)}
+        log(f"autotune: {tuning}")
 
     stats = None
     for i in range(args.warmup):
@@ -323,6 +336,7 @@ def main():
             "inflate_launches_per_step": launches / args.steps,
             "phase_ms": {k: round(stats[k], 3) for k in ("ms_find", "ms_inflate", "ms_resolve", "ms_crc", "ms_total")},
             "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3)},
+            "tuning": tuning,
         },
         "roofline": {"bound": "hbm", "kernel": "k_inflate<unsigned char>", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,

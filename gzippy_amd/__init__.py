@@ -30,7 +30,7 @@ from ._native import GzgpuError, Stats, Tuning  # noqa: F401
 __all__ = [
     "GzippyError", "decompress", "decompress_with_threads", "decompress_to_writer",
     "decompress_to_writer_with_threads", "decompress_device", "decompress_batch", "find_blocks", "crc32",
-    "inflate_piece", "context", "Server",
+    "inflate_piece", "context", "Server", "autotune",
 ]
 
 
@@ -182,6 +182,23 @@ def decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None, ra
     if st != _native.OK and raise_on_error:
         _raise(ctx, st)
     return status, olen, crcs, st_obj.as_dict()
+
+
+def autotune(d_in, in_off, in_len, d_out, out_off, out_cap, budget_ms=2000, ctx=None):
+    """Pick the fastest tuning for these members (device time) and keep it in
+    the context; returns (Tuning, ms)."""
+    ctx = ctx or context(d_in.device.index or 0)
+    _sync_torch(d_in)
+    a = [np.ascontiguousarray(np.asarray(x, dtype=np.uint64)) for x in (in_off, in_len, out_off, out_cap)]
+    P64 = ctypes.POINTER(ctypes.c_uint64)
+    best = Tuning()
+    ms = ctypes.c_double()
+    st = ctx.lib.gzgpu_autotune(ctx.h, _ptr(d_in), a[0].ctypes.data_as(P64), a[1].ctypes.data_as(P64), _ptr(d_out),
+                                a[2].ctypes.data_as(P64), a[3].ctypes.data_as(P64), len(in_off), int(budget_ms),
+                                ctypes.byref(best), ctypes.byref(ms))
+    if st != _native.OK:
+        _raise(ctx, st)
+    return best, ms.value
 
 
 def find_blocks(d_in, start_bit=0, end_bit=None, in_len=None, ctx=None):

@@ -23,7 +23,7 @@ EXPORTS = [
     "gzgpu_status_string", "gzgpu_decompress", "gzgpu_decompress_to_writer", "gzgpu_decompress_device",
     "gzgpu_decompress_batch", "gzgpu_find_blocks", "gzgpu_crc32", "gzgpu_inflate_piece",
     "gzgpu_server_create", "gzgpu_server_destroy", "gzgpu_submit", "gzgpu_poll", "gzgpu_fetch",
-    "gzgpu_server_get_stats",
+    "gzgpu_server_get_stats", "gzgpu_autotune",
 ]
 PENDING = 11
 
@@ -127,6 +127,8 @@ def load_library():
         L.gzgpu_poll.argtypes = [vp, u64, ctypes.c_int32, P(RequestInfo)]
         L.gzgpu_fetch.argtypes = [vp, u64, vp, sz, P(sz)]
         L.gzgpu_server_get_stats.argtypes = [vp, P(ServerStats)]
+        L.gzgpu_autotune.argtypes = [vp, vp, P(u64), P(u64), vp, P(u64), P(u64), sz, ctypes.c_uint32, P(Tuning),
+                                     P(ctypes.c_double)]
         for name in EXPORTS:
             getattr(L, name)  # every declared symbol must resolve
         _lib = L

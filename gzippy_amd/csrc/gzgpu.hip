@@ -1124,11 +1124,7 @@ int gzgpu_set_tuning(gzgpu_ctx *c, const gzgpu_tuning *t) {
 
 int gzgpu_get_tuning(gzgpu_ctx *c, gzgpu_tuning *t) {
     if (!c || !t) return GZGPU_INVALID_ARGUMENT;
-    *t = c->tune;
-    if (!t->chunk_bytes) t->chunk_bytes = 512 * KIB;
-    if (!t->pieces_per_chunk) t->pieces_per_chunk = 8;
-    if (!t->waves_per_cu) t->waves_per_cu = 16;
-
+    *t = c->tune;  // as set: 0 fields stay 0 (automatic)
     return GZGPU_OK;
 }
 
@@ -1305,6 +1301,52 @@ int gzgpu_decompress_batch(gzgpu_ctx *c, const void *d_in, const uint64_t *in_of
         if (stats) *stats = c->st;
         return first_bad;
     });
+}
+
+int gzgpu_autotune(gzgpu_ctx *c, const void *d_in, const uint64_t *in_off, const uint64_t *in_len, void *d_out,
+                   const uint64_t *out_off, const uint64_t *out_cap, size_t n, uint32_t budget_ms,
+                   gzgpu_tuning *best, double *best_ms) {
+    if (!c || !n || !d_in || !d_out || !in_off || !in_len || !out_off || !out_cap) return GZGPU_INVALID_ARGUMENT;
+    // candidates: the planner's own grid (piece per resident slot, by decode
+    // cost) at two occupancies, then explicit grids; batch split rules
+    std::vector<gzgpu_tuning> cand;
+    for (uint64_t split : {(uint64_t)0, (uint64_t)(1 * MIB), (uint64_t)(4 * MIB)}) {
+        cand.push_back({0, 0, 0, split});
+        if (split) continue;
+        cand.push_back({0, 0, 3, split});
+        for (uint64_t chunk : {256 * KIB, 512 * KIB, 1024 * KIB})
+            for (uint32_t ppc : {4u, 8u, 16u}) cand.push_back({chunk, ppc, 0, split});
+    }
+    const gzgpu_tuning keep = c->tune;
+    std::vector<int32_t> status(n);
+    std::vector<uint64_t> olen(n);
+    std::vector<uint32_t> crc(n);
+    double best_t = 1e30, spent = 0;
+    gzgpu_tuning pick = keep;
+    for (auto &t : cand) {
+        if (spent > (double)budget_ms && best_t < 1e30) break;
+        c->tune = t;
+        double tmin = 1e30;
+        for (int rep = 0; rep < 2; rep++) {  // first run warms the buffers
+            gzgpu_stats st{};
+            int r = gzgpu_decompress_batch(c, d_in, in_off, in_len, d_out, out_off, out_cap, n, status.data(),
+                                           olen.data(), crc.data(), &st);
+            if (r != GZGPU_OK) {
+                c->tune = keep;
+                return r;
+            }
+            spent += st.ms_total;
+            if (rep) tmin = std::min(tmin, st.ms_total);
+        }
+        if (tmin < best_t) {
+            best_t = tmin;
+            pick = t;
+        }
+    }
+    c->tune = pick;
+    if (best) *best = pick;
+    if (best_ms) *best_ms = best_t;
+    return GZGPU_OK;
 }
 
 int gzgpu_find_blocks(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t start_bit, uint64_t end_bit,

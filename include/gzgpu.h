@@ -49,11 +49,14 @@ typedef enum {
 
 typedef struct gzgpu_ctx gzgpu_ctx;
 
-/* Tunables (all 0 = library default / autotuned). */
+/* Tunables; a 0 field is automatic and gzgpu_get_tuning returns it as 0.
+ * pieces_per_chunk 0: pieces sized so that each resident wave slot gets an
+ * equal share of the call's decode cost (chunk_bytes unused); otherwise a
+ * fixed grid of chunk_bytes / pieces_per_chunk. */
 typedef struct {
-    uint64_t chunk_bytes;       /* speculation grid, compressed bytes per chunk (default 512 KiB) */
+    uint64_t chunk_bytes;       /* speculation grid, compressed bytes per chunk (0: 512 KiB) */
     uint32_t pieces_per_chunk;  /* wavefronts per chunk: sub-splits at found block boundaries */
-    uint32_t waves_per_cu;      /* persistent-grid occupancy target */
+    uint32_t waves_per_cu;      /* persistent-grid occupancy cap (0: as many as LDS allows) */
     uint64_t batch_split_bytes; /* batch: split only members above this compressed size (0: by expected output) */
 } gzgpu_tuning;
 
@@ -121,6 +124,18 @@ int gzgpu_decompress_device(gzgpu_ctx *ctx, const void *d_in, size_t in_len, voi
 int gzgpu_decompress_batch(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_off, const uint64_t *in_len,
                            void *d_out, const uint64_t *out_off, const uint64_t *out_cap, size_t n,
                            int32_t *status, uint64_t *out_len, uint32_t *crc, gzgpu_stats *stats);
+
+/* ---- autotuner ---------------------------------------------------------- */
+/* Times candidate tunings (piece grid, wavefronts per chunk, waves per CU,
+ * batch split rule) on the caller's own members, laid out as for
+ * gzgpu_decompress_batch (n = 1 for a single member), keeps the fastest by
+ * device time in the context and returns it in *best.  d_out is overwritten.
+ * Stops after about budget_ms of device time.  No counterpart in the
+ * reference, whose chunk size comes from a fixed formula
+ * (adjusted_chunk_size_bytes, single_member.rs:128). */
+int gzgpu_autotune(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_off, const uint64_t *in_len, void *d_out,
+                   const uint64_t *out_off, const uint64_t *out_cap, size_t n, uint32_t budget_ms,
+                   gzgpu_tuning *best, double *best_ms);
 
 /* ---- exposed kernels (parity tests, tools) ------------------------------- */
 typedef struct {

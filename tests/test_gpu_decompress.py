@@ -203,3 +203,35 @@ def test_unaligned_input_pointers(gpu, fine_grid):
             if ref_hits is None:
                 ref_hits = hits
             assert hits == ref_hits, shift
+
+
+def test_autotune_keeps_output_exact(gpu):
+    """The autotuner's pick is one of its candidates, stays in the context,
+    and every member still decodes bit-exact under it."""
+    import torch
+    ctx = gpu.context()
+    old = ctx.tuning()
+    try:
+        mem = W.batch_members(40, seed=77, max_size=2 << 20)
+        ins = [m[1] for m in mem]
+        raws = [m[0] for m in mem]
+        in_off = np.cumsum([0] + [x.size for x in ins[:-1]])
+        in_len = np.array([x.size for x in ins])
+        out_cap = np.array([r.size for r in raws])
+        out_off = np.cumsum([0] + list(out_cap[:-1]))
+        d_in = dev(np.concatenate(ins))
+        d_out = torch.zeros(int(out_cap.sum()) + 1, dtype=torch.uint8, device="cuda")
+        best, ms = gpu.autotune(d_in, in_off, in_len, d_out, out_off, out_cap, budget_ms=500)
+        say("autotune", best.chunk_bytes, best.pieces_per_chunk, best.waves_per_cu, best.batch_split_bytes, ms)
+        assert ms > 0
+        now = ctx.tuning()
+        assert (now.chunk_bytes, now.pieces_per_chunk, now.waves_per_cu, now.batch_split_bytes) == \
+            (best.chunk_bytes, best.pieces_per_chunk, best.waves_per_cu, best.batch_split_bytes)
+        d_out.zero_()
+        status, olen, crc, _ = gpu.decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap)
+        assert (status == 0).all()
+        host = d_out.cpu().numpy()
+        for i, r in enumerate(raws):
+            assert np.array_equal(host[out_off[i]: out_off[i] + r.size], r), i
+    finally:
+        ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
