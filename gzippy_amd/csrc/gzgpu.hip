@@ -263,7 +263,7 @@ struct gzgpu_ctx {
     gzgpu_tuning tune{};
     std::string err;
     DevBuf d_in, d_out, d_pieces, d_pout, d_order, d_counters, d_surv, d_hits, d_slab, d_crc_tab, d_seg_off,
-        d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len;
+        d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len, d_bf_ranges;
     std::vector<DevBuf> extra_slabs;
     PinnedArena pin;
     hipEvent_t ev[10]{};
@@ -301,14 +301,24 @@ T *download(gzgpu_ctx *c, const void *d, size_t count) {
     return h;
 }
 
-// ---- block finder over [start_bit, end_bit) of d_in -----------------------
-std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, uint64_t start_bit,
-                                      uint64_t end_bit) {
+// ---- block finder over ranges [start_bit, end_bit) of d_in ----------------
+// One stage-A launch over every range; hits of all ranges, sorted, duplicates
+// (ranges that overlap) removed.  Every hit lies inside the range it came from.
+std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len,
+                                      const std::vector<std::pair<uint64_t, uint64_t>> &rng) {
     std::vector<gzgpu_boundary> out;
-    if (end_bit <= start_bit) return out;
-    uint64_t nbits = end_bit - start_bit;
+    std::vector<BfRange> table;
+    uint64_t nchunk = 0, nbits = 0;
+    for (auto &r : rng) {
+        if (r.second <= r.first) continue;
+        table.push_back({r.first, r.second, nchunk});
+        nchunk += (r.second - (r.first & ~31ull) + 31) / 32;
+        nbits += r.second - r.first;
+    }
+    if (table.empty()) return out;
     uint32_t surv_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(nbits / 64, 4096), 1ull << 28);
     uint32_t hit_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(nbits / 1024, 4096), 1ull << 26);
+    upload(c, c->d_bf_ranges, table.data(), table.size() * sizeof(BfRange));
     for (int attempt = 0; attempt < 3; attempt++) {
         c->d_counters.ensure(64);
         c->d_surv.ensure((size_t)surv_cap * 8);
@@ -316,11 +326,10 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
         HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
         uint32_t *cnt = c->d_counters.as<uint32_t>();
         // stage A: one lane per 32 bit offsets (1 KiB of input per workgroup pass)
-        uint64_t chunks = (end_bit - (start_bit & ~31ull) + 31) / 32;
-        uint64_t blocks = std::min<uint64_t>((chunks + 255) / 256, (uint64_t)c->num_cus * 16);
-        hipLaunchKernelGGL(k_bf_stage_a, dim3((uint32_t)blocks), dim3(256), 0, c->stream, d_in, in_len, start_bit, end_bit,
-                           c->d_surv.as<uint64_t>(), surv_cap, cnt + 0, c->d_hits.as<gzgpu_boundary>(), hit_cap,
-                           cnt + 1);
+        uint64_t blocks = std::min<uint64_t>((nchunk + 255) / 256, (uint64_t)c->num_cus * 16);
+        hipLaunchKernelGGL(k_bf_stage_a, dim3((uint32_t)blocks), dim3(256), 0, c->stream, d_in, in_len,
+                           c->d_bf_ranges.as<BfRange>(), (uint32_t)table.size(), nchunk, c->d_surv.as<uint64_t>(),
+                           surv_cap, cnt + 0, c->d_hits.as<gzgpu_boundary>(), hit_cap, cnt + 1);
         HIPCHK(hipGetLastError());
         uint32_t *h_cnt = download<uint32_t>(c, cnt, 2);
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -348,12 +357,24 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
             HIPCHK(hipMemcpyAsync(out.data(), c->d_hits.p, out.size() * sizeof(gzgpu_boundary), hipMemcpyDeviceToHost,
                                   c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        std::sort(out.begin(), out.end(), [](const gzgpu_boundary &a, const gzgpu_boundary &b) {
+        auto lt = [](const gzgpu_boundary &a, const gzgpu_boundary &b) {
             return a.bit_offset != b.bit_offset ? a.bit_offset < b.bit_offset : a.seek_bit < b.seek_bit;
-        });
+        };
+        std::sort(out.begin(), out.end(), lt);
+        if (table.size() > 1)
+            out.erase(std::unique(out.begin(), out.end(),
+                                  [](const gzgpu_boundary &a, const gzgpu_boundary &b) {
+                                      return a.bit_offset == b.bit_offset && a.seek_bit == b.seek_bit;
+                                  }),
+                      out.end());
         return out;
     }
     throw StatusError{GZGPU_OUT_OF_DEVICE_MEMORY, "block finder: candidate buffers kept overflowing"};
+}
+
+std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, uint64_t start_bit,
+                                      uint64_t end_bit) {
+    return find_hits(c, d_in, in_len, std::vector<std::pair<uint64_t, uint64_t>>{{start_bit, end_bit}});
 }
 
 uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
@@ -370,13 +391,22 @@ bool debug_enabled() {
 }
 
 // One launch of the persistent inflate kernel over `descs` (pulled
-// longest-first by `cost`); returns one PieceOut per desc.
+// longest-first by `cost`) on the context stream; finish_inflate waits for
+// it and reads the per-piece results back.
+struct InflateLaunch {
+    const PieceOut *hro = nullptr;
+    uint32_t n = 0, grid = 0;
+    bool u16 = false;
+};
+
 template <typename RT>
-std::vector<PieceOut> run_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len,
-                                  const std::vector<PieceDesc> &descs, const std::vector<uint64_t> &cost, float &ms) {
-    const uint32_t n = (uint32_t)descs.size();
-    std::vector<uint32_t> order(n);
-    for (uint32_t i = 0; i < n; i++) order[i] = i;
+InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, const std::vector<PieceDesc> &descs,
+                              const std::vector<uint64_t> &cost) {
+    InflateLaunch L;
+    L.n = (uint32_t)descs.size();
+    L.u16 = sizeof(RT) == 2;
+    std::vector<uint32_t> order(L.n);
+    for (uint32_t i = 0; i < L.n; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
     upload(c, c->d_pieces, descs.data(), descs.size() * sizeof(PieceDesc));
     upload(c, c->d_order, order.data(), order.size() * 4);
@@ -384,18 +414,23 @@ std::vector<PieceOut> run_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in
     c->d_counters.ensure(64);
     HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
     // LDS bounds residency (160 KiB per CU): u16 ring -> 2 waves/CU, u8 ring -> 4
-    uint32_t per_cu = sizeof(RT) == 2 ? 2 : 4;
+    uint32_t per_cu = L.u16 ? 2 : 4;
     if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
-    uint32_t grid = (uint32_t)std::min<uint64_t>(n, (uint64_t)c->num_cus * per_cu);
+    L.grid = (uint32_t)std::min<uint64_t>(L.n, (uint64_t)c->num_cus * per_cu);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    hipLaunchKernelGGL(k_inflate<RT>, dim3(grid), dim3(64), 0, c->stream, d_in, in_len, c->d_pieces.as<PieceDesc>(),
-                       c->d_order.as<uint32_t>(), n, c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
+    hipLaunchKernelGGL(k_inflate<RT>, dim3(L.grid), dim3(64), 0, c->stream, d_in, in_len, c->d_pieces.as<PieceDesc>(),
+                       c->d_order.as<uint32_t>(), L.n, c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     c->st.launches++;
-    const PieceOut *hro = download<PieceOut>(c, c->d_pout.p, n);
+    L.hro = download<PieceOut>(c, c->d_pout.p, L.n);
+    return L;
+}
+
+std::vector<PieceOut> finish_inflate(gzgpu_ctx *c, const InflateLaunch &L, const std::vector<PieceDesc> &descs,
+                                     float &ms) {
     HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<PieceOut> ro(hro, hro + n);
+    std::vector<PieceOut> ro(L.hro, L.hro + L.n);
     ms = ev_ms(c->ev[5], c->ev[6]);
     c->st.ms_inflate_kernel += ms;
     c->st.inflate_launches++;
@@ -415,12 +450,12 @@ std::vector<PieceOut> run_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in
             if (ro[i].t_end - ro[i].t_begin > ro[slow].t_end - ro[slow].t_begin) slow = i;
         std::sort(us.begin(), us.end());
         fprintf(stderr,
-                "[gzgpu] inflate<%s> pieces=%u grid=%u kernel=%.3f ms span=%.3f ms piece_us min=%.1f p50=%.1f "
+                "[gzgpu] inflate<%s> pieces=%u grid=%u kernel=%.3f ms t0=%.1f us span=%.3f ms piece_us min=%.1f p50=%.1f "
                 "max=%.1f entries=%llu (u16 %llu) flipped=%llu per-wave MB/s p50=%.1f | slowest: %llu entries, "
                 "%llu bits, %u blocks, flags %u, status %u\n",
-                sizeof(RT) == 2 ? "u16" : "u8", n, grid, ms, (double)(t1 - t0) / 1e5, us.front(), us[us.size() / 2],
+                L.u16 ? "u16" : "u8", L.n, L.grid, ms, (double)(t0 % 100000000ull) / 100.0, (double)(t1 - t0) / 1e5, us.front(), us[us.size() / 2],
                 us.back(), (unsigned long long)bytes, (unsigned long long)n16, (unsigned long long)flipped,
-                us[us.size() / 2] > 0 ? (double)bytes / n / us[us.size() / 2] : 0.0,
+                us[us.size() / 2] > 0 ? (double)bytes / L.n / us[us.size() / 2] : 0.0,
                 (unsigned long long)(ro[slow].n16 + ro[slow].n8), (unsigned long long)ro[slow].bits_done,
                 ro[slow].nblocks, descs[slow].flags, ro[slow].status);
     }
@@ -429,7 +464,11 @@ std::vector<PieceOut> run_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in
 
 // Inflate the pieces in `todo`; results land in R.  Marker pieces run in the
 // u16-ring kernel until their flip; the rest of every flipped piece and all
-// window-known pieces then run in the u8-ring kernel at twice the occupancy.
+// window-known pieces (a member's first piece, re-decodes) then run in one
+// u8-ring launch at twice the occupancy, longest first.  (Running the
+// window-known pieces on a second stream beside the marker launch gains
+// nothing: two u16 waves fill a CU's LDS, and two u8 launches pack worse
+// than one.)
 void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<PieceHost> &P,
                     const std::vector<Member> &mem, std::vector<uint32_t> &todo, std::vector<PieceOut> &R) {
     auto stop_of = [&](const PieceDesc &d) { return d.stop_bit == NO_STOP ? mem[d.member].de : d.stop_bit; };
@@ -445,7 +484,8 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
             cost.push_back(stop_of(d) - d.start_bit);
         }
         float ms = 0;
-        std::vector<PieceOut> r1 = run_inflate<uint16_t>(c, d_in, in_len, d1, cost, ms);
+        InflateLaunch l1 = enqueue_inflate<uint16_t>(c, d_in, in_len, d1, cost);
+        std::vector<PieceOut> r1 = finish_inflate(c, l1, d1, ms);
         c->st.ms_inflate_marker += ms;
         for (size_t i = 0; i < marker_idx.size(); i++) R[marker_idx[i]] = r1[i];
     }
@@ -477,7 +517,8 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
     }
     if (d2.empty()) return;
     float ms = 0;
-    std::vector<PieceOut> r2 = run_inflate<uint8_t>(c, d_in, in_len, d2, cost, ms);
+    InflateLaunch l2 = enqueue_inflate<uint8_t>(c, d_in, in_len, d2, cost);
+    std::vector<PieceOut> r2 = finish_inflate(c, l2, d2, ms);
     c->st.ms_inflate_clean += ms;
     for (size_t i = 0; i < d2.size(); i++) {
         const uint32_t pi = who[i];
@@ -561,7 +602,17 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     };
 
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    // ---- 2/3: find + plan
+    // ---- 2/3: find (one pass over every member to be split) + plan
+    auto splits = [&](const Member &M) {
+        const uint64_t spacing = spacing_of(M);
+        return M.status == GZGPU_OK && M.split && spacing != ~0ull && M.de > M.ds + spacing;
+    };
+    std::vector<std::pair<uint64_t, uint64_t>> ranges;
+    for (auto &M : mem)
+        if (splits(M)) ranges.push_back({M.ds + 1, M.de});
+    const std::vector<gzgpu_boundary> all_hits = find_hits(c, d_in, in_len, ranges);
+    c->st.candidates += (uint32_t)all_hits.size();
+    auto hit_lt = [](const gzgpu_boundary &a, uint64_t bit) { return a.bit_offset < bit; };
     std::vector<PieceHost> P;
     for (uint32_t m = 0; m < mem.size(); m++) {
         Member &M = mem[m];
@@ -581,22 +632,34 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         M.pieces.clear();
         M.pieces.push_back((uint32_t)P.size());
         P.push_back(p0);
+        if (!splits(M)) continue;
         const uint64_t spacing = spacing_of(M);
-        if (!M.split || spacing == ~0ull || M.de <= M.ds + spacing) continue;
-        std::vector<gzgpu_boundary> hits = find_hits(c, d_in, in_len, M.ds + 1, M.de);
-        c->st.candidates += (uint32_t)hits.size();
+        const std::vector<gzgpu_boundary> hits(
+            std::lower_bound(all_hits.begin(), all_hits.end(), M.ds + 1, hit_lt),
+            std::lower_bound(all_hits.begin(), all_hits.end(), M.de, hit_lt));
         // piece starts: per grid point the hit nearest to it within half a
         // spacing either way (pieces come out near equal, and the last one
-        // to finish sets the launch time); a dynamic-block hit beats a
-        // stored one, and a stored hit counts only when the block after its
-        // payload also parses as a header (info bit 0), which keeps LEN/~NLEN
-        // coincidences inside compressed data from becoming starts.  A
-        // scheduling choice only: any start is verified by the predecessor's
-        // end.
+        // to finish sets the launch time).  Preference: a dynamic-block hit;
+        // else a stored hit whose payload ends where another hit's accepted
+        // range begins (a final stored block's range starts one bit late, past
+        // its BFINAL bit) or at the member's end.  That rules out LEN/~NLEN
+        // coincidences inside stored data and the copy of a full block's
+        // LEN/NLEN seen one byte early (00 FF FF 00 00 also reads as LEN
+        // 0xFF00, NLEN 0x00FF), which both cost a re-decode round when
+        // chosen.  A scheduling choice only: any start is verified by the
+        // predecessor's end.
         size_t h = 0;
         uint64_t last = M.ds;
         std::vector<size_t> chosen;
-        auto eligible = [&](size_t i) { return hits[i].kind == 2 || (hits[i].info & 1); };
+        auto chained = [&](size_t i) {
+            const uint64_t o = hits[i].seek_bit + 3;  // the LEN field
+            const uint64_t next = (o / 8 + 4 + (hits[i].info >> 16)) * 8;
+            if (next == M.de) return true;
+            auto k = std::lower_bound(hits.begin(), hits.end(), next > 16 ? next - 16 : 0, hit_lt);
+            for (; k != hits.end() && k->bit_offset <= next + 1; ++k)
+                if (k->seek_bit >= next) return true;
+            return false;
+        };
         for (uint64_t g = M.ds + spacing; g < M.de; g += spacing) {
             const uint64_t lo = std::max<uint64_t>(last + 1, g - spacing / 2), hi = g + spacing / 2;
             while (h < hits.size() && hits[h].bit_offset < lo) h++;
@@ -604,7 +667,8 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             uint64_t bestd = ~0ull;
             for (int pass = 0; pass < 2 && best == hits.size(); pass++)
                 for (size_t j = h; j < hits.size() && hits[j].bit_offset < hi; j++) {
-                    if (pass == 0 ? hits[j].kind != 2 : !eligible(j)) continue;
+                    const bool ok = pass == 0 ? hits[j].kind == 2 : hits[j].kind == 0 && chained(j);
+                    if (!ok) continue;
                     const uint64_t d = hits[j].bit_offset > g ? hits[j].bit_offset - g : g - hits[j].bit_offset;
                     if (d < bestd) {
                         bestd = d;
@@ -1106,7 +1170,7 @@ void gzgpu_ctx_destroy(gzgpu_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf *b : {&c->d_in, &c->d_out, &c->d_pieces, &c->d_pout, &c->d_order, &c->d_counters, &c->d_surv, &c->d_hits,
                       &c->d_slab, &c->d_crc_tab, &c->d_seg_off, &c->d_seg_len, &c->d_seg_crc, &c->d_mseg, &c->d_mlen,
-                      &c->d_mcrc, &c->d_rp, &c->d_tiles, &c->d_chains, &c->d_env, &c->d_env_off, &c->d_env_len})
+                      &c->d_mcrc, &c->d_rp, &c->d_tiles, &c->d_chains, &c->d_env, &c->d_env_off, &c->d_env_len, &c->d_bf_ranges})
         b->release();
     for (auto &b : c->extra_slabs) b.release();
     c->pin.release();

@@ -109,31 +109,45 @@ __device__ __forceinline__ bool stored_hit(const uint8_t *__restrict__ in, uint6
     return true;
 }
 
-// Stage A.  Each lane owns 32 consecutive bit offsets [o0, o0 + 32) and
-// reads their 128 bits once into registers.  The 13-bit header test runs on
-// all 32 offsets from the registers (v_alignbit); the precode sum only on the
-// offsets that pass it (about one in ten); the stored-block LEN/NLEN test on
-// the 4 byte-aligned offsets, the rest of the stored conditions only on a
-// match.  Survivors and hits are appended with one atomic per wave.
-__global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ in, uint64_t n, uint64_t start_bit,
-                                                    uint64_t end_bit, uint64_t *__restrict__ surv, uint32_t surv_cap,
+// One scan range of stage A: candidate offsets in [start_bit, end_bit);
+// chunk0 is the global index of its first 32-offset chunk (ranges are laid
+// end to end in chunk space, so one launch covers every member of a batch).
+struct BfRange {
+    uint64_t start_bit, end_bit, chunk0;
+};
+
+// Stage A.  Each lane owns 32 consecutive bit offsets [o0, o0 + 32) of one
+// range and reads their 128 bits once into registers.  The 13-bit header test
+// runs on all 32 offsets from the registers (v_alignbit); the precode sum only
+// on the offsets that pass it (about one in ten); the stored-block LEN/NLEN
+// test on the 4 byte-aligned offsets, the rest of the stored conditions only
+// on a match.  Survivors and hits are appended with one atomic per wave.
+__global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ in, uint64_t n,
+                                                    const BfRange *__restrict__ ranges, uint32_t nranges,
+                                                    uint64_t nchunk, uint64_t *__restrict__ surv, uint32_t surv_cap,
                                                     uint32_t *surv_count, gzgpu_boundary *__restrict__ hits,
                                                     uint32_t hit_cap, uint32_t *hit_count) {
     const uint64_t total = n * 8;
-    const uint64_t sb = ((start_bit + 3 + 7) / 8) > 1 ? (start_bit + 3 + 7) / 8 : 1;
-    uint64_t until = (end_bit + 7) / 8;
-    if (n < 4) until = 0;
-    else if (until > n - 4) until = n - 4;
-    // dynamic candidates need o + 17 < total
-    const uint64_t dyn_end = total > 17 ? (end_bit < total - 17 ? end_bit : total - 17) : 0;
-    const uint64_t first = start_bit & ~31ull;
-    const uint64_t nchunk = end_bit > first ? (end_bit - first + 31) / 32 : 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     // wave-aligned iterations so every lane reaches the ballots
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nchunk; base += stride) {
         const uint64_t c = base + threadIdx.x;
-        const uint64_t o0 = first + 32 * c;
         const bool live = c < nchunk;
+        // this lane's range: the last one whose chunk0 <= c
+        uint32_t lo_r = 0, hi_r = nranges;
+        while (hi_r - lo_r > 1) {
+            const uint32_t mid = (lo_r + hi_r) >> 1;
+            if (ranges[mid].chunk0 <= c) lo_r = mid;
+            else hi_r = mid;
+        }
+        const uint64_t start_bit = ranges[lo_r].start_bit, end_bit = ranges[lo_r].end_bit;
+        const uint64_t o0 = (start_bit & ~31ull) + 32 * (c - ranges[lo_r].chunk0);
+        const uint64_t sb = ((start_bit + 3 + 7) / 8) > 1 ? (start_bit + 3 + 7) / 8 : 1;
+        uint64_t until = (end_bit + 7) / 8;
+        if (n < 4) until = 0;
+        else if (until > n - 4) until = n - 4;
+        // dynamic candidates need o + 17 < total
+        const uint64_t dyn_end = total > 17 ? (end_bit < total - 17 ? end_bit : total - 17) : 0;
         uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
         if (live) {
             const uint64_t wi = o0 >> 5;
@@ -209,7 +223,7 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
                 hits[slot].bit_offset = earliest;
                 hits[slot].seek_bit = latest;
                 hits[slot].kind = 0;
-                hits[slot].info = succ ? 1u : 0u;
+                hits[slot].info = (succ ? 1u : 0u) | (size & 0xFFFFu) << 16;
             }
         }
     }

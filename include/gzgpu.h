@@ -142,7 +142,8 @@ typedef struct {
     uint64_t bit_offset; /* earliest valid start */
     uint64_t seek_bit;   /* where decoding starts */
     uint32_t kind;       /* 0 stored, 2 dynamic */
-    uint32_t info;       /* dynamic: hlit | hdist << 8 | hclen << 16 */
+    uint32_t info;       /* dynamic: hlit | hdist << 8 | hclen << 16; stored: LEN << 16, bit 0 set
+                            when the bits after the LEN-byte payload parse as a block header */
 } gzgpu_boundary;
 
 /* Every block-finder hit in bit range [start_bit, end_bit) of the device

@@ -235,3 +235,43 @@ def test_autotune_keeps_output_exact(gpu):
             assert np.array_equal(host[out_off[i]: out_off[i] + r.size], r), i
     finally:
         ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
+
+
+def test_batch_block_finder_covers_every_member(gpu):
+    """The batch runs one block-finder pass over all members to be split: its
+    candidate count equals the sum of the counts found member by member, also
+    when a member appears twice at the same input offset."""
+    import torch
+    ctx = gpu.context()
+    old = ctx.tuning()
+    ctx.set_tuning(chunk_bytes=64 << 10, pieces_per_chunk=4, batch_split_bytes=32 << 10)
+    try:
+        raws, ins = [], []
+        for i, k in enumerate(["text", "u32", "text", "runs"]):
+            raw = W.generate(k, (1 << 20) + 777 * i, 900 + i)
+            raws.append(raw)
+            ins.append(W.gzip_compress(raw, (1, 6, 9, 6)[i]))
+        d_in = dev(np.concatenate(ins))
+        in_off = np.cumsum([0] + [x.size for x in ins[:-1]])
+        in_len = np.array([x.size for x in ins])
+        cand_one = []
+        for i in range(len(ins)):
+            d_out = torch.zeros(raws[i].size, dtype=torch.uint8, device="cuda")
+            status, olen, _, st = gpu.decompress_batch(d_in, [in_off[i]], [in_len[i]], d_out, [0], [raws[i].size])
+            assert status[0] == 0
+            cand_one.append(st["candidates"])
+        say("per-member candidates", cand_one)
+        assert sum(c > 0 for c in cand_one) >= 3  # the highly compressible member is not split
+        # all members, member 1 twice (same input range)
+        order = [0, 1, 2, 3, 1]
+        out_cap = np.array([raws[i].size for i in order])
+        out_off = np.cumsum([0] + list(out_cap[:-1]))
+        d_out = torch.zeros(int(out_cap.sum()), dtype=torch.uint8, device="cuda")
+        status, olen, _, st = gpu.decompress_batch(d_in, in_off[order], in_len[order], d_out, out_off, out_cap)
+        assert (status == 0).all(), status
+        assert st["candidates"] == sum(cand_one), (st["candidates"], cand_one)
+        host = d_out.cpu().numpy()
+        for j, i in enumerate(order):
+            assert np.array_equal(host[out_off[j]: out_off[j] + raws[i].size], raws[i]), j
+    finally:
+        ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
