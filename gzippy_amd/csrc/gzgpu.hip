@@ -379,6 +379,14 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
 
 uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
+// inflate wavefronts per CU: u8 ring 16 KB of LDS each, u16 ring 24 KB
+constexpr uint32_t U8_WAVES_PER_CU = 8, U16_WAVES_PER_CU = 6;
+// planner: smallest piece (compressed KiB; the reference's block finder
+// keeps 32 KiB, gzip_block_finder.rs:109 — half that fills the 2048 u8 slots
+// of a 1 GiB text member and measured 12% faster) and the first piece's share
+constexpr uint64_t MIN_SPACING_KIB = 16;
+constexpr double FIRST_PIECE = 0.75;
+
 uint64_t marker_slab_bytes(uint64_t expect_entries) { return round_up(2 * (expect_entries + 64 * KIB) + 512, 256); }
 
 bool debug_enabled() {
@@ -413,8 +421,8 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     c->d_pout.ensure(descs.size() * sizeof(PieceOut));
     c->d_counters.ensure(64);
     HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
-    // LDS bounds residency (160 KiB per CU): u16 ring -> 2 waves/CU, u8 ring -> 4
-    uint32_t per_cu = L.u16 ? 2 : 4;
+    // LDS bounds residency (160 KiB per CU): u16 ring -> 6 waves/CU, u8 ring -> 8
+    uint32_t per_cu = L.u16 ? U16_WAVES_PER_CU : U8_WAVES_PER_CU;
     if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
     L.grid = (uint32_t)std::min<uint64_t>(L.n, (uint64_t)c->num_cus * per_cu);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
@@ -462,13 +470,28 @@ std::vector<PieceOut> finish_inflate(gzgpu_ctx *c, const InflateLaunch &L, const
     return ro;
 }
 
+// The rest of a marker piece that stopped at its flip: u8 entries after its
+// u16 ones in the same slab, the last 32768 u16 entries (all literal) as the
+// window, restarting at the block header (tables) and, inside a block, the
+// saved symbol.
+PieceDesc resume_desc(const PieceDesc &m, const PieceOut &r) {
+    PieceDesc d = m;
+    d.flags = PF_RESUME;
+    d.start_bit = r.in_block ? r.hdr_bit : r.end_bit;
+    d.resume_bit = r.in_block ? r.end_bit : 0;
+    d.seg0 = r.n16;
+    d.window = m.dst + 2 * (r.n16 - WIN);
+    d.dst = m.dst + r.u8_off;
+    d.cap = m.cap - r.u8_off;
+    return d;
+}
+
 // Inflate the pieces in `todo`; results land in R.  Marker pieces run in the
 // u16-ring kernel until their flip; the rest of every flipped piece and all
 // window-known pieces (a member's first piece, re-decodes) then run in one
-// u8-ring launch at twice the occupancy, longest first.  (Running the
-// window-known pieces on a second stream beside the marker launch gains
-// nothing: two u16 waves fill a CU's LDS, and two u8 launches pack worse
-// than one.)
+// u8-ring launch, longest first.  (Running the window-known pieces on a
+// second stream beside the marker launch measured slower: the marker launch
+// leaves little LDS free, and two u8 launches pack worse than one.)
 void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<PieceHost> &P,
                     const std::vector<Member> &mem, std::vector<uint32_t> &todo, std::vector<PieceOut> &R) {
     auto stop_of = [&](const PieceDesc &d) { return d.stop_bit == NO_STOP ? mem[d.member].de : d.stop_bit; };
@@ -478,10 +501,8 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
         std::vector<PieceDesc> d1;
         std::vector<uint64_t> cost;
         for (uint32_t pi : marker_idx) {
-            PieceDesc d = P[pi].d;
-            d.flags |= PF_STOP_AT_FLIP;
-            d1.push_back(d);
-            cost.push_back(stop_of(d) - d.start_bit);
+            d1.push_back(P[pi].d);
+            cost.push_back(stop_of(P[pi].d) - P[pi].d.start_bit);
         }
         float ms = 0;
         InflateLaunch l1 = enqueue_inflate<uint16_t>(c, d_in, in_len, d1, cost);
@@ -502,15 +523,7 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
         const PieceOut &r = R[pi];
         if (r.status != PS_FLIPPED) continue;
         const PieceDesc &m = P[pi].d;
-        PieceDesc d = m;
-        d.flags = PF_RESUME;
-        d.start_bit = r.in_block ? r.hdr_bit : r.end_bit;
-        d.resume_bit = r.in_block ? r.end_bit : 0;
-        d.seg0 = r.n16;
-        d.window = m.dst + 2 * (r.n16 - WIN);  // the last 32768 u16 entries, all literal
-        d.dst = m.dst + r.u8_off;
-        d.cap = m.cap - r.u8_off;
-        d2.push_back(d);
+        d2.push_back(resume_desc(m, r));
         who.push_back(pi);
         cost.push_back(stop_of(m) - r.end_bit);
         from.push_back(r.end_bit);
@@ -576,8 +589,10 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
 
 void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
     const uint64_t chunk_bits = (c->tune.chunk_bytes ? c->tune.chunk_bytes : 512 * KIB) * 8;
-    const uint64_t min_spacing = 8 * 32 * KIB;  // gzip_block_finder.rs:109 (spacing >= 32 KiB)
-    const uint64_t slots = (uint64_t)c->num_cus * 4;  // resident u8-ring inflate wavefronts
+    const uint64_t min_spacing = 8 * MIN_SPACING_KIB * KIB;
+    const uint64_t slots = (uint64_t)c->num_cus *  // resident u8-ring inflate wavefronts
+                           (c->tune.waves_per_cu ? std::min<uint32_t>(c->tune.waves_per_cu, U8_WAVES_PER_CU)
+                                                 : U8_WAVES_PER_CU);
     // Piece grid.  Explicit pieces_per_chunk: chunk_bits / ppc compressed bits.
     // Default: balance decode time.  A wavefront decodes Huffman-coded input
     // at a roughly constant rate of compressed bits, whatever the ratio;
@@ -648,6 +663,14 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         // 0xFF00, NLEN 0x00FF), which both cost a re-decode round when
         // chosen.  A scheduling choice only: any start is verified by the
         // predecessor's end.
+        // A member's first piece runs only in the u8 phase, from its start;
+        // every other piece decodes its part up to the flip in the marker
+        // phase.  The first piece gets FIRST_PIECE of the others' span so
+        // that the u8 phase ends evenly.
+        const uint64_t span = M.de - M.ds;
+        const uint64_t np = std::max<uint64_t>(2, (span + spacing / 2) / spacing);
+        const uint64_t first = (uint64_t)(FIRST_PIECE * (double)span / ((double)(np - 1) + FIRST_PIECE));
+        const uint64_t rest = (span - first) / (np - 1);
         size_t h = 0;
         uint64_t last = M.ds;
         std::vector<size_t> chosen;
@@ -660,8 +683,8 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 if (k->seek_bit >= next) return true;
             return false;
         };
-        for (uint64_t g = M.ds + spacing; g < M.de; g += spacing) {
-            const uint64_t lo = std::max<uint64_t>(last + 1, g - spacing / 2), hi = g + spacing / 2;
+        for (uint64_t g = M.ds + first; g < M.de; g += rest) {
+            const uint64_t lo = std::max<uint64_t>(last + 1, g - rest / 2), hi = g + rest / 2;
             while (h < hits.size() && hits[h].bit_offset < lo) h++;
             size_t best = hits.size();
             uint64_t bestd = ~0ull;
@@ -1503,24 +1526,27 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
             pd.cap = cap;
             pd.flags = PF_MARKER;
         }
-        uint32_t order = 0;
-        upload(c, c->d_pieces, &pd, sizeof pd);
-        upload(c, c->d_order, &order, 4);
-        c->d_pout.ensure(sizeof(PieceOut));
-        c->d_counters.ensure(64);
-        HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
-        if (window)
-            hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(1), dim3(64), 0, c->stream, ai.base,
-                               (uint64_t)in_len + ai.shift, c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u,
-                               c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
-        else
-            hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(1), dim3(64), 0, c->stream, ai.base,
-                               (uint64_t)in_len + ai.shift, c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), 1u,
-                               c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
-        HIPCHK(hipGetLastError());
+        // a marker piece stops at its flip; its rest runs as a RESUME piece
+        std::vector<PieceDesc> d1{pd};
+        std::vector<uint64_t> cost{1};
+        float ms = 0;
         PieceOut po;
-        HIPCHK(hipMemcpyAsync(&po, c->d_pout.p, sizeof po, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        if (window) {
+            po = finish_inflate(c, enqueue_inflate<uint8_t>(c, ai.base, (uint64_t)in_len + ai.shift, d1, cost), d1,
+                                ms)[0];
+        } else {
+            po = finish_inflate(c, enqueue_inflate<uint16_t>(c, ai.base, (uint64_t)in_len + ai.shift, d1, cost), d1,
+                                ms)[0];
+            if (po.status == PS_FLIPPED) {
+                std::vector<PieceDesc> d2{resume_desc(pd, po)};
+                const PieceOut q = finish_inflate(
+                    c, enqueue_inflate<uint8_t>(c, ai.base, (uint64_t)in_len + ai.shift, d2, cost), d2, ms)[0];
+                po.end_bit = q.end_bit;
+                po.n8 = q.n8;
+                po.status = q.status;
+                po.err = q.err;
+            }
+        }
         res->end_bit = po.end_bit >= sb ? po.end_bit - sb : 0;
         res->bfinal_hit = po.status == PS_FINAL;
         res->status = po.status == PS_FINAL || po.status == PS_STOPPED ? GZGPU_OK

@@ -20,7 +20,8 @@
 //            reaches before the piece start copies markers for free.  Output
 //            is u16 entries (< 256 literal, >= 32768 marker) until the last
 //            32768 entries hold no marker (WidthRing::should_flip,
-//            width_ring.rs:310); from then on the output is u8.
+//            width_ring.rs:310): there the piece stops (PS_FLIPPED) and a
+//            RESUME piece carries on with u8 entries.
 //
 //   RESUME : the rest of a MARKER piece after its flip.  The window is the
 //            piece's last 32768 u16 entries (all literal), the ring is
@@ -29,10 +30,9 @@
 //            the saved symbol bit.
 //
 // Two kernels split a marker piece at its flip: the u16-ring kernel
-// (64 KiB ring, 2 wavefronts per CU) decodes until the flip (STOP_AT_FLIP)
-// and the u8-ring kernel (32 KiB ring, 4 wavefronts per CU) decodes the rest
-// together with the DIRECT pieces, so most of the output is produced at the
-// higher occupancy.
+// (MARKER pieces only, 24 KB of LDS, 6 wavefronts per CU) decodes until the
+// flip and the u8-ring kernel (all other pieces, 16 KB, 8 per CU) decodes
+// the rest together with the DIRECT pieces.
 //
 // Slab layout of a MARKER piece (bytes): [u16 entries: 2*n16] (256-aligned)
 // [u8 entries: n8].
@@ -41,7 +41,7 @@
 
 namespace gz {
 
-enum : uint32_t { PF_DIRECT = 1, PF_MARKER = 2, PF_WINDOW = 4, PF_RESUME = 8, PF_STOP_AT_FLIP = 16 };
+enum : uint32_t { PF_DIRECT = 1, PF_MARKER = 2, PF_WINDOW = 4, PF_RESUME = 8 };
 
 struct PieceDesc {
     uint64_t start_bit;      // first block header to decode (RESUME: header of the resumed block)
@@ -72,22 +72,30 @@ struct PieceOut {
     uint32_t pad;
 };
 
-constexpr uint32_t RING = WIN;
-constexpr uint32_t RMASK = RING - 1;
+// Ring of the most recent 8192 entries in LDS; a copy from farther back (up
+// to the 32768 of the window) reads the piece's own output from HBM, or the
+// window before the piece (a u8 or u16 buffer, or, for a marker piece, the
+// marker values themselves).  The small ring lets 6 (u16 entries) or 8 (u8)
+// wavefronts share a CU, two per SIMD, which hides the decode loop's LDS,
+// scalar-load and readlane latencies: per-wave speed drops about 20% while
+// twice the wavefronts run.
+template <typename RT>
+constexpr uint32_t RING = 8192u;
+template <typename RT>
+constexpr uint32_t RMASK = RING<RT> - 1;
 
 template <typename RT>
 struct __align__(16) InflateShared {
     // Two-level Huffman LUTs: a root of LLR / DR bits, then subtables for
     // the codes longer than the root (zlib's inflate_table layout), in the
-    // same array after the root.  The u16-ring kernel runs 2 wavefronts per
-    // CU and has room for 11/10-bit roots; the u8-ring kernel keeps 10/8 so
-    // that four fit in a CU's 160 KiB of LDS (40 KB each).  A subtable that
-    // does not fit its reserve leaves its codes to the canonical slow path.
-    static constexpr int LLR = sizeof(RT) == 2 ? 11 : 10;
-    static constexpr int DR = sizeof(RT) == 2 ? 10 : 8;
-    static constexpr int LL_SUB = sizeof(RT) == 2 ? 512 : 384;
-    static constexpr int D_SUB = sizeof(RT) == 2 ? 128 : 64;
-    RT ring[RING];
+    // same array after the root.  10/8-bit roots keep a wavefront's LDS at
+    // 16 KB (u8 ring) or 24 KB (u16 ring).  A subtable that does not fit its
+    // reserve leaves its codes to the canonical slow path.
+    static constexpr int LLR = 10;
+    static constexpr int DR = 8;
+    static constexpr int LL_SUB = 384;
+    static constexpr int D_SUB = 64;
+    RT ring[RING<RT>];
     uint32_t ll[(1 << LLR) + LL_SUB];
     uint32_t dl[(1 << DR) + D_SUB];
     uint16_t ll_sorted[288];
@@ -100,6 +108,11 @@ struct __align__(16) InflateShared {
     uint8_t pre_lens[32];
     uint32_t qi;
     uint32_t sub_next;
+    // far-copy sources: piece output (RT entries) whose entry 0 is piece
+    // position far_base0, and the 32768 entries before that position
+    // (far_win_kind 0: none, 1: u8 buffer, 2: u16 buffer, 3: markers)
+    uint64_t far_out, far_win, far_base0;
+    uint32_t far_win_kind;
 };
 
 // ---- canonical table construction (whole wave) ----------------------------
@@ -314,21 +327,19 @@ __device__ __attribute__((always_inline)) uint32_t read_dynamic(SH &sh, BitReade
 }
 
 // ---- output segment ----------------------------------------------------
-// Entries go to `p` as u16 (wide: marker mode) or u8.  `seg0` is the piece
-// position of entry 0 of this segment (the ring index of entry i is
-// seg0 + i).  One decode loop serves both widths through a wave-uniform
-// flag, which keeps a single copy of the loop (and its registers) per kernel.
+// Entries go to `p` as RT: u16 in the marker kernel, u8 in the other.
+// `seg0` is the piece position of entry 0 of this segment (the ring index of
+// entry i is seg0 + i).
 struct OutSeg {
     gptr<uint8_t> p;
     uint64_t pos, cap, seg0;
-    uint32_t wide;
 };
 
 // Entries are u8 in the u8-ring kernel at compile time; the u16-ring kernel
 // writes u16 until its flip.
 template <typename RT>
 __device__ __forceinline__ void put(const OutSeg &o, uint64_t i, uint32_t v) {
-    if (sizeof(RT) == 2 && o.wide) ((gptr<uint16_t>)o.p)[i] = (uint16_t)v;
+    if (sizeof(RT) == 2) ((gptr<uint16_t>)o.p)[i] = (uint16_t)v;
     else o.p[i] = (uint8_t)v;
 }
 
@@ -346,20 +357,53 @@ __device__ __forceinline__ void copy_chunk_out(RT *ring, const OutSeg &o, uint32
                                                uint32_t len, uint32_t v, uint64_t &last_marker_end) {
     const bool act = i < len;
     if (act) {
-        ring[(gpos + i) & RMASK] = (RT)v;
+        ring[(gpos + i) & RMASK<RT>] = (RT)v;
         put<RT>(o, pos + i, v);
     }
-    if (sizeof(RT) == 2 && o.wide) {
+    if (sizeof(RT) == 2) {
         const uint64_t m = ballot(act && v >= 256);
         if (m) last_marker_end = o.seg0 + pos + (i - lane_id()) + 64 - (uint64_t)__clzll(m);
     }
 }
 
-template <typename RT>
-__device__ __attribute__((always_inline)) void wave_copy(RT *ring, const OutSeg &o, uint32_t pos, uint32_t g0,
+template <typename RT, typename SH>
+__device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o, uint32_t pos, uint32_t g0,
                                                          uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
+    RT *ring = sh.ring;
     const uint32_t lane = lane_id();
     const uint32_t gpos = g0 + pos;
+    if (dist > RING<RT>) {
+        // Source more than a ring back (never overlapping the copy: dist >
+        // 8192 > len): the piece's output in HBM, written by this wavefront
+        // at least a ring earlier, or the window before the piece.  A
+        // wavefront's vector memory operations reach the CU's L1 in order
+        // and the loads' wait covers its earlier stores.  Entries more than
+        // 32768 before the piece do not exist (a distance the window cannot
+        // serve): they read as 0 without a memory access.
+        const uint64_t out = uni64(sh.far_out), win = uni64(sh.far_win);
+        const uint32_t wk = uni(sh.far_win_kind);
+        const int64_t rel = (int64_t)(o.seg0 - uni64(sh.far_base0)) + (int64_t)pos - (int64_t)dist;
+        uint32_t v[5];
+#pragma unroll
+        for (uint32_t k = 0; k < 5; k++) {
+            const uint32_t i = lane + WAVE * k;
+            const int64_t r = rel + (int64_t)i;
+            uint32_t x = 0;
+            if (i < len) {
+                if (r >= 0) x = to_global((const RT *)out)[r];
+                else if (r >= -(int64_t)WIN && wk == 1) x = to_global((const uint8_t *)win)[WIN + r];
+                else if (r >= -(int64_t)WIN && wk == 2) x = to_global((const uint16_t *)win)[WIN + r];
+                else if (r >= -(int64_t)WIN && wk == 3) x = MARKER_BASE + (uint32_t)(WIN + r);
+            }
+            v[k] = x;
+        }
+        compiler_fence();
+#pragma unroll
+        for (uint32_t k = 0; k < 5; k++)
+            if (len > WAVE * k) copy_chunk_out<RT>(ring, o, gpos, pos, lane + WAVE * k, len, v[k], last_marker_end);
+        compiler_fence();
+        return;
+    }
     const uint32_t src0 = gpos - dist;  // ring position of the source's first entry
     // source offsets of the (up to) five chunks: i, or i mod dist when the
     // copy overlaps itself (quotient from the hardware reciprocal, corrected
@@ -383,14 +427,14 @@ __device__ __attribute__((always_inline)) void wave_copy(RT *ring, const OutSeg 
     // every read before any write (ring indices are masked, so lanes past
     // the copy's end read harmlessly)
     uint32_t v0, v1, v2, v3, v4;
-    v0 = ring[(src0 + s0) & RMASK];
+    v0 = ring[(src0 + s0) & RMASK<RT>];
     if (len > 64) {
-        v1 = ring[(src0 + s1) & RMASK];
+        v1 = ring[(src0 + s1) & RMASK<RT>];
         if (len > 128) {
-            v2 = ring[(src0 + s2) & RMASK];
+            v2 = ring[(src0 + s2) & RMASK<RT>];
             if (len > 192) {
-                v3 = ring[(src0 + s3) & RMASK];
-                if (len > 256) v4 = ring[(src0 + s4) & RMASK];
+                v3 = ring[(src0 + s3) & RMASK<RT>];
+                if (len > 256) v4 = ring[(src0 + s4) & RMASK<RT>];
             }
         }
     }
@@ -480,7 +524,7 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
     if (kind == K_LIT) {
         if (pos >= cap) return R_OVERFLOW;
         if (lane == 0) {
-            sh.ring[(g0 + pos) & RMASK] = (RT)(e >> 16);
+            sh.ring[(g0 + pos) & RMASK<RT>] = (RT)(e >> 16);
             put<RT>(o, pos, e >> 16);
         }
         compiler_fence();
@@ -515,7 +559,7 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
         err = E_DIST_FAR;
         return R_ERROR;
     }
-    wave_copy<RT>(sh.ring, o, pos, g0, len, dist, last_marker_end);
+    wave_copy<RT>(sh, o, pos, g0, len, dist, last_marker_end);
     pos += len;
     P += off + DL + DE;
     return R_CONT;
@@ -599,7 +643,6 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
     o.pos = uni64(o.pos);
     o.cap = uni64(o.cap);
     o.seg0 = uni64(o.seg0);
-    o.wide = uni(o.wide);
     limit_bit = uni64(limit_bit);
     // 32-bit segment positions: the segment is rebased before 2^30 entries
     uint32_t pos = (uint32_t)o.pos;
@@ -654,14 +697,28 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             P = uni64(P);
             lookahead(sh, in, n, P, la);
         } else {
-            // a copy whose source slot a later entry of this round would
-            // overwrite first (distance near 32768): end the round after it
-            const uint64_t BAD = ballot(((M & ~LITM & ~EOBM) >> lane) & 1 && la.dist + run - opos > RING);
-            if (BAD) {
-                const uint32_t fb = uni((uint32_t)__builtin_ctzll(BAD));
-                M &= fb == 63 ? ~0ull : (2ull << fb) - 1;
-                run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, (int)fb));
-                s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)fb));
+            // a ring copy whose source slot a later entry of this round would
+            // overwrite first (distance near the ring size): end the round
+            // after it.  A copy from beyond the ring whose source
+            // reaches into this round's own output, not yet in HBM: end the
+            // round before it (never the round's first symbol: distance >
+            // ring > length).
+            const bool is_copy = ((M & ~LITM & ~EOBM) >> lane) & 1;
+            const bool far = la.dist > RING<RT>;
+            const uint64_t BAD = ballot(is_copy && !far && la.dist + run - opos > RING<RT>);
+            const uint64_t BADF = ballot(lane != 0 && is_copy && far && la.dist < opos + la.olen);
+            if (BAD | BADF) {
+                const uint32_t fa = BAD ? uni((uint32_t)__builtin_ctzll(BAD)) : 64u;
+                const uint32_t ff = BADF ? uni((uint32_t)__builtin_ctzll(BADF)) : 64u;
+                if (fa < ff) {
+                    M &= fa == 63 ? ~0ull : (2ull << fa) - 1;
+                    run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, (int)fa));
+                    s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)fa));
+                } else {
+                    M &= (1ull << ff) - 1;
+                    run = uni((uint32_t)__builtin_amdgcn_readlane((int)opos, (int)ff));
+                    s = ff;
+                }
                 eob = 0;
             }
             eob = uni(eob);
@@ -673,7 +730,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             // ---- literals of the round: one LDS write and one HBM store
             if ((lits >> lane) & 1) {
                 const uint32_t q = pos + opos;
-                sh.ring[(g0 + q) & RMASK] = (RT)(c_e >> 16);
+                sh.ring[(g0 + q) & RMASK<RT>] = (RT)(c_e >> 16);
                 put<RT>(o, q, c_e >> 16);
             }
             compiler_fence();
@@ -683,7 +740,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
                 const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)opos, (int)mi);
                 const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)c_dist, (int)mi);
                 const uint32_t ln = (uint32_t)__builtin_amdgcn_readlane((int)c_olen, (int)mi);
-                wave_copy<RT>(sh.ring, o, pos + m, g0, ln, d, last_marker_end);
+                wave_copy<RT>(sh, o, pos + m, g0, ln, d, last_marker_end);
             }
             pos += run;
             if (eob) {
@@ -691,13 +748,13 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
                 break;
             }
         }
-        if (o.wide && o.seg0 + pos >= WIN && o.seg0 + pos - last_marker_end >= WIN) {
+        if (sizeof(RT) == 2 && o.seg0 + pos >= WIN && o.seg0 + pos - last_marker_end >= WIN) {
             r = R_FLIP;
             break;
         }
         if (pos >= 0x40000000u) {
             // rebase the segment so positions stay 32-bit
-            o.p = o.p + (o.wide ? 2ull : 1ull) * pos;
+            o.p = o.p + sizeof(RT) * pos;
             o.seg0 += pos;
             o.cap -= pos;
             g0 += pos;
@@ -725,8 +782,8 @@ __device__ __attribute__((always_inline)) void stored_copy(const uint8_t *in_, u
     const uint32_t sh = (uint32_t)(src_byte & 3) * 8;
     const uint64_t w0 = src_byte >> 2;
     const uint64_t nwords = n >> 2;  // whole words inside the buffer
-    const uint32_t ring_from = len > RING ? len - RING : 0;
-    const bool dst_aligned = sizeof(RT) == 1 || !o.wide ? ((((uint64_t)o.p + o.pos) & 3) == 0) : false;
+    const uint32_t ring_from = len > RING<RT> ? len - RING<RT> : 0;
+    const bool dst_aligned = sizeof(RT) == 1 && (((uint64_t)o.p + o.pos) & 3) == 0;
     for (uint32_t k = 0; k < len; k += G * WAVE * 4) {
         uint32_t lo[G], hi[G];
         const uint64_t qk = (uint64_t)(k >> 2);
@@ -751,14 +808,14 @@ __device__ __attribute__((always_inline)) void stored_copy(const uint8_t *in_, u
             if (t < len) {
                 const uint32_t v = __builtin_amdgcn_alignbit(hi[g], lo[g], sh);
                 const uint32_t nb = len - t < 4 ? len - t : 4;
-                if (dst_aligned && nb == 4 && (sizeof(RT) == 1 || !o.wide)) {
+                if (dst_aligned && nb == 4) {
                     *(gptr<uint32_t>)(o.p + o.pos + t) = v;
                 } else {
                     for (uint32_t j = 0; j < nb; j++) put<RT>(o, o.pos + t + j, (v >> (8 * j)) & 0xFF);
                 }
                 if (t + 4 > ring_from) {
                     for (uint32_t j = 0; j < nb; j++)
-                        if (t + j >= ring_from) ring[(uint32_t)(gpos + t + j) & RMASK] = (RT)((v >> (8 * j)) & 0xFF);
+                        if (t + j >= ring_from) ring[(uint32_t)(gpos + t + j) & RMASK<RT>] = (RT)((v >> (8 * j)) & 0xFF);
                 }
             }
         }
@@ -771,7 +828,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
                                                              uint64_t in_len, const PieceDesc &pd, PieceOut *po) {
     const uint32_t lane = lane_id();
     const bool marker = (pd.flags & PF_MARKER) != 0;
-    if (marker && sizeof(RT) == 1) {  // host error: marker pieces need the u16 ring
+    if (marker != (sizeof(RT) == 2)) {  // host error: marker pieces and only they take the u16 kernel
         if (lane == 0) {
             po->status = PS_ERROR;
             po->err = E_NONE;
@@ -783,13 +840,11 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
     const bool resume = (pd.flags & PF_RESUME) != 0;
     const uint64_t base0 = resume ? pd.seg0 : 0;  // entries produced before dst
     uint64_t resume_bit = resume ? pd.resume_bit : 0;
-    // marker pieces start with u16 entries; everything else writes u8
     OutSeg o;
     o.p = dst;
     o.pos = 0;
     o.seg0 = base0;
-    o.wide = marker ? 1u : 0u;
-    o.cap = marker ? pd.cap / 2 : pd.cap;
+    o.cap = pd.cap / sizeof(RT);
     uint64_t n16 = 0, u8_off = 0;
     uint64_t last_marker_end = 0;
     uint32_t status = PS_NONE, err = E_NONE, nblocks = 0, in_block = 0;
@@ -797,17 +852,25 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
     uint64_t t_begin = wall_clock64();
     uint32_t fixed_built = 0;  // the fixed tables are in the LUTs
 
-    // ring = the window before the piece (position base0 - 32768 + i at
-    // slot (base0 + i) mod 32768)
+    // ring = the last RING entries of the window before the piece (piece
+    // position a at slot a mod RING)
     if (marker) {
-        for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[q] = (RT)(MARKER_BASE + q);
+        for (uint32_t q = lane; q < RING<RT>; q += WAVE) sh.ring[q] = (RT)(MARKER_BASE + WIN - RING<RT> + q);
     } else if (resume) {
+        // the window's last RING entries (piece positions base0 - RING ..)
         gptr<const uint16_t> w16 = (gptr<const uint16_t>)to_global(pd.window);
-        for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[(uint32_t)(base0 + q) & RMASK] = (RT)w16[q];
+        for (uint32_t q = WIN - RING<RT> + lane; q < WIN; q += WAVE)
+            sh.ring[(uint32_t)(base0 + q) & RMASK<RT>] = (RT)w16[q];
     } else if (pd.flags & PF_WINDOW) {
-        for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[q] = to_global(pd.window)[q];
+        for (uint32_t q = WIN - RING<RT> + lane; q < WIN; q += WAVE) sh.ring[q & RMASK<RT>] = to_global(pd.window)[q];
     } else {
-        for (uint32_t q = lane; q < RING; q += WAVE) sh.ring[q] = 0;
+        for (uint32_t q = lane; q < RING<RT>; q += WAVE) sh.ring[q] = 0;
+    }
+    if (lane == 0) {
+        sh.far_out = (uint64_t)pd.dst;
+        sh.far_win = (uint64_t)pd.window;
+        sh.far_base0 = base0;
+        sh.far_win_kind = marker ? 3u : resume ? 2u : (pd.flags & PF_WINDOW) ? 1u : 0u;
     }
     __syncthreads();
 
@@ -861,9 +924,9 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
             } else {
                 stored_copy<RT>(in, in_len, src_byte, sh.ring, o, len);
                 o.pos += len;
-                if (o.wide && o.seg0 + o.pos >= WIN && o.seg0 + o.pos - last_marker_end >= WIN) r = R_FLIP;
+                if (sizeof(RT) == 2 && o.seg0 + o.pos >= WIN && o.seg0 + o.pos - last_marker_end >= WIN) r = R_FLIP;
                 if (o.pos >= 0x40000000u) {  // keep segment positions 32-bit
-                    o.p = o.p + (o.wide ? 2ull : 1ull) * o.pos;
+                    o.p = o.p + sizeof(RT) * o.pos;
                     o.seg0 += o.pos;
                     o.cap -= o.pos;
                     o.pos = 0;
@@ -891,36 +954,23 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
             br.init(in, in_len, P);
         }
         if (r == R_FLIP) {
-            // the ring already holds the last 32768 entries, all literal:
-            // continue with u8 entries after the u16 ones
-            n16 = o.seg0 + o.pos;
-            u8_off = (2 * n16 + 255) & ~255ull;
-            if (u8_off >= pd.cap) {
-                r = R_OVERFLOW;
-            } else if ((pd.flags & PF_STOP_AT_FLIP) && !(btype == 0 && bfinal)) {
-                // hand the rest to the u8-ring kernel: resume at the next
-                // header (after a stored block) or inside this block
-                o.seg0 = n16;
-                o.pos = 0;
-                o.wide = 0;
-                status = PS_FLIPPED;
-                in_block = btype != 0;
-                hdr_bit = hbit;
-                end_bit = br.pos();
-                break;
+            // the last 32768 entries are all literal: the u8 kernel takes the
+            // rest (u8 entries from slab byte u8_off on), resuming at the next
+            // header (after a stored block) or inside this block.  After a
+            // final stored block there is no rest: the piece just ends.
+            if (btype == 0 && bfinal) {
+                r = R_EOB;
             } else {
-                o.p = dst + u8_off;
-                o.seg0 = n16;
-                o.cap = pd.cap - u8_off;
-                o.pos = 0;
-                o.wide = 0;
-                // a flip inside a block resumes that block's symbols
-                if (btype != 0) {
-                    uint64_t P = br.pos();
-                    r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err);
-                    br.init(in, in_len, P);
+                n16 = o.seg0 + o.pos;
+                u8_off = (2 * n16 + 255) & ~255ull;
+                if (u8_off >= pd.cap) {
+                    r = R_OVERFLOW;
                 } else {
-                    r = R_EOB;
+                    status = PS_FLIPPED;
+                    in_block = btype != 0;
+                    hdr_bit = hbit;
+                    end_bit = br.pos();
+                    break;
                 }
             }
         }
@@ -948,11 +998,11 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
     }
     __syncthreads();
     if (lane == 0) {
-        const bool flipped = marker && o.wide == 0;
+        const bool flipped = status == PS_FLIPPED;
         po->end_bit = end_bit;
         const uint64_t total = o.seg0 + o.pos;
         po->n16 = marker ? (flipped ? n16 : total) : 0;
-        po->n8 = marker ? (flipped ? total - n16 : 0) : total - base0;
+        po->n8 = marker ? 0 : total - base0;
         po->u8_off = flipped ? u8_off : 0;
         po->hdr_bit = hdr_bit;
         po->in_block = in_block;
@@ -968,9 +1018,8 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
 
 // Persistent inflate kernel: every workgroup (one wavefront) pulls piece
 // indices from a device queue until it is drained; `order` lists the pieces
-// longest-first.  RT = ring entry type: uint16_t when any piece decodes with
-// markers (64 KiB ring, 2 waves per CU), uint8_t for window-known pieces only
-// (32 KiB ring, 4 waves per CU).
+// longest-first.  RT = ring entry type: uint16_t for MARKER pieces, uint8_t
+// for all others.
 template <typename RT>
 __global__ void __launch_bounds__(64) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
                                                 const PieceDesc *__restrict__ pieces,
