@@ -4,13 +4,14 @@ Workload (default): 1 GiB from the text generator (order-1 Markov chain over
 96 printable bytes, Zipf(1.2) rows, 35 % copies of U[3,258] bytes from
 log-uniform distances <= 32 KiB), compressed at level 6 into one RFC 1952
 member (libdeflate stand-in encoder), decoded by speculative parallel decode
-on a 512 KiB compressed-chunk grid.  Input and output stay resident in HBM;
-a step is one full decode (find, inflate, resolve, CRC32 + ISIZE verify).
+(pieces planned by decode cost, tuning picked by the autotuner).  Input and
+output stay resident in HBM; a step is one full decode (find, inflate,
+resolve, CRC32 + ISIZE verify).
 
 Prints ONE JSON line (see the keys below).  `roofline` uses the dominant
-kernel (k_inflate): algorithmic bytes C + D per step over its summed
-HIP-event duration per step.  `cpu_baseline` is the oracle's serial port (one
-core) on the same member.
+kernel (the step's one k_inflate launch): the compressed bytes it consumes
+plus the bytes it writes, over its HIP-event duration.  `cpu_baseline` is the
+oracle's serial port (one core) on the same member.
 """
 import argparse
 import json
@@ -54,21 +55,22 @@ def cpu_baseline(gz, raw_size, budget_s=12.0, max_runs=3):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_inflate_hbm_traffic_1gib_text_l6.md")
 
 
-def measured_traffic(args):
-    """HBM bytes per k_inflate<unsigned char> launch, from the committed
-    FETCH_SIZE / WRITE_SIZE summary of this exact workload (default size,
-    level and chunk grid)."""
-    if (args.size_mib, args.level, args.chunk_kib, args.pieces_per_chunk) != (1024, 6, 512, 0):
+def measured_traffic(args, kernel):
+    """HBM bytes per launch of `kernel`, from the committed FETCH_SIZE /
+    WRITE_SIZE summary of this exact workload (default size and level)."""
+    if (args.size_mib, args.level) != (1024, 6):
         return None
+    got = {}
     try:
         with open(TRAFFIC_PROFILE) as f:
             for ln in f:
-                if ln.startswith("traffic_bytes_per_launch:"):
-                    return {"bytes_per_launch": int(ln.split(":", 1)[1].strip()),
-                            "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
+                if ":" in ln and ln.split(":", 1)[0] in ("kernel", "traffic_bytes_per_launch"):
+                    got[ln.split(":", 1)[0]] = ln.split(":", 1)[1].strip()
     except OSError:
-        pass
-    return None
+        return None
+    if got.get("kernel") != kernel or "traffic_bytes_per_launch" not in got:
+        return None
+    return {"bytes_per_launch": int(got["traffic_bytes_per_launch"]), "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
 
 
 def batch_leg(args, dev, ctx):
@@ -291,7 +293,7 @@ def main():
     del d_raw
 
     ms_infl, launches, ms_dev = 0.0, 0, 0.0
-    ms_clean, ms_marker, clean_bytes = 0.0, 0.0, 0
+    ms_clean, ms_marker, clean_bytes, marker_bytes = 0.0, 0.0, 0, 0
     per_step = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -303,6 +305,7 @@ def main():
         ms_clean += st["ms_inflate_clean"]
         ms_marker += st["ms_inflate_marker"]
         clean_bytes += st["clean_in_bytes"] + st["clean_out_bytes"]
+        marker_bytes += st["marker_in_bytes"] + st["marker_out_bytes"]
         launches += st["inflate_launches"]
         ms_dev += st["ms_total"]
         stats = st
@@ -312,12 +315,16 @@ def main():
     gbs = n * args.steps / t / 1e9
     C, D = gz.size, n
     infl_ms_step = ms_infl / args.steps
-    # dominant kernel: the u8-ring inflate (window-known pieces and the rest
-    # of every flipped piece), one launch per step; its algorithmic bytes are
-    # the compressed bytes it consumes plus the bytes it writes
-    clean_ms_launch = ms_clean / args.steps
-    clean_alg = clean_bytes // args.steps
-    achieved = clean_alg / (clean_ms_launch / 1e3) / 1e9
+    # dominant kernel: the step's inflate launch (the u16-ring kernel when the
+    # member is split into speculative pieces); its algorithmic bytes are the
+    # compressed bytes it consumes plus the bytes it writes (u16 entries
+    # count 2)
+    u16 = ms_marker >= ms_clean
+    kname = "k_inflate<unsigned short>" if u16 else "k_inflate<unsigned char>"
+    k_launches = launches / args.steps
+    k_ms_launch = (ms_marker if u16 else ms_clean) / args.steps / max(1.0, k_launches)
+    k_alg = (marker_bytes if u16 else clean_bytes) // args.steps / max(1.0, k_launches)
+    achieved = k_alg / (k_ms_launch / 1e3) / 1e9
     stage_gbs = (C + D) / ((ms_clean + ms_marker) / args.steps / 1e3) / 1e9
     lat = sorted(per_step)
     p50 = lat[len(lat) // 2] * 1e3
@@ -329,8 +336,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {
             "workload": f"single gzip member: {args.size_mib} MiB text generator (order-1 Markov, Zipf 1.2, "
-                        f"35% copies), {W.encoder_name()} level {args.level}, speculative parallel decode "
-                        f"on {args.chunk_kib} KiB compressed chunks",
+                        f"35% copies), {W.encoder_name()} level {args.level}, speculative parallel decode",
             "compressed_bytes": int(C), "decompressed_bytes": int(D), "seed": args.seed,
             "pieces": stats["pieces"], "redecoded": stats["redecoded"],
             "inflate_launches_per_step": launches / args.steps,
@@ -338,12 +344,11 @@ def main():
             "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3)},
             "tuning": tuning,
         },
-        "roofline": {"bound": "hbm", "kernel": "k_inflate<unsigned char>", "achieved": round(achieved, 2),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "algorithmic_bytes_per_launch": int(clean_alg), "kernel_ms_per_launch": round(clean_ms_launch, 3),
-                     "inflate_stage": {"kernels": ["k_inflate<unsigned short>", "k_inflate<unsigned char>"],
-                                       "ms_per_step": round(infl_ms_step, 3),
-                                       "marker_ms": round(ms_marker / args.steps, 3),
+                     "algorithmic_bytes_per_launch": int(k_alg), "kernel_ms_per_launch": round(k_ms_launch, 3),
+                     "launches_per_step": k_launches,
+                     "inflate_stage": {"ms_per_step": round(infl_ms_step, 3),
                                        "achieved_c_plus_d_gbs": round(stage_gbs, 2)}},
     }
     if args.batch_members:
@@ -353,7 +358,7 @@ def main():
     if args.serve_seconds > 0:
         torch.cuda.empty_cache()
         line["serving"] = serve_leg(args, (200, 1000, 5000))
-    traffic = measured_traffic(args)
+    traffic = measured_traffic(args, kname)
     if traffic is not None:
         line["roofline"]["traffic"] = traffic["bytes_per_launch"]
         line["roofline"]["traffic_source"] = traffic["source"]

@@ -27,7 +27,6 @@ enum : uint32_t {
     PS_FINAL = 2,     // decoded the final (BFINAL) block
     PS_OVERFLOW = 3,  // slab / output capacity exhausted
     PS_ERROR = 4,     // invalid stream from this start
-    PS_FLIPPED = 5,   // marker piece handed over to the u8 kernel at its flip
 };
 // error sub-codes (PieceOut::err)
 enum : uint32_t {

@@ -379,8 +379,8 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
 
 uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-// inflate wavefronts per CU: u8 ring 16 KB of LDS each, u16 ring 24 KB
-constexpr uint32_t U8_WAVES_PER_CU = 8, U16_WAVES_PER_CU = 6;
+// inflate wavefronts per CU (16 KB of LDS each, two per SIMD)
+constexpr uint32_t U8_WAVES_PER_CU = 8, U16_WAVES_PER_CU = 8;
 // planner: smallest piece (compressed KiB; the reference's block finder
 // keeps 32 KiB, gzip_block_finder.rs:109 — half that fills the 2048 u8 slots
 // of a 1 GiB text member and measured 12% faster) and the first piece's share
@@ -421,7 +421,7 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     c->d_pout.ensure(descs.size() * sizeof(PieceOut));
     c->d_counters.ensure(64);
     HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
-    // LDS bounds residency (160 KiB per CU): u16 ring -> 6 waves/CU, u8 ring -> 8
+    // LDS bounds residency (160 KiB per CU): 16 KB per wavefront
     uint32_t per_cu = L.u16 ? U16_WAVES_PER_CU : U8_WAVES_PER_CU;
     if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
     L.grid = (uint32_t)std::min<uint64_t>(L.n, (uint64_t)c->num_cus * per_cu);
@@ -451,7 +451,7 @@ std::vector<PieceOut> finish_inflate(gzgpu_ctx *c, const InflateLaunch &L, const
             t1 = std::max(t1, r.t_end);
             bytes += r.n16 + r.n8;
             n16 += r.n16;
-            flipped += r.status == PS_FLIPPED;
+            flipped += r.mode == 2;
         }
         size_t slow = 0;
         for (size_t i = 1; i < ro.size(); i++)
@@ -470,87 +470,39 @@ std::vector<PieceOut> finish_inflate(gzgpu_ctx *c, const InflateLaunch &L, const
     return ro;
 }
 
-// The rest of a marker piece that stopped at its flip: u8 entries after its
-// u16 ones in the same slab, the last 32768 u16 entries (all literal) as the
-// window, restarting at the block header (tables) and, inside a block, the
-// saved symbol.
-PieceDesc resume_desc(const PieceDesc &m, const PieceOut &r) {
-    PieceDesc d = m;
-    d.flags = PF_RESUME;
-    d.start_bit = r.in_block ? r.hdr_bit : r.end_bit;
-    d.resume_bit = r.in_block ? r.end_bit : 0;
-    d.seg0 = r.n16;
-    d.window = m.dst + 2 * (r.n16 - WIN);
-    d.dst = m.dst + r.u8_off;
-    d.cap = m.cap - r.u8_off;
-    return d;
-}
-
-// Inflate the pieces in `todo`; results land in R.  Marker pieces run in the
-// u16-ring kernel until their flip; the rest of every flipped piece and all
-// window-known pieces (a member's first piece, re-decodes) then run in one
-// u8-ring launch, longest first.  (Running the window-known pieces on a
-// second stream beside the marker launch measured slower: the marker launch
-// leaves little LDS free, and two u8 launches pack worse than one.)
+// Inflate the pieces in `todo` in one persistent launch; results land in R.
+// A call with marker pieces runs all its pieces in the u16-ring kernel (a
+// marker piece flips to u8 output in place), one without in the u8-ring
+// kernel.  Longest first.
 void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<PieceHost> &P,
                     const std::vector<Member> &mem, std::vector<uint32_t> &todo, std::vector<PieceOut> &R) {
     auto stop_of = [&](const PieceDesc &d) { return d.stop_bit == NO_STOP ? mem[d.member].de : d.stop_bit; };
-    std::vector<uint32_t> marker_idx, clean_idx;
-    for (uint32_t pi : todo) ((P[pi].d.flags & PF_MARKER) ? marker_idx : clean_idx).push_back(pi);
-    if (!marker_idx.empty()) {
-        std::vector<PieceDesc> d1;
-        std::vector<uint64_t> cost;
-        for (uint32_t pi : marker_idx) {
-            d1.push_back(P[pi].d);
-            cost.push_back(stop_of(P[pi].d) - P[pi].d.start_bit);
-        }
-        float ms = 0;
-        InflateLaunch l1 = enqueue_inflate<uint16_t>(c, d_in, in_len, d1, cost);
-        std::vector<PieceOut> r1 = finish_inflate(c, l1, d1, ms);
-        c->st.ms_inflate_marker += ms;
-        for (size_t i = 0; i < marker_idx.size(); i++) R[marker_idx[i]] = r1[i];
-    }
-    std::vector<PieceDesc> d2;
-    std::vector<uint32_t> who;
-    std::vector<uint64_t> cost, from;
-    for (uint32_t pi : clean_idx) {
-        d2.push_back(P[pi].d);
-        who.push_back(pi);
+    if (todo.empty()) return;
+    std::vector<PieceDesc> d;
+    std::vector<uint64_t> cost;
+    bool any_marker = false;
+    for (uint32_t pi : todo) {
+        d.push_back(P[pi].d);
         cost.push_back(stop_of(P[pi].d) - P[pi].d.start_bit);
-        from.push_back(P[pi].d.start_bit);
+        any_marker |= (P[pi].d.flags & PF_MARKER) != 0;
     }
-    for (uint32_t pi : marker_idx) {
-        const PieceOut &r = R[pi];
-        if (r.status != PS_FLIPPED) continue;
-        const PieceDesc &m = P[pi].d;
-        d2.push_back(resume_desc(m, r));
-        who.push_back(pi);
-        cost.push_back(stop_of(m) - r.end_bit);
-        from.push_back(r.end_bit);
-    }
-    if (d2.empty()) return;
     float ms = 0;
-    InflateLaunch l2 = enqueue_inflate<uint8_t>(c, d_in, in_len, d2, cost);
-    std::vector<PieceOut> r2 = finish_inflate(c, l2, d2, ms);
-    c->st.ms_inflate_clean += ms;
-    for (size_t i = 0; i < d2.size(); i++) {
-        const uint32_t pi = who[i];
-        const PieceOut &q = r2[i];
-        c->st.clean_in_bytes += (q.end_bit > from[i] ? q.end_bit - from[i] : 0) / 8;
-        c->st.clean_out_bytes += q.n8;
-        if (!(P[pi].d.flags & PF_MARKER)) {
-            R[pi] = q;
-            continue;
-        }
-        PieceOut &r = R[pi];
-        r.end_bit = q.end_bit;
-        r.n8 = q.n8;
-        r.bits_done = q.bits_done + (d2[i].start_bit - P[pi].d.start_bit);
-        r.t_end = q.t_end;
-        r.status = q.status;
-        r.err = q.err;
-        r.nblocks += q.nblocks;
-        r.mode = 2;
+    std::vector<PieceOut> r = any_marker ? finish_inflate(c, enqueue_inflate<uint16_t>(c, d_in, in_len, d, cost), d, ms)
+                                         : finish_inflate(c, enqueue_inflate<uint8_t>(c, d_in, in_len, d, cost), d, ms);
+    uint64_t in_bytes = 0, out_bytes = 0;
+    for (size_t i = 0; i < todo.size(); i++) {
+        R[todo[i]] = r[i];
+        in_bytes += (r[i].end_bit > d[i].start_bit ? r[i].end_bit - d[i].start_bit : 0) / 8;
+        out_bytes += 2 * r[i].n16 + r[i].n8;
+    }
+    if (any_marker) {
+        c->st.ms_inflate_marker += ms;
+        c->st.marker_in_bytes += in_bytes;
+        c->st.marker_out_bytes += out_bytes;
+    } else {
+        c->st.ms_inflate_clean += ms;
+        c->st.clean_in_bytes += in_bytes;
+        c->st.clean_out_bytes += out_bytes;
     }
 }
 
@@ -663,10 +615,10 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         // 0xFF00, NLEN 0x00FF), which both cost a re-decode round when
         // chosen.  A scheduling choice only: any start is verified by the
         // predecessor's end.
-        // A member's first piece runs only in the u8 phase, from its start;
-        // every other piece decodes its part up to the flip in the marker
-        // phase.  The first piece gets FIRST_PIECE of the others' span so
-        // that the u8 phase ends evenly.
+        // A member's first piece knows its window and writes its final
+        // bytes from the start; the others write u16 entries until their
+        // flip and are resolved afterwards.  The first piece gets
+        // FIRST_PIECE of the others' span.
         const uint64_t span = M.de - M.ds;
         const uint64_t np = std::max<uint64_t>(2, (span + spacing / 2) / spacing);
         const uint64_t first = (uint64_t)(FIRST_PIECE * (double)span / ((double)(np - 1) + FIRST_PIECE));
@@ -1526,27 +1478,12 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
             pd.cap = cap;
             pd.flags = PF_MARKER;
         }
-        // a marker piece stops at its flip; its rest runs as a RESUME piece
         std::vector<PieceDesc> d1{pd};
         std::vector<uint64_t> cost{1};
         float ms = 0;
-        PieceOut po;
-        if (window) {
-            po = finish_inflate(c, enqueue_inflate<uint8_t>(c, ai.base, (uint64_t)in_len + ai.shift, d1, cost), d1,
-                                ms)[0];
-        } else {
-            po = finish_inflate(c, enqueue_inflate<uint16_t>(c, ai.base, (uint64_t)in_len + ai.shift, d1, cost), d1,
-                                ms)[0];
-            if (po.status == PS_FLIPPED) {
-                std::vector<PieceDesc> d2{resume_desc(pd, po)};
-                const PieceOut q = finish_inflate(
-                    c, enqueue_inflate<uint8_t>(c, ai.base, (uint64_t)in_len + ai.shift, d2, cost), d2, ms)[0];
-                po.end_bit = q.end_bit;
-                po.n8 = q.n8;
-                po.status = q.status;
-                po.err = q.err;
-            }
-        }
+        const uint64_t n_in = (uint64_t)in_len + ai.shift;
+        const PieceOut po = window ? finish_inflate(c, enqueue_inflate<uint8_t>(c, ai.base, n_in, d1, cost), d1, ms)[0]
+                                   : finish_inflate(c, enqueue_inflate<uint16_t>(c, ai.base, n_in, d1, cost), d1, ms)[0];
         res->end_bit = po.end_bit >= sb ? po.end_bit - sb : 0;
         res->bfinal_hit = po.status == PS_FINAL;
         res->status = po.status == PS_FINAL || po.status == PS_STOPPED ? GZGPU_OK

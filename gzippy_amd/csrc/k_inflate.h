@@ -5,9 +5,11 @@
 // the first non-final block header at or after its stop bit, or at the end of
 // the final block (stop rule of chunk_decode.rs:2029-2040).
 //
-// The 32 KiB sliding window lives in LDS as a ring of 32768 entries, so a
-// back-reference never reads HBM: copies read the ring and write both the
-// ring and the piece's output (store-only HBM traffic).
+// The most recent entries of the sliding window live in LDS as a ring (4096
+// u16 or 8192 u8 entries); copies read the ring and write both the ring and
+// the piece's output.  A copy from farther back (up to the 32768 of the
+// window) reads the piece's own output from HBM, or the window before the
+// piece.
 //
 // Window / output modes
 //   DIRECT : the window is all zeros (stream start, WindowMap{0: zeros},
@@ -20,19 +22,12 @@
 //            reaches before the piece start copies markers for free.  Output
 //            is u16 entries (< 256 literal, >= 32768 marker) until the last
 //            32768 entries hold no marker (WidthRing::should_flip,
-//            width_ring.rs:310): there the piece stops (PS_FLIPPED) and a
-//            RESUME piece carries on with u8 entries.
+//            width_ring.rs:310); from then on the output is u8.
 //
-//   RESUME : the rest of a MARKER piece after its flip.  The window is the
-//            piece's last 32768 u16 entries (all literal), the ring is
-//            aligned to the entries already produced (seg0), and decoding
-//            restarts at the saved block header (tables rebuilt) and jumps to
-//            the saved symbol bit.
-//
-// Two kernels split a marker piece at its flip: the u16-ring kernel
-// (MARKER pieces only, 24 KB of LDS, 6 wavefronts per CU) decodes until the
-// flip and the u8-ring kernel (all other pieces, 16 KB, 8 per CU) decodes
-// the rest together with the DIRECT pieces.
+// Two kernels: the u16-ring kernel runs every piece of a call that has
+// MARKER pieces (DIRECT and WINDOW pieces too, so that one launch holds the
+// whole call), the u8-ring kernel calls without them.  Both use 16 KB of LDS
+// per wavefront, 8 wavefronts per CU (two per SIMD).
 //
 // Slab layout of a MARKER piece (bytes): [u16 entries: 2*n16] (256-aligned)
 // [u8 entries: n8].
@@ -41,17 +36,15 @@
 
 namespace gz {
 
-enum : uint32_t { PF_DIRECT = 1, PF_MARKER = 2, PF_WINDOW = 4, PF_RESUME = 8 };
+enum : uint32_t { PF_DIRECT = 1, PF_MARKER = 2, PF_WINDOW = 4 };
 
 struct PieceDesc {
-    uint64_t start_bit;      // first block header to decode (RESUME: header of the resumed block)
+    uint64_t start_bit;      // first block header to decode
     uint64_t stop_bit;       // stop before a non-final block header at >= stop_bit
     uint64_t limit_bit;      // end of the member's readable deflate bits
-    uint8_t *dst;            // DIRECT/WINDOW/RESUME: output bytes; MARKER: slab base
-    uint64_t cap;            // DIRECT/WINDOW/RESUME: output capacity (bytes); MARKER: slab bytes
-    const uint8_t *window;   // WINDOW: 32768 bytes; RESUME: 32768 u16 entries; oldest first
-    uint64_t resume_bit;     // RESUME inside a block: bit of the next symbol (0: start at a header)
-    uint64_t seg0;           // RESUME: entries the piece produced before dst
+    uint8_t *dst;            // DIRECT/WINDOW: output bytes; MARKER: slab base
+    uint64_t cap;            // DIRECT/WINDOW: output capacity (bytes); MARKER: slab bytes
+    const uint8_t *window;   // WINDOW: 32768 bytes, oldest first
     uint32_t flags;
     uint32_t member;
 };
@@ -63,24 +56,20 @@ struct PieceOut {
     uint64_t u8_off;    // slab byte offset of the u8 entries
     uint64_t bits_done; // bits consumed when the piece stopped (sizing on overflow)
     uint64_t t_begin, t_end;  // wall clock (100 MHz) around the piece
-    uint64_t hdr_bit;   // PS_FLIPPED inside a block: that block's header bit
     uint32_t status;
     uint32_t err;
     uint32_t nblocks;
     uint32_t mode;
-    uint32_t in_block;  // PS_FLIPPED: 1 = resume at end_bit inside the block at hdr_bit
-    uint32_t pad;
 };
 
-// Ring of the most recent 8192 entries in LDS; a copy from farther back (up
-// to the 32768 of the window) reads the piece's own output from HBM, or the
-// window before the piece (a u8 or u16 buffer, or, for a marker piece, the
-// marker values themselves).  The small ring lets 6 (u16 entries) or 8 (u8)
-// wavefronts share a CU, two per SIMD, which hides the decode loop's LDS,
-// scalar-load and readlane latencies: per-wave speed drops about 20% while
-// twice the wavefronts run.
+// Ring of the most recent entries in LDS: 4096 u16 or 8192 u8 entries, 8 KiB
+// either way, so that with the LUTs a wavefront needs 16 KB and 8 share a CU,
+// two per SIMD, which hides the decode loop's LDS, scalar-load and readlane
+// latencies (per-wave speed drops about 20% while twice the wavefronts run).
+// A copy from farther back (up to the 32768 of the window) reads the piece's
+// own output from HBM, or the window before the piece.
 template <typename RT>
-constexpr uint32_t RING = 8192u;
+constexpr uint32_t RING = sizeof(RT) == 2 ? 4096u : 8192u;
 template <typename RT>
 constexpr uint32_t RMASK = RING<RT> - 1;
 
@@ -89,8 +78,8 @@ struct __align__(16) InflateShared {
     // Two-level Huffman LUTs: a root of LLR / DR bits, then subtables for
     // the codes longer than the root (zlib's inflate_table layout), in the
     // same array after the root.  10/8-bit roots keep a wavefront's LDS at
-    // 16 KB (u8 ring) or 24 KB (u16 ring).  A subtable that does not fit its
-    // reserve leaves its codes to the canonical slow path.
+    // 16 KB.  A subtable that does not fit its reserve leaves its codes to
+    // the canonical slow path.
     static constexpr int LLR = 10;
     static constexpr int DR = 8;
     static constexpr int LL_SUB = 384;
@@ -108,10 +97,11 @@ struct __align__(16) InflateShared {
     uint8_t pre_lens[32];
     uint32_t qi;
     uint32_t sub_next;
-    // far-copy sources: piece output (RT entries) whose entry 0 is piece
-    // position far_base0, and the 32768 entries before that position
-    // (far_win_kind 0: none, 1: u8 buffer, 2: u16 buffer, 3: markers)
-    uint64_t far_out, far_win, far_base0;
+    // far-copy sources: piece positions [0, far_n16) are u16 entries at
+    // far_out16, from far_n16 on u8 entries at far_out8 (far_n16 moves from
+    // "all" to the flip position when a marker piece flips); before position
+    // 0 the window (far_win_kind 0: none, 1: u8 buffer, 3: markers)
+    uint64_t far_out16, far_out8, far_n16, far_win;
     uint32_t far_win_kind;
 };
 
@@ -327,19 +317,25 @@ __device__ __attribute__((always_inline)) uint32_t read_dynamic(SH &sh, BitReade
 }
 
 // ---- output segment ----------------------------------------------------
-// Entries go to `p` as RT: u16 in the marker kernel, u8 in the other.
-// `seg0` is the piece position of entry 0 of this segment (the ring index of
-// entry i is seg0 + i).
+// Entries go to `p` as u16 (wide: a marker piece before its flip, only in
+// the u16-ring kernel) or u8.  `seg0` is the piece position of entry 0 of
+// this segment (the ring index of entry i is seg0 + i).
 struct OutSeg {
     gptr<uint8_t> p;
     uint64_t pos, cap, seg0;
+    uint32_t wide;
 };
+
+template <typename RT>
+__device__ __forceinline__ bool is_wide(const OutSeg &o) {
+    return sizeof(RT) == 2 && o.wide;
+}
 
 // Entries are u8 in the u8-ring kernel at compile time; the u16-ring kernel
 // writes u16 until its flip.
 template <typename RT>
 __device__ __forceinline__ void put(const OutSeg &o, uint64_t i, uint32_t v) {
-    if (sizeof(RT) == 2) ((gptr<uint16_t>)o.p)[i] = (uint16_t)v;
+    if (is_wide<RT>(o)) ((gptr<uint16_t>)o.p)[i] = (uint16_t)v;
     else o.p[i] = (uint8_t)v;
 }
 
@@ -360,7 +356,7 @@ __device__ __forceinline__ void copy_chunk_out(RT *ring, const OutSeg &o, uint32
         ring[(gpos + i) & RMASK<RT>] = (RT)v;
         put<RT>(o, pos + i, v);
     }
-    if (sizeof(RT) == 2) {
+    if (is_wide<RT>(o)) {
         const uint64_t m = ballot(act && v >= 256);
         if (m) last_marker_end = o.seg0 + pos + (i - lane_id()) + 64 - (uint64_t)__clzll(m);
     }
@@ -380,9 +376,10 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
         // and the loads' wait covers its earlier stores.  Entries more than
         // 32768 before the piece do not exist (a distance the window cannot
         // serve): they read as 0 without a memory access.
-        const uint64_t out = uni64(sh.far_out), win = uni64(sh.far_win);
+        const uint64_t out16 = uni64(sh.far_out16), out8 = uni64(sh.far_out8), n16 = uni64(sh.far_n16);
+        const uint64_t win = uni64(sh.far_win);
         const uint32_t wk = uni(sh.far_win_kind);
-        const int64_t rel = (int64_t)(o.seg0 - uni64(sh.far_base0)) + (int64_t)pos - (int64_t)dist;
+        const int64_t rel = (int64_t)o.seg0 + (int64_t)pos - (int64_t)dist;  // piece position of entry 0
         uint32_t v[5];
 #pragma unroll
         for (uint32_t k = 0; k < 5; k++) {
@@ -390,10 +387,14 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
             const int64_t r = rel + (int64_t)i;
             uint32_t x = 0;
             if (i < len) {
-                if (r >= 0) x = to_global((const RT *)out)[r];
-                else if (r >= -(int64_t)WIN && wk == 1) x = to_global((const uint8_t *)win)[WIN + r];
-                else if (r >= -(int64_t)WIN && wk == 2) x = to_global((const uint16_t *)win)[WIN + r];
-                else if (r >= -(int64_t)WIN && wk == 3) x = MARKER_BASE + (uint32_t)(WIN + r);
+                if (r >= 0) {
+                    if (sizeof(RT) == 2 && (uint64_t)r < n16) x = to_global((const uint16_t *)out16)[r];
+                    else x = to_global((const uint8_t *)out8)[(uint64_t)r - n16];
+                } else if (r >= -(int64_t)WIN && wk == 1) {
+                    x = to_global((const uint8_t *)win)[WIN + r];
+                } else if (r >= -(int64_t)WIN && wk == 3) {
+                    x = MARKER_BASE + (uint32_t)(WIN + r);
+                }
             }
             v[k] = x;
         }
@@ -643,6 +644,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
     o.pos = uni64(o.pos);
     o.cap = uni64(o.cap);
     o.seg0 = uni64(o.seg0);
+    o.wide = uni(o.wide);
     limit_bit = uni64(limit_bit);
     // 32-bit segment positions: the segment is rebased before 2^30 entries
     uint32_t pos = (uint32_t)o.pos;
@@ -748,13 +750,13 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
                 break;
             }
         }
-        if (sizeof(RT) == 2 && o.seg0 + pos >= WIN && o.seg0 + pos - last_marker_end >= WIN) {
+        if (is_wide<RT>(o) && o.seg0 + pos >= WIN && o.seg0 + pos - last_marker_end >= WIN) {
             r = R_FLIP;
             break;
         }
         if (pos >= 0x40000000u) {
             // rebase the segment so positions stay 32-bit
-            o.p = o.p + sizeof(RT) * pos;
+            o.p = o.p + (is_wide<RT>(o) ? 2ull : 1ull) * pos;
             o.seg0 += pos;
             o.cap -= pos;
             g0 += pos;
@@ -783,7 +785,7 @@ __device__ __attribute__((always_inline)) void stored_copy(const uint8_t *in_, u
     const uint64_t w0 = src_byte >> 2;
     const uint64_t nwords = n >> 2;  // whole words inside the buffer
     const uint32_t ring_from = len > RING<RT> ? len - RING<RT> : 0;
-    const bool dst_aligned = sizeof(RT) == 1 && (((uint64_t)o.p + o.pos) & 3) == 0;
+    const bool dst_aligned = !is_wide<RT>(o) && (((uint64_t)o.p + o.pos) & 3) == 0;
     for (uint32_t k = 0; k < len; k += G * WAVE * 4) {
         uint32_t lo[G], hi[G];
         const uint64_t qk = (uint64_t)(k >> 2);
@@ -828,7 +830,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
                                                              uint64_t in_len, const PieceDesc &pd, PieceOut *po) {
     const uint32_t lane = lane_id();
     const bool marker = (pd.flags & PF_MARKER) != 0;
-    if (marker != (sizeof(RT) == 2)) {  // host error: marker pieces and only they take the u16 kernel
+    if (marker && sizeof(RT) == 1) {  // host error: marker pieces need the u16-ring kernel
         if (lane == 0) {
             po->status = PS_ERROR;
             po->err = E_NONE;
@@ -837,18 +839,17 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
         return;
     }
     gptr<uint8_t> dst = to_global(pd.dst);
-    const bool resume = (pd.flags & PF_RESUME) != 0;
-    const uint64_t base0 = resume ? pd.seg0 : 0;  // entries produced before dst
-    uint64_t resume_bit = resume ? pd.resume_bit : 0;
+    // marker pieces start with u16 entries; everything else writes u8
     OutSeg o;
     o.p = dst;
     o.pos = 0;
-    o.seg0 = base0;
-    o.cap = pd.cap / sizeof(RT);
+    o.seg0 = 0;
+    o.wide = marker ? 1u : 0u;
+    o.cap = marker ? pd.cap / 2 : pd.cap;
     uint64_t n16 = 0, u8_off = 0;
     uint64_t last_marker_end = 0;
-    uint32_t status = PS_NONE, err = E_NONE, nblocks = 0, in_block = 0;
-    uint64_t end_bit = 0, hdr_bit = 0;
+    uint32_t status = PS_NONE, err = E_NONE, nblocks = 0;
+    uint64_t end_bit = 0;
     uint64_t t_begin = wall_clock64();
     uint32_t fixed_built = 0;  // the fixed tables are in the LUTs
 
@@ -856,21 +857,17 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
     // position a at slot a mod RING)
     if (marker) {
         for (uint32_t q = lane; q < RING<RT>; q += WAVE) sh.ring[q] = (RT)(MARKER_BASE + WIN - RING<RT> + q);
-    } else if (resume) {
-        // the window's last RING entries (piece positions base0 - RING ..)
-        gptr<const uint16_t> w16 = (gptr<const uint16_t>)to_global(pd.window);
-        for (uint32_t q = WIN - RING<RT> + lane; q < WIN; q += WAVE)
-            sh.ring[(uint32_t)(base0 + q) & RMASK<RT>] = (RT)w16[q];
     } else if (pd.flags & PF_WINDOW) {
         for (uint32_t q = WIN - RING<RT> + lane; q < WIN; q += WAVE) sh.ring[q & RMASK<RT>] = to_global(pd.window)[q];
     } else {
         for (uint32_t q = lane; q < RING<RT>; q += WAVE) sh.ring[q] = 0;
     }
     if (lane == 0) {
-        sh.far_out = (uint64_t)pd.dst;
+        sh.far_out16 = (uint64_t)pd.dst;
+        sh.far_out8 = (uint64_t)pd.dst;
+        sh.far_n16 = marker ? ~0ull : 0ull;
         sh.far_win = (uint64_t)pd.window;
-        sh.far_base0 = base0;
-        sh.far_win_kind = marker ? 3u : resume ? 2u : (pd.flags & PF_WINDOW) ? 1u : 0u;
+        sh.far_win_kind = marker ? 3u : (pd.flags & PF_WINDOW) ? 1u : 0u;
     }
     __syncthreads();
 
@@ -924,9 +921,9 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
             } else {
                 stored_copy<RT>(in, in_len, src_byte, sh.ring, o, len);
                 o.pos += len;
-                if (sizeof(RT) == 2 && o.seg0 + o.pos >= WIN && o.seg0 + o.pos - last_marker_end >= WIN) r = R_FLIP;
+                if (is_wide<RT>(o) && o.seg0 + o.pos >= WIN && o.seg0 + o.pos - last_marker_end >= WIN) r = R_FLIP;
                 if (o.pos >= 0x40000000u) {  // keep segment positions 32-bit
-                    o.p = o.p + sizeof(RT) * o.pos;
+                    o.p = o.p + (is_wide<RT>(o) ? 2ull : 1ull) * o.pos;
                     o.seg0 += o.pos;
                     o.cap -= o.pos;
                     o.pos = 0;
@@ -946,31 +943,35 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
                     break;
                 }
             }
-            // a resumed piece re-read its block's header for the tables and
-            // continues at the saved symbol
-            uint64_t P = resume_bit ? resume_bit : br.pos();
-            resume_bit = 0;
+            uint64_t P = br.pos();
             r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err);
             br.init(in, in_len, P);
         }
         if (r == R_FLIP) {
-            // the last 32768 entries are all literal: the u8 kernel takes the
-            // rest (u8 entries from slab byte u8_off on), resuming at the next
-            // header (after a stored block) or inside this block.  After a
-            // final stored block there is no rest: the piece just ends.
-            if (btype == 0 && bfinal) {
-                r = R_EOB;
+            // the ring already holds the last entries, all literal: continue
+            // with u8 entries after the u16 ones (slab byte u8_off on)
+            n16 = o.seg0 + o.pos;
+            u8_off = (2 * n16 + 255) & ~255ull;
+            if (u8_off >= pd.cap) {
+                r = R_OVERFLOW;
             } else {
-                n16 = o.seg0 + o.pos;
-                u8_off = (2 * n16 + 255) & ~255ull;
-                if (u8_off >= pd.cap) {
-                    r = R_OVERFLOW;
+                o.p = dst + u8_off;
+                o.seg0 = n16;
+                o.cap = pd.cap - u8_off;
+                o.pos = 0;
+                o.wide = 0;
+                if (lane == 0) {
+                    sh.far_n16 = n16;
+                    sh.far_out8 = (uint64_t)pd.dst + u8_off;
+                }
+                __syncthreads();
+                // a flip inside a block resumes that block's symbols
+                if (btype != 0) {
+                    uint64_t P = br.pos();
+                    r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err);
+                    br.init(in, in_len, P);
                 } else {
-                    status = PS_FLIPPED;
-                    in_block = btype != 0;
-                    hdr_bit = hbit;
-                    end_bit = br.pos();
-                    break;
+                    r = R_EOB;
                 }
             }
         }
@@ -998,14 +999,12 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
     }
     __syncthreads();
     if (lane == 0) {
-        const bool flipped = status == PS_FLIPPED;
+        const bool flipped = marker && o.wide == 0;
         po->end_bit = end_bit;
         const uint64_t total = o.seg0 + o.pos;
         po->n16 = marker ? (flipped ? n16 : total) : 0;
-        po->n8 = marker ? 0 : total - base0;
+        po->n8 = marker ? (flipped ? total - n16 : 0) : total;
         po->u8_off = flipped ? u8_off : 0;
-        po->hdr_bit = hdr_bit;
-        po->in_block = in_block;
         po->bits_done = br.pos() - pd.start_bit;
         po->t_begin = t_begin;
         po->t_end = wall_clock64();
@@ -1021,7 +1020,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
 // longest-first.  RT = ring entry type: uint16_t for MARKER pieces, uint8_t
 // for all others.
 template <typename RT>
-__global__ void __launch_bounds__(64) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
                                                 const PieceDesc *__restrict__ pieces,
                                                 const uint32_t *__restrict__ order, uint32_t n, uint32_t *queue,
                                                 PieceOut *out) {

@@ -76,10 +76,12 @@ typedef struct {
     double ms_inflate_kernel;   /* summed k_inflate durations (HIP events around each launch) */
     uint32_t inflate_launches;  /* k_inflate launches (2 per speculation round: marker, clean) */
     uint32_t pad_;
-    double ms_inflate_marker;   /* u16-ring kernel: marker pieces up to their flip */
-    double ms_inflate_clean;    /* u8-ring kernel: window-known pieces and the rest of flipped ones */
+    double ms_inflate_marker;   /* u16-ring kernel launches (calls with speculative pieces) */
+    double ms_inflate_clean;    /* u8-ring kernel launches (calls without) */
     uint64_t clean_in_bytes;    /* compressed bytes consumed by the u8-ring kernel */
     uint64_t clean_out_bytes;   /* bytes written by the u8-ring kernel */
+    uint64_t marker_in_bytes;   /* compressed bytes consumed by the u16-ring kernel */
+    uint64_t marker_out_bytes;  /* bytes written by the u16-ring kernel (u16 entries count 2) */
 } gzgpu_stats;
 
 /* ---- context ---------------------------------------------------------- */
