@@ -54,17 +54,29 @@ __device__ __forceinline__ bool dyn_stage_a(const uint8_t *__restrict__ in, uint
     return leaves == 128 || (leaves == 64 && gt1 == 0);
 }
 
-// Precode condition of the dynamic predicate, from the 57 bits after the
-// 17 header bits (`p`) and HCLEN.
-__device__ __forceinline__ bool precode_leaves_ok(uint64_t p, uint32_t hclen) {
+// Leaf-sum table for the precode test: entry v (four 3-bit code lengths,
+// first in the low bits) = sum of 2^(7-l) over its nonzero lengths, plus the
+// count of lengths > 1 in bits 16 and up.  Five lookups cover the 19 lengths.
+__device__ __forceinline__ uint32_t precode_lut_entry(uint32_t v) {
     uint32_t leaves = 0, gt1 = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < 19; i++) {
-        uint32_t l = (uint32_t)(p >> (3 * i)) & 7;
-        if (i >= hclen + 4) l = 0;
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t l = (v >> (3 * i)) & 7;
         leaves += l ? (1u << (7 - l)) : 0u;
         gt1 += l > 1;
     }
+    return leaves | gt1 << 16;
+}
+
+// Precode condition of the dynamic predicate, from the 57 bits after the 17
+// header bits (`p`) and HCLEN: leaf count 128, or 64 with no length > 1
+// (lengths past HCLEN + 4 count as 0).
+__device__ __forceinline__ bool precode_ok_lut(const uint32_t *lut, uint64_t p, uint32_t hclen) {
+    const uint32_t nb = 3 * (hclen + 4);  // 12 .. 57
+    p &= (1ull << nb) - 1;
+    const uint32_t s = lut[(uint32_t)p & 0xFFF] + lut[(uint32_t)(p >> 12) & 0xFFF] + lut[(uint32_t)(p >> 24) & 0xFFF] +
+                       lut[(uint32_t)(p >> 36) & 0xFFF] + lut[(uint32_t)(p >> 48) & 0x1FF];
+    const uint32_t leaves = s & 0xFFFF, gt1 = s >> 16;
     return leaves == 128 || (leaves == 64 && gt1 == 0);
 }
 
@@ -118,15 +130,36 @@ struct BfRange {
 
 // Stage A.  Each lane owns 32 consecutive bit offsets [o0, o0 + 32) of one
 // range and reads their 128 bits once into registers.  The 13-bit header test
-// runs on all 32 offsets from the registers (v_alignbit); the precode sum only
-// on the offsets that pass it (about one in ten); the stored-block LEN/NLEN
-// test on the 4 byte-aligned offsets, the rest of the stored conditions only
-// on a match.  Survivors and hits are appended with one atomic per wave.
+// runs on all 32 offsets at once as bit operations on a 64-bit word; the
+// precode leaf sum (five lookups in an LDS table) only on the offsets that
+// pass it (about one in nine); the stored-block LEN/NLEN test on the 4
+// byte-aligned offsets, the rest of the stored conditions only on a match.
+// Survivors collect in a per-wave LDS buffer and go to HBM with one atomic
+// per 512 (one counter shared by every wave is the contended resource; an
+// atomic per wave and iteration made the kernel several times slower).
+// Hits are appended with one atomic per wave.
 __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ in, uint64_t n,
                                                     const BfRange *__restrict__ ranges, uint32_t nranges,
                                                     uint64_t nchunk, uint64_t *__restrict__ surv, uint32_t surv_cap,
                                                     uint32_t *surv_count, gzgpu_boundary *__restrict__ hits,
                                                     uint32_t hit_cap, uint32_t *hit_count) {
+    constexpr uint32_t WBUF = 512;
+    __shared__ uint32_t pre_lut[4096];
+    __shared__ uint64_t wbuf[4][WBUF];  // blockDim.x == 256: four wavefronts
+    for (uint32_t v = threadIdx.x; v < 4096; v += blockDim.x) pre_lut[v] = precode_lut_entry(v);
+    __syncthreads();
+    uint64_t *mybuf = wbuf[threadIdx.x / WAVE];
+    uint32_t wcnt = 0;  // wave-uniform fill of mybuf
+    auto flush = [&]() {
+        compiler_fence();
+        uint32_t base_slot = 0;
+        if (lane_id() == 0) base_slot = atomicAdd(surv_count, wcnt);
+        base_slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)base_slot);
+        for (uint32_t i = lane_id(); i < wcnt; i += WAVE)
+            if (base_slot + i < surv_cap) surv[base_slot + i] = mybuf[i];
+        compiler_fence();
+        wcnt = 0;
+    };
     const uint64_t total = n * 8;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     // wave-aligned iterations so every lane reaches the ballots
@@ -164,14 +197,12 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
                 w3 = load_word(in, n, wi + 3);
             }
         }
-        // ---- header test on all 32 offsets
-        uint32_t hm = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < 32; j++) {
-            const uint32_t h = __builtin_amdgcn_alignbit(w1, w0, j);
-            const bool pass = (h & 7) == 4 && ((h >> 3) & 31) <= 29 && ((h >> 8) & 31) <= 29;
-            hm |= (uint32_t)pass << j;
-        }
+        // ---- header test on all 32 offsets: bit j of the mask is the test
+        // at o0 + j.  BFINAL = 0 and BTYPE = 10 (bits 0, 1, 2 = 0, 0, 1);
+        // HLIT <= 29 and HDIST <= 29 (not all of bits 4-7, not all of 9-12)
+        const uint64_t X = (uint64_t)w0 | (uint64_t)w1 << 32;
+        uint32_t hm = (uint32_t)(~X & ~(X >> 1) & (X >> 2) & ~((X >> 4) & (X >> 5) & (X >> 6) & (X >> 7)) &
+                                 ~((X >> 9) & (X >> 10) & (X >> 11) & (X >> 12)));
         // offsets inside [start_bit, dyn_end)
         {
             const uint64_t lo = start_bit > o0 ? start_bit - o0 : 0;
@@ -191,21 +222,28 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
             const uint32_t a = k ? w1 : w0, b = k ? w2 : w1, d = k ? w3 : w2;
             const uint64_t p =
                 (uint64_t)__builtin_amdgcn_alignbit(b, a, r) | (uint64_t)__builtin_amdgcn_alignbit(d, b, r) << 32;
-            if (precode_leaves_ok(p, (h >> 13) & 15)) sm |= 1u << j;
+            if (precode_ok_lut(pre_lut, p, (h >> 13) & 15)) sm |= 1u << j;
         }
-        // one atomic per wave: lanes write their survivors after a prefix sum
+        // survivors: lanes write after a prefix sum, into the wave's LDS
+        // buffer (flushed when it would overflow), or straight to HBM with
+        // their own atomic when one iteration alone has more than it holds
         {
             const uint32_t cnt = (uint32_t)__builtin_popcount(sm);
             const uint32_t incl = wave_scan_incl(cnt);
             const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-            uint32_t base_slot = 0;
-            if (tot) {
+            if (tot > WBUF) {
+                uint32_t base_slot = 0;
                 if (lane_id() == 0) base_slot = atomicAdd(surv_count, tot);
                 base_slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)base_slot);
+                uint32_t slot = base_slot + incl - cnt;
+                for (uint32_t ss = sm; ss; ss &= ss - 1, slot++)
+                    if (slot < surv_cap) surv[slot] = o0 + (uint32_t)__builtin_ctz(ss);
+            } else if (tot) {
+                if (wcnt + tot > WBUF) flush();
+                uint32_t slot = wcnt + incl - cnt;
+                for (uint32_t ss = sm; ss; ss &= ss - 1, slot++) mybuf[slot] = o0 + (uint32_t)__builtin_ctz(ss);
+                wcnt += tot;
             }
-            uint32_t slot = base_slot + incl - cnt;
-            for (uint32_t ss = sm; ss; ss &= ss - 1, slot++)
-                if (slot < surv_cap) surv[slot] = o0 + (uint32_t)__builtin_ctz(ss);
         }
         // ---- stored blocks at the 4 byte-aligned offsets
 #pragma unroll
@@ -227,6 +265,7 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
             }
         }
     }
+    if (wcnt) flush();
 }
 
 // Full dynamic-header validation of one survivor (one lane).
