@@ -379,11 +379,12 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
 
 uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-// inflate wavefronts per CU (16 KB of LDS each, two per SIMD)
-constexpr uint32_t U8_WAVES_PER_CU = 8, U16_WAVES_PER_CU = 8;
+// inflate wavefronts per CU (12 KB of LDS each, three per SIMD)
+constexpr uint32_t U8_WAVES_PER_CU = 12, U16_WAVES_PER_CU = 12;
 // planner: smallest piece (compressed KiB; the reference's block finder
-// keeps 32 KiB, gzip_block_finder.rs:109 — half that fills the 2048 u8 slots
-// of a 1 GiB text member and measured 12% faster) and the first piece's share
+// keeps 32 KiB, gzip_block_finder.rs:109 — half that gives a 1 GiB text
+// member about one piece per resident wavefront and measured 12% faster)
+// and the first piece's share
 constexpr uint64_t MIN_SPACING_KIB = 16;
 constexpr double FIRST_PIECE = 0.75;
 
@@ -421,7 +422,7 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     c->d_pout.ensure(descs.size() * sizeof(PieceOut));
     c->d_counters.ensure(64);
     HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
-    // LDS bounds residency (160 KiB per CU): 16 KB per wavefront
+    // LDS bounds residency (160 KiB per CU): 12 KB per wavefront
     uint32_t per_cu = L.u16 ? U16_WAVES_PER_CU : U8_WAVES_PER_CU;
     if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
     L.grid = (uint32_t)std::min<uint64_t>(L.n, (uint64_t)c->num_cus * per_cu);

@@ -26,8 +26,8 @@
 //
 // Two kernels: the u16-ring kernel runs every piece of a call that has
 // MARKER pieces (DIRECT and WINDOW pieces too, so that one launch holds the
-// whole call), the u8-ring kernel calls without them.  Both use 16 KB of LDS
-// per wavefront, 8 wavefronts per CU (two per SIMD).
+// whole call), the u8-ring kernel calls without them.  Both use 12 KB of LDS
+// per wavefront, 12 wavefronts per CU (three per SIMD).
 //
 // Slab layout of a MARKER piece (bytes): [u16 entries: 2*n16] (256-aligned)
 // [u8 entries: n8].
@@ -62,14 +62,14 @@ struct PieceOut {
     uint32_t mode;
 };
 
-// Ring of the most recent entries in LDS: 4096 u16 or 8192 u8 entries, 8 KiB
-// either way, so that with the LUTs a wavefront needs 16 KB and 8 share a CU,
-// two per SIMD, which hides the decode loop's LDS, scalar-load and readlane
-// latencies (per-wave speed drops about 20% while twice the wavefronts run).
+// Ring of the most recent entries in LDS: 2048 u16 or 4096 u8 entries, 4 KiB
+// either way, so that with the LUTs a wavefront needs 12 KB and 12 share a
+// CU, three per SIMD (the kernel stays under 168 VGPRs), which hides the
+// decode loop's LDS, scalar-load and readlane latencies.
 // A copy from farther back (up to the 32768 of the window) reads the piece's
 // own output from HBM, or the window before the piece.
 template <typename RT>
-constexpr uint32_t RING = sizeof(RT) == 2 ? 4096u : 8192u;
+constexpr uint32_t RING = sizeof(RT) == 2 ? 2048u : 4096u;
 template <typename RT>
 constexpr uint32_t RMASK = RING<RT> - 1;
 
@@ -699,29 +699,44 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             P = uni64(P);
             lookahead(sh, in, n, P, la);
         } else {
-            // a ring copy whose source slot a later entry of this round would
-            // overwrite first (distance near the ring size): end the round
-            // after it.  A copy from beyond the ring whose source
-            // reaches into this round's own output, not yet in HBM: end the
-            // round before it (never the round's first symbol: distance >
-            // ring > length).
+            // The round writes its literals to the ring before its copies, so
+            // it is cut short where that order would matter:
+            //  - a ring copy whose source slot a later entry of the round
+            //    would overwrite first (distance + rest of the round > ring):
+            //    the round ends after it;
+            //  - a symbol whose output would end more than a ring past the
+            //    round's start: a copy's destination slot would alias a
+            //    later literal's, and the copy, written second, would win.
+            //    The round ends before it;
+            //  - a copy from beyond the ring whose source reaches into this
+            //    round's own output, not yet in HBM: the round ends before
+            //    it.
+            // None of these can cut before the round's first symbol (lengths
+            // are at most 258, the ring holds 4096 or more entries).
+            const bool inM = (M >> lane) & 1;
             const bool is_copy = ((M & ~LITM & ~EOBM) >> lane) & 1;
             const bool far = la.dist > RING<RT>;
             const uint64_t BAD = ballot(is_copy && !far && la.dist + run - opos > RING<RT>);
             const uint64_t BADF = ballot(lane != 0 && is_copy && far && la.dist < opos + la.olen);
-            if (BAD | BADF) {
-                const uint32_t fa = BAD ? uni((uint32_t)__builtin_ctzll(BAD)) : 64u;
-                const uint32_t ff = BADF ? uni((uint32_t)__builtin_ctzll(BADF)) : 64u;
-                if (fa < ff) {
-                    M &= fa == 63 ? ~0ull : (2ull << fa) - 1;
-                    run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, (int)fa));
-                    s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)fa));
-                } else {
-                    M &= (1ull << ff) - 1;
-                    run = uni((uint32_t)__builtin_amdgcn_readlane((int)opos, (int)ff));
-                    s = ff;
+            const uint64_t OVER = ballot(lane != 0 && inM && incl > RING<RT>);
+            if (BAD | BADF | OVER) {
+                uint32_t cut = 64;  // keep the symbols at lanes below
+                if (BAD) cut = uni((uint32_t)__builtin_ctzll(BAD)) + 1;
+                if (BADF) {
+                    const uint32_t ff = uni((uint32_t)__builtin_ctzll(BADF));
+                    cut = ff < cut ? ff : cut;
                 }
-                eob = 0;
+                if (OVER) {
+                    const uint32_t fo = uni((uint32_t)__builtin_ctzll(OVER));
+                    cut = fo < cut ? fo : cut;
+                }
+                if (cut < 64) {
+                    M &= (1ull << cut) - 1;
+                    const uint32_t last = uni((uint32_t)(63 - __builtin_clzll(M)));
+                    run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, (int)last));
+                    s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)last));
+                    eob = ((EOBM >> last) & 1) ? eob : 0u;
+                }
             }
             eob = uni(eob);
             const uint64_t lits = uni64(M & LITM), matches = uni64(M & ~LITM & ~EOBM);
@@ -769,15 +784,15 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
 }
 
 // Stored block body: LEN bytes from the input at byte `src_byte`.  The wave
-// moves 4 KiB per step: every lane issues its aligned dword loads for 16
+// moves 1 KiB per step: every lane issues its aligned dword loads for 4
 // four-byte groups before the one wait, realigns them with v_alignbit and
 // writes the output (one dword store per group when the output is dword
-// aligned and the entries are bytes).  Only the block's last 32768 entries
+// aligned and the entries are bytes).  Only the block's last RING entries
 // go to the ring; earlier ones would be overwritten there.
 template <typename RT>
 __device__ __attribute__((always_inline)) void stored_copy(const uint8_t *in_, uint64_t n, uint64_t src_byte, RT *ring,
                                                            const OutSeg &o, uint32_t len) {
-    constexpr uint32_t G = 16;  // groups per lane per step
+    constexpr uint32_t G = 4;  // groups per lane per step (more would raise the kernel's VGPR peak)
     const uint32_t lane = lane_id();
     gptr<const uint32_t> inw = (gptr<const uint32_t>)to_global(in_);
     const uint64_t gpos = o.seg0 + o.pos;
@@ -1020,7 +1035,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
 // longest-first.  RT = ring entry type: uint16_t for MARKER pieces, uint8_t
 // for all others.
 template <typename RT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
                                                 const PieceDesc *__restrict__ pieces,
                                                 const uint32_t *__restrict__ order, uint32_t n, uint32_t *queue,
                                                 PieceOut *out) {

@@ -117,3 +117,58 @@ def test_inflate_piece_markers_equal_oracle(gpu, oracle, kind, level):
         assert r.status == 0 and r.out_len == ores.out_len and r.end_bit == ores.end_bit
         got = m[: r.out_len].cpu().numpy().view(np.uint16)
         assert np.array_equal(got, omarks), (kind, level, bi)
+
+
+def long_round_stream(run_len, reps=120, seed=5):
+    """Runs of one byte long enough that a 64-bit lookahead window holds a
+    ring's worth of output (many 258-byte copies with 2-3 bit codes), each
+    followed by literals that later copies reference at short distance: a
+    round whose copy destinations alias its own later literals in the ring."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for i in range(reps):
+        a, b = rng.integers(65, 91, 2)
+        tail = bytes([b, (b + 1) % 256, (b + 2) % 256]) * 4
+        parts.append(bytes([a]) * run_len + tail + bytes(rng.integers(0, 256, 3, dtype=np.uint8)))
+    return np.frombuffer(b"".join(parts), np.uint8).copy()
+
+
+@pytest.mark.parametrize("run_len", [4500, 7740, 9100, 16500, 33000])
+def test_round_longer_than_ring(gpu, oracle, run_len):
+    """Both kernels on rounds that produce more entries than their LDS ring:
+    marker mode (u16 ring) and window mode (u8 ring) from the stream start,
+    and the whole member through the pipeline."""
+    import torch
+    raw = long_round_stream(run_len)
+    say("long rounds", run_len, raw.size)
+    for level in (6, 9):
+        d = deflate_raw(raw, level)
+        td = dev(d)
+        m = torch.zeros(raw.size + 8192, dtype=torch.int16, device="cuda")
+        r = gpu.inflate_piece(td, 0, 2**64 - 1, m)
+        assert r.status == 0 and r.out_len == raw.size
+        got = m[: raw.size].cpu().numpy().view(np.uint16)
+        assert np.array_equal(got, raw.astype(np.uint16)), (run_len, level, "marker")
+        o = torch.zeros(raw.size + 64, dtype=torch.uint8, device="cuda")
+        r = gpu.inflate_piece(td, 0, 2**64 - 1, o, window=bytes(32768))
+        assert r.status == 0 and r.out_len == raw.size
+        assert np.array_equal(o[: raw.size].cpu().numpy(), raw), (run_len, level, "window")
+        gz = W.gzip_compress(raw, level)
+        assert gpu.decompress(gz) == raw.tobytes(), (run_len, level, "member")
+
+
+def test_runs_level9_split_members(gpu):
+    """Long-run data at level 9 split into speculative pieces (the marker
+    kernel): copies far back and rounds of several thousand entries."""
+    ctx = gpu.context()
+    old = ctx.tuning()
+    ctx.set_tuning(chunk_bytes=64 << 10, pieces_per_chunk=4)
+    try:
+        for seed in (9113, 77, 4242):
+            raw = W.generate("runs", 8 << 20, seed)
+            gz = W.gzip_compress(raw, 9)
+            st = {}
+            assert gpu.decompress(gz, stats=st) == raw.tobytes(), seed
+            say("runs L9", seed, gz.size, "pieces", st["pieces"])
+    finally:
+        ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
