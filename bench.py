@@ -156,6 +156,63 @@ def batch_leg(args, dev, ctx):
             "check": "every member's bytes equal the generator's; CRC32 and ISIZE verified per member"}
 
 
+def graph_leg(args, dev, ctx, members, max_kib):
+    """A batch of small members (lognormal around 16 KiB, at most `max_kib`;
+    the four kinds, levels 1/6/9): the hipGraph form (one replay: header
+    parse, inflate from a device-side queue, CRC32 and trailer checks on the
+    device; members decoded whole) against gzgpu_decompress_batch (host
+    planning, members split into pieces) on the same members."""
+    import numpy as np
+    import torch
+
+    import gzippy_amd
+    from gzippy_amd import workloads as W
+
+    mem = W.batch_members(members, seed=args.seed + 5, median=16 << 10, max_size=max_kib << 10)
+    raws = [m[0] for m in mem]
+    ins = [m[1] for m in mem]
+    in_len = np.array([x.size for x in ins], dtype=np.uint64)
+    in_off = np.concatenate([[0], np.cumsum(in_len)[:-1]]).astype(np.uint64)
+    out_cap = np.array([r.size for r in raws], dtype=np.uint64)
+    out_off = np.concatenate([[0], np.cumsum(out_cap)[:-1]]).astype(np.uint64)
+    C, D = int(in_len.sum()), int(out_cap.sum())
+    d_in = torch.from_numpy(np.concatenate(ins)).to(dev)
+    d_out = torch.empty(D + 1, dtype=torch.uint8, device=dev)
+    ref = torch.from_numpy(np.concatenate(raws)).to(dev)
+    del raws, ins, mem
+
+    def timed(fn, steps):
+        ts_all = []
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts_all.append(time.perf_counter() - ts)
+        return sorted(ts_all)[len(ts_all) // 2]
+
+    d_out.zero_()
+    with gzippy_amd.BatchGraph(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=ctx) as g:
+        status, olen, _, _ = g.launch()
+        torch.cuda.synchronize()
+        if not (bool((status == 0).all()) and torch.equal(d_out[:D], ref)):
+            log("FAILED: graph output differs")
+            sys.exit(1)
+        g_s = timed(lambda: g.launch(), args.batch_steps)
+    d_out.zero_()
+    gzippy_amd.decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=ctx)
+    torch.cuda.synchronize()
+    if not torch.equal(d_out[:D], ref):
+        log("FAILED: batched output differs (small members)")
+        sys.exit(1)
+    b_s = timed(lambda: gzippy_amd.decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=ctx),
+                args.batch_steps)
+    return {"members": len(in_off), "compressed_bytes": C, "decompressed_bytes": D, "seed": args.seed + 5,
+            "graph_gbs": round(D / g_s / 1e9, 3), "graph_ms": round(g_s * 1e3, 3),
+            "batched_gbs": round(D / b_s / 1e9, 3), "batched_ms": round(b_s * 1e3, 3),
+            "check": "every member's bytes equal the generator's; CRC32 and ISIZE verified per member on the device"}
+
+
 def serve_leg(args, rates):
     """Open-loop Poisson arrivals at each rate for args.serve_seconds; requests
     follow the batch mix, 10 % of them 4-64-member concatenations.  Latency is
@@ -355,6 +412,8 @@ def main():
         del d_in, d_out
         torch.cuda.empty_cache()
         line["batch"] = batch_leg(args, dev, ctx)
+        torch.cuda.empty_cache()
+        line["batch_graph"] = [graph_leg(args, dev, ctx, 512, 64), graph_leg(args, dev, ctx, 10000, 256)]
     if args.serve_seconds > 0:
         torch.cuda.empty_cache()
         line["serving"] = serve_leg(args, (200, 1000, 5000))

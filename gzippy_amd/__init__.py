@@ -30,7 +30,7 @@ from ._native import GzgpuError, Stats, Tuning  # noqa: F401
 __all__ = [
     "GzippyError", "decompress", "decompress_with_threads", "decompress_to_writer",
     "decompress_to_writer_with_threads", "decompress_device", "decompress_batch", "find_blocks", "crc32",
-    "inflate_piece", "context", "Server", "autotune",
+    "inflate_piece", "context", "Server", "autotune", "BatchGraph",
 ]
 
 
@@ -182,6 +182,61 @@ def decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None, ra
     if st != _native.OK and raise_on_error:
         _raise(ctx, st)
     return status, olen, crcs, st_obj.as_dict()
+
+
+class BatchGraph:
+    """A fixed batch layout captured as one hipGraph (gzgpu_batch_graph_*):
+    header parse, the persistent inflate kernel fed by a device-side queue,
+    CRC32 and trailer checks all on the device.  launch() replays it; the
+    contents of d_in may change between launches, the layout may not.  Members
+    are decoded whole, one wavefront each."""
+
+    def __init__(self, d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None):
+        self.ctx = ctx or context(d_in.device.index or 0)
+        self._keep = (d_in, d_out)
+        _sync_torch(d_in)
+        self.n = len(in_off)
+        a = [np.ascontiguousarray(np.asarray(x, dtype=np.uint64)) for x in (in_off, in_len, out_off, out_cap)]
+        P64 = ctypes.POINTER(ctypes.c_uint64)
+        h = ctypes.c_void_p()
+        st = self.ctx.lib.gzgpu_batch_graph_create(
+            self.ctx.h, _ptr(d_in), a[0].ctypes.data_as(P64), a[1].ctypes.data_as(P64), _ptr(d_out),
+            a[2].ctypes.data_as(P64), a[3].ctypes.data_as(P64), self.n, ctypes.byref(h))
+        if st != _native.OK:
+            _raise(self.ctx, st)
+        self.h = h
+
+    def launch(self, raise_on_error=True):
+        """Replay; returns (status[], out_len[], crc[], stats dict)."""
+        _sync_torch(self._keep[0])
+        status = np.zeros(self.n, dtype=np.int32)
+        olen = np.zeros(self.n, dtype=np.uint64)
+        crcs = np.zeros(self.n, dtype=np.uint32)
+        st_obj = Stats()
+        st = self.ctx.lib.gzgpu_batch_graph_launch(
+            self.h, status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            olen.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), crcs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+            ctypes.byref(st_obj))
+        if st != _native.OK and raise_on_error:
+            _raise(self.ctx, st)
+        return status, olen, crcs, st_obj.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.gzgpu_batch_graph_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def autotune(d_in, in_off, in_len, d_out, out_off, out_cap, budget_ms=2000, ctx=None):

@@ -23,7 +23,8 @@ EXPORTS = [
     "gzgpu_status_string", "gzgpu_decompress", "gzgpu_decompress_to_writer", "gzgpu_decompress_device",
     "gzgpu_decompress_batch", "gzgpu_find_blocks", "gzgpu_crc32", "gzgpu_inflate_piece",
     "gzgpu_server_create", "gzgpu_server_destroy", "gzgpu_submit", "gzgpu_poll", "gzgpu_fetch",
-    "gzgpu_server_get_stats", "gzgpu_autotune",
+    "gzgpu_server_get_stats", "gzgpu_autotune", "gzgpu_batch_graph_create", "gzgpu_batch_graph_launch",
+    "gzgpu_batch_graph_destroy",
 ]
 PENDING = 11
 
@@ -130,6 +131,12 @@ def load_library():
         L.gzgpu_server_get_stats.argtypes = [vp, P(ServerStats)]
         L.gzgpu_autotune.argtypes = [vp, vp, P(u64), P(u64), vp, P(u64), P(u64), sz, ctypes.c_uint32, P(Tuning),
                                      P(ctypes.c_double)]
+        L.gzgpu_batch_graph_create.restype = ctypes.c_int
+        L.gzgpu_batch_graph_create.argtypes = [vp, vp, P(u64), P(u64), vp, P(u64), P(u64), sz, P(vp)]
+        L.gzgpu_batch_graph_launch.restype = ctypes.c_int
+        L.gzgpu_batch_graph_launch.argtypes = [vp, P(ctypes.c_int32), P(u64), P(ctypes.c_uint32), P(Stats)]
+        L.gzgpu_batch_graph_destroy.restype = None
+        L.gzgpu_batch_graph_destroy.argtypes = [vp]
         for name in EXPORTS:
             getattr(L, name)  # every declared symbol must resolve
         _lib = L

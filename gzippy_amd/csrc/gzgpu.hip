@@ -1512,3 +1512,4 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
 }  // extern "C"
 
 #include "serve.h"
+#include "graph.h"

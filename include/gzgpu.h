@@ -139,6 +139,27 @@ int gzgpu_autotune(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_off, con
                    const uint64_t *out_off, const uint64_t *out_cap, size_t n, uint32_t budget_ms,
                    gzgpu_tuning *best, double *best_ms);
 
+/* ---- batch as a replayable hipGraph ------------------------------------- */
+/* Captures the whole decode of a fixed batch layout (members as for
+ * gzgpu_decompress_batch) into one hipGraph: header parse, a persistent
+ * inflate kernel fed by a device-side work queue (one wavefront per member,
+ * longest first), CRC32 and the trailer checks all run on the device, so a
+ * launch is one hipGraphLaunch and one small copy back.  The contents of d_in
+ * may change between launches; offsets, lengths and capacities may not.
+ * Members are decoded whole (no speculative split): meant for batches of many
+ * small members.  Destroy a graph before its context.  No counterpart in the
+ * reference (the batched form of decompress_multi_member_parallel_to_vec,
+ * bgzf.rs:2370, with the device keeping every decision). */
+typedef struct gzgpu_batch_graph gzgpu_batch_graph;
+int gzgpu_batch_graph_create(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_off, const uint64_t *in_len,
+                             void *d_out, const uint64_t *out_off, const uint64_t *out_cap, size_t n,
+                             gzgpu_batch_graph **out);
+/* status / out_len / crc: n entries each (NULL to skip); returns the first
+ * member's error or GZGPU_OK. */
+int gzgpu_batch_graph_launch(gzgpu_batch_graph *g, int32_t *status, uint64_t *out_len, uint32_t *crc,
+                             gzgpu_stats *stats);
+void gzgpu_batch_graph_destroy(gzgpu_batch_graph *g);
+
 /* ---- exposed kernels (parity tests, tools) ------------------------------- */
 typedef struct {
     uint64_t bit_offset; /* earliest valid start */

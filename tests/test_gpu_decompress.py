@@ -275,3 +275,51 @@ def test_batch_block_finder_covers_every_member(gpu):
             assert np.array_equal(host[out_off[j]: out_off[j] + raws[i].size], raws[i]), j
     finally:
         ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
+
+
+def test_batch_graph_replays_match_batch(gpu):
+    """The hipGraph form: every replay equals gzgpu_decompress_batch on the
+    same members (status, length, CRC, bytes), with torch work and syncs
+    between replays; a byte changed in d_in between launches shows up in the
+    next launch (the graph reads the current contents) and goes away when
+    restored.  Bad members fail alone."""
+    import torch
+    mem = W.batch_members(300, seed=515, max_size=1 << 20)
+    ins = [m[1].copy() for m in mem]
+    raws = [m[0] for m in mem]
+    ins[5] = np.frombuffer(b"not a gzip member at all....", np.uint8).copy()  # bad header
+    raws[5] = np.zeros(0, np.uint8)
+    bad_crc = ins[7].copy()
+    bad_crc[-8] ^= 0x5A
+    ins[7] = bad_crc
+    in_off = np.cumsum([0] + [x.size for x in ins[:-1]])
+    in_len = np.array([x.size for x in ins])
+    out_cap = np.array([max(r.size, 1) for r in raws])
+    out_cap[9] = max(1, raws[9].size // 2)  # too small
+    out_off = np.cumsum([0] + list(out_cap[:-1]))
+    host_in = np.concatenate(ins)
+    d_in = dev(host_in)
+    ref_out = torch.zeros(int(out_cap.sum()), dtype=torch.uint8, device="cuda")
+    ref = gpu.decompress_batch(d_in, in_off, in_len, ref_out, out_off, out_cap, raise_on_error=False)
+    d_out = torch.zeros_like(ref_out)
+    with gpu.BatchGraph(d_in, in_off, in_len, d_out, out_off, out_cap) as g:
+        for rep in range(3):
+            d_out.zero_()
+            torch.cuda.synchronize()
+            status, olen, crc, st = g.launch(raise_on_error=False)
+            say("graph replay", rep, "ms", round(st["ms_total"], 3), "bad", np.nonzero(status)[0][:5])
+            assert np.array_equal(status, ref[0]), rep
+            good = status == 0
+            assert np.array_equal(olen[good], ref[1][good]) and np.array_equal(crc[good], ref[2][good])
+            for i in np.nonzero(good)[0][:50]:
+                assert np.array_equal(d_out[out_off[i]: out_off[i] + raws[i].size].cpu().numpy(), raws[i]), (rep, i)
+        # change one member's trailer CRC in place, then restore it
+        j = int(np.nonzero(ref[0] == 0)[0][3])
+        pos = int(in_off[j] + in_len[j] - 8)
+        d_in[pos] ^= 0xFF
+        status, _, _, _ = g.launch(raise_on_error=False)
+        assert status[j] != 0 and (status[np.arange(len(status)) != j] == ref[0][np.arange(len(status)) != j]).all()
+        d_in[pos] ^= 0xFF
+        status, _, _, _ = g.launch(raise_on_error=False)
+        assert np.array_equal(status, ref[0])
+    assert ref[0][5] != 0 and ref[0][7] != 0 and ref[0][9] != 0
