@@ -716,10 +716,11 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             const bool inM = (M >> lane) & 1;
             const bool is_copy = ((M & ~LITM & ~EOBM) >> lane) & 1;
             const bool far = la.dist > RING<RT>;
-            const uint64_t BAD = ballot(is_copy && !far && la.dist + run - opos > RING<RT>);
-            const uint64_t BADF = ballot(lane != 0 && is_copy && far && la.dist < opos + la.olen);
-            const uint64_t OVER = ballot(lane != 0 && inM && incl > RING<RT>);
-            if (BAD | BADF | OVER) {
+            const bool bad = is_copy && !far && la.dist + run - opos > RING<RT>;
+            const bool badf = lane != 0 && is_copy && far && la.dist < opos + la.olen;
+            const bool over = lane != 0 && inM && incl > RING<RT>;
+            if (ballot(bad || badf || over)) {  // rare: one ballot in the common case
+                const uint64_t BAD = ballot(bad), BADF = ballot(badf), OVER = ballot(over);
                 uint32_t cut = 64;  // keep the symbols at lanes below
                 if (BAD) cut = uni((uint32_t)__builtin_ctzll(BAD)) + 1;
                 if (BADF) {
