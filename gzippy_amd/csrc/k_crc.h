@@ -5,11 +5,9 @@
 // bytes from a zero register.  The lane CRCs combine with the GF(2) shift
 // crc(A||B) = crc(A) * x^(8|B|) mod P ^ crc(B) (combine_crc32,
 // crc32.rs:127-133; polynomial_multiply_modulo :64; x_power_modulo :104):
-// for a full, 16-byte-aligned segment lane l's CRC is multiplied by the
-// constant x^(8 * 256 * (255 - l)) and the 256 products are XORed; the
-// segment is first staged through LDS with coalesced 16-byte loads.  A short
-// or unaligned segment (a member's last, or an unaligned member) folds the
-// lanes in a tree with computed shifts.
+// lane l's CRC is multiplied by the constant x^(8 * 256 * (255 - l)) and the
+// 256 products are XORed.  A segment shorter than 64 KiB is placed at the end
+// of the 64 KiB frame behind zeros, which a raw CRC register ignores.
 // Stage 2 (k_crc_members): per member, fold its segment CRCs the same way
 // and apply the init/final inversion: crc = ~(x^(8D) * 0xFFFFFFFF ^ raw).
 #pragma once
@@ -49,21 +47,27 @@ __constant__ uint32_t c_kseg[256];
 // shift a raw CRC register over `len` zero bytes
 __device__ __forceinline__ uint32_t crc_shift(uint32_t c, uint64_t len) { return len ? gf2_mulmod(xpow_bytes(len), c) : c; }
 
-__device__ __forceinline__ uint32_t crc_byte(const uint32_t *t, uint32_t c, uint8_t b) { return (c >> 8) ^ t[(c ^ b) & 0xFF]; }
 
-// Segments: seg_off[i] bytes into `base`, seg_len[i] <= CRC_SEG.
+// Segments: seg_off[i] bytes into `base`, seg_len[i] <= CRC_SEG.  Every
+// segment is folded as the last seg_len bytes of a 64 KiB frame whose first
+// bytes are zero: leading zeros leave a raw CRC register at 0, so the frame's
+// CRC is the segment's.  The frame is staged in LDS (coalesced loads), then
+// lane l folds frame bytes [256 l, 256 l + 256) with slicing-by-8 and the
+// lanes combine with the constant shifts c_kseg.  A full, 16-byte-aligned
+// segment is staged with 16-byte loads; any other with dword loads realigned
+// by v_alignbit (only dwords that hold segment bytes are read).
 __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict__ base, const uint64_t *__restrict__ seg_off,
                                                       const uint32_t *__restrict__ seg_len, uint32_t nseg,
                                                       const uint32_t *__restrict__ tables, uint32_t *__restrict__ seg_crc) {
     __shared__ uint32_t t[8][256];
-    __shared__ uint32_t part[256];
-    __shared__ uint32_t buf[256 * 65];  // lane l's 256 bytes at dword 65 * l (conflict-free reads)
+    __shared__ uint32_t buf[256 * 65];  // frame row r (bytes [256 r, 256 r + 256)) at dword 65 r (conflict-free reads)
     __shared__ uint32_t red;
     for (uint32_t i = threadIdx.x; i < 8 * 256; i += blockDim.x) t[i >> 8][i & 255] = tables[i];
     __syncthreads();
     for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
         const uint8_t *p = base + seg_off[s];
         const uint32_t len = seg_len[s];
+        const uint32_t f0 = CRC_SEG - len;  // frame byte of the segment's first byte
         if (len == CRC_SEG && (((uintptr_t)p) & 15) == 0) {
             typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
             gptr<const u32x4> src = (gptr<const u32x4>)to_global(p);
@@ -77,10 +81,31 @@ __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict_
                 d[2] = v.z;
                 d[3] = v.w;
             }
-            if (threadIdx.x == 0) red = 0;
-            __syncthreads();
+        } else if (len) {
+            // frame dword D = bytes [4D, 4D + 4) = segment bytes [4D - f0, ..):
+            // aligned source dwords a and a + 1, shifted by the misalignment
+            const uintptr_t g0 = (uintptr_t)p - f0;  // address of frame byte 0 (never dereferenced)
+            const uint32_t sh = (uint32_t)(g0 & 3) * 8;
+            const uintptr_t lo_dw = (uintptr_t)p >> 2, hi_dw = ((uintptr_t)p + len - 1) >> 2;  // dwords holding segment bytes
+            for (uint32_t D = (f0 >> 2) + threadIdx.x; D < CRC_SEG / 4; D += blockDim.x) {
+                const uintptr_t a = (g0 >> 2) + D;
+                const uint32_t w0 = a >= lo_dw && a <= hi_dw ? *(gptr<const uint32_t>)to_global((const uint32_t *)(a << 2)) : 0u;
+                const uint32_t w1 = sh && a + 1 >= lo_dw && a + 1 <= hi_dw
+                                        ? *(gptr<const uint32_t>)to_global((const uint32_t *)((a + 1) << 2))
+                                        : 0u;
+                uint32_t v = sh ? __builtin_amdgcn_alignbit(w1, w0, sh) : w0;
+                if (4 * D < f0) v &= 0xFFFFFFFFu << (8 * (f0 - 4 * D));  // bytes before the segment
+                buf[65 * (D >> 6) + (D & 63)] = v;
+            }
+            // the first data row's bytes before the segment read as zero
+            const uint32_t r0 = f0 >> 8;
+            for (uint32_t D = r0 * 64 + threadIdx.x; D < (f0 >> 2); D += blockDim.x) buf[65 * (D >> 6) + (D & 63)] = 0;
+        }
+        if (threadIdx.x == 0) red = 0;
+        __syncthreads();
+        uint32_t c = 0;
+        if (len && threadIdx.x >= (f0 >> 8)) {  // rows before the segment are all zero: CRC 0
             const uint32_t *my = &buf[65 * threadIdx.x];
-            uint32_t c = 0;
 #pragma unroll 4
             for (uint32_t j = 0; j < 32; j++) {
                 const uint32_t lo = c ^ my[2 * j], hi = my[2 * j + 1];
@@ -88,40 +113,10 @@ __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict_
                     t[3][hi & 0xFF] ^ t[2][(hi >> 8) & 0xFF] ^ t[1][(hi >> 16) & 0xFF] ^ t[0][hi >> 24];
             }
             c = gf2_mulmod(c_kseg[threadIdx.x], c);
-            atomicXor(&red, c);
-            __syncthreads();
-            if (threadIdx.x == 0) seg_crc[s] = red;
-            __syncthreads();
-            continue;
         }
-        const uint32_t b0 = threadIdx.x * 256;
-        uint32_t mine = b0 < len ? (len - b0 < 256 ? len - b0 : 256) : 0;
-        const uint8_t *q = p + b0;
-        uint32_t c = 0, k = 0;
-        if ((((uintptr_t)q) & 3) == 0) {
-            const uint32_t *w = reinterpret_cast<const uint32_t *>(q);
-            for (; k + 8 <= mine; k += 8) {
-                uint32_t lo = c ^ w[k / 4], hi = w[k / 4 + 1];
-                c = t[7][lo & 0xFF] ^ t[6][(lo >> 8) & 0xFF] ^ t[5][(lo >> 16) & 0xFF] ^ t[4][lo >> 24] ^
-                    t[3][hi & 0xFF] ^ t[2][(hi >> 8) & 0xFF] ^ t[1][(hi >> 16) & 0xFF] ^ t[0][hi >> 24];
-            }
-        }
-        for (; k < mine; k++) c = crc_byte(t[0], c, q[k]);
-        part[threadIdx.x] = c;
+        if (c) atomicXor(&red, c);
         __syncthreads();
-        // tree: node t covers [t*256, ...); right neighbour at stride `w`
-        for (uint32_t w = 1; w < 256; w <<= 1) {
-            uint32_t i = threadIdx.x;
-            if ((i & (2 * w - 1)) == 0 && i + w < 256) {
-                uint64_t rs = (uint64_t)(i + w) * 256;
-                uint64_t re = rs + (uint64_t)w * 256;
-                if (re > len) re = len;
-                uint64_t rlen = re > rs ? re - rs : 0;
-                if (rlen) part[i] = crc_shift(part[i], rlen) ^ part[i + w];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) seg_crc[s] = part[0];
+        if (threadIdx.x == 0) seg_crc[s] = red;
         __syncthreads();
     }
 }
