@@ -379,8 +379,8 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
 
 uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-// inflate wavefronts per CU (12 KB of LDS each, three per SIMD)
-constexpr uint32_t U8_WAVES_PER_CU = 12, U16_WAVES_PER_CU = 12;
+// inflate wavefronts per CU (10 KB of LDS each, four per SIMD)
+constexpr uint32_t U8_WAVES_PER_CU = 16, U16_WAVES_PER_CU = 16;
 // planner: smallest piece (compressed KiB; the reference's block finder
 // keeps 32 KiB, gzip_block_finder.rs:109) and pieces per resident wavefront.
 // Pieces start at block boundaries, so their sizes vary with the blocks';
@@ -423,7 +423,7 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     c->d_pout.ensure(descs.size() * sizeof(PieceOut));
     c->d_counters.ensure(64);
     HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
-    // LDS bounds residency (160 KiB per CU): 12 KB per wavefront
+    // LDS bounds residency (160 KiB per CU): 10 KB per wavefront
     uint32_t per_cu = L.u16 ? U16_WAVES_PER_CU : U8_WAVES_PER_CU;
     if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
     L.grid = (uint32_t)std::min<uint64_t>(L.n, (uint64_t)c->num_cus * per_cu);

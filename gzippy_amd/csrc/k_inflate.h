@@ -26,8 +26,8 @@
 //
 // Two kernels: the u16-ring kernel runs every piece of a call that has
 // MARKER pieces (DIRECT and WINDOW pieces too, so that one launch holds the
-// whole call), the u8-ring kernel calls without them.  Both use 12 KB of LDS
-// per wavefront, 12 wavefronts per CU (three per SIMD).
+// whole call), the u8-ring kernel calls without them.  Both use 10 KB of LDS
+// per wavefront, 16 wavefronts per CU (four per SIMD).
 //
 // Slab layout of a MARKER piece (bytes): [u16 entries: 2*n16] (256-aligned)
 // [u8 entries: n8].
@@ -63,9 +63,10 @@ struct PieceOut {
 };
 
 // Ring of the most recent entries in LDS: 2048 u16 or 4096 u8 entries, 4 KiB
-// either way, so that with the LUTs a wavefront needs 12 KB and 12 share a
-// CU, three per SIMD (the kernel stays under 168 VGPRs), which hides the
-// decode loop's LDS, scalar-load and readlane latencies.
+// either way, so that with the LUTs a wavefront needs 10 KB and 16 share a
+// CU, four per SIMD (the kernel stays at 128 VGPRs: the dynamic-header
+// reader, whose table builds are the register peak, is not inlined), which
+// hides the decode loop's LDS, scalar-load and readlane latencies.
 // A copy from farther back (up to the 32768 of the window) reads the piece's
 // own output from HBM, or the window before the piece.
 template <typename RT>
@@ -77,10 +78,10 @@ template <typename RT>
 struct __align__(16) InflateShared {
     // Two-level Huffman LUTs: a root of LLR / DR bits, then subtables for
     // the codes longer than the root (zlib's inflate_table layout), in the
-    // same array after the root.  10/8-bit roots keep a wavefront's LDS at
-    // 16 KB.  A subtable that does not fit its reserve leaves its codes to
+    // same array after the root.  9/8-bit roots keep a wavefront's LDS at
+    // 10 KB.  A subtable that does not fit its reserve leaves its codes to
     // the canonical slow path.
-    static constexpr int LLR = 10;
+    static constexpr int LLR = 9;
     static constexpr int DR = 8;
     static constexpr int LL_SUB = 384;
     static constexpr int D_SUB = 64;
@@ -255,7 +256,7 @@ __device__ void build_fixed(SH &sh) {
 // read_literal_and_distance_code_lengths (:4447-4510).  Returns E_NONE or an
 // error code.
 template <typename SH>
-__device__ __attribute__((always_inline)) uint32_t read_dynamic(SH &sh, BitReader &br) {
+__device__ __attribute__((noinline)) uint32_t read_dynamic(SH &sh, BitReader &br) {
     const uint32_t lane = lane_id();
     br.refill();
     uint32_t hlit = br.read(5) + 257, hdist = br.read(5) + 1, hclen = br.read(4) + 4;
@@ -1036,7 +1037,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
 // longest-first.  RT = ring entry type: uint16_t for MARKER pieces, uint8_t
 // for all others.
 template <typename RT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
                                                 const PieceDesc *__restrict__ pieces,
                                                 const uint32_t *__restrict__ order, uint32_t n, uint32_t *queue,
                                                 PieceOut *out) {
