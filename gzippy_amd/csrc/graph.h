@@ -87,16 +87,20 @@ __global__ void k_graph_prep(const uint8_t *__restrict__ in, uint8_t *out, const
 }
 
 // Segment s of member seg_member[s] covers its bytes [seg_start[s],
-// seg_start[s] + 64 KiB), clipped to what the member decoded.
+// seg_start[s] + 64 KiB), clipped to what the member decoded; seg_tail is
+// what the member decoded after it.
 __global__ void k_graph_seglen(const PieceOut *__restrict__ pout, const uint32_t *__restrict__ seg_member,
                                const uint64_t *__restrict__ seg_start, uint32_t nseg, uint32_t n,
-                               uint32_t *__restrict__ seg_len, uint64_t *__restrict__ mlen) {
+                               uint32_t *__restrict__ seg_len, uint64_t *__restrict__ seg_tail,
+                               uint64_t *__restrict__ mlen) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nseg) {
         const PieceOut &r = pout[seg_member[i]];
         const uint64_t got = r.status == PS_FINAL ? r.n8 : 0;
         const uint64_t s = seg_start[i];
-        seg_len[i] = got > s ? (uint32_t)(got - s < CRC_SEG ? got - s : CRC_SEG) : 0u;
+        const uint32_t l = got > s ? (uint32_t)(got - s < CRC_SEG ? got - s : CRC_SEG) : 0u;
+        seg_len[i] = l;
+        seg_tail[i] = got > s + l ? got - s - l : 0;
     }
     if (i < n) mlen[i] = pout[i].status == PS_FINAL ? pout[i].n8 : 0;
 }
@@ -129,8 +133,8 @@ __global__ void k_graph_verify(const PieceOut *__restrict__ pout, const GraphHea
 struct gzgpu_batch_graph {
     gzgpu_ctx *c = nullptr;
     uint32_t n = 0, nseg = 0;
-    DevBuf members, pieces, order, pout, queue, head, seg_member, seg_start, seg_off, seg_len, mseg, mlen, seg_crc, mcrc,
-        status, olen, ocrc;
+    DevBuf members, pieces, order, pout, queue, head, seg_member, seg_start, seg_off, seg_len, seg_tail, mseg, mlen,
+        seg_crc, mcrc, status, olen, ocrc;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void release() {
@@ -139,7 +143,7 @@ struct gzgpu_batch_graph {
         exec = nullptr;
         graph = nullptr;
         for (DevBuf *b : {&members, &pieces, &order, &pout, &queue, &head, &seg_member, &seg_start, &seg_off, &seg_len,
-                          &mseg, &mlen, &seg_crc, &mcrc, &status, &olen, &ocrc})
+                          &seg_tail, &mseg, &mlen, &seg_crc, &mcrc, &status, &olen, &ocrc})
             b->release();
     }
 };
@@ -194,6 +198,7 @@ int gzgpu_batch_graph_create(gzgpu_ctx *c, const void *d_in, const uint64_t *in_
         g->queue.ensure(64);
         g->head.ensure(n * sizeof(GraphHead));
         g->seg_len.ensure(std::max<size_t>(g->nseg, 1) * 4);
+        g->seg_tail.ensure(std::max<size_t>(g->nseg, 1) * 8);
         g->seg_crc.ensure(std::max<size_t>(g->nseg, 1) * 4);
         g->mlen.ensure(n * 8);
         g->mcrc.ensure(n * 4);
@@ -218,14 +223,15 @@ int gzgpu_batch_graph_create(gzgpu_ctx *c, const void *d_in, const uint64_t *in_
                            g->pout.as<PieceOut>());
         hipLaunchKernelGGL(k_graph_seglen, dim3(sb), dim3(256), 0, c->stream, g->pout.as<PieceOut>(),
                            g->seg_member.as<uint32_t>(), g->seg_start.as<uint64_t>(), g->nseg, g->n,
-                           g->seg_len.as<uint32_t>(), g->mlen.as<uint64_t>());
+                           g->seg_len.as<uint32_t>(), g->seg_tail.as<uint64_t>(), g->mlen.as<uint64_t>());
         if (g->nseg)
             hipLaunchKernelGGL(k_crc_segments, dim3(std::min<uint32_t>(g->nseg, (uint32_t)c->num_cus * 8)), dim3(256), 0,
                                c->stream, (const uint8_t *)d_out, g->seg_off.as<uint64_t>(), g->seg_len.as<uint32_t>(),
-                               g->nseg, c->d_crc_tab.as<uint32_t>(), g->seg_crc.as<uint32_t>());
+                               g->seg_tail.as<uint64_t>(), g->nseg, c->d_crc_tab.as<uint32_t>(),
+                               g->seg_crc.as<uint32_t>());
         hipLaunchKernelGGL(k_crc_members, dim3(std::min<uint32_t>(g->n, (uint32_t)c->num_cus * 8)), dim3(256), 0,
-                           c->stream, g->seg_crc.as<uint32_t>(), g->seg_len.as<uint32_t>(), g->mseg.as<uint32_t>(),
-                           g->mlen.as<uint64_t>(), g->n, g->mcrc.as<uint32_t>());
+                           c->stream, g->seg_crc.as<uint32_t>(), g->mseg.as<uint32_t>(), g->mlen.as<uint64_t>(), g->n,
+                           g->mcrc.as<uint32_t>());
         hipLaunchKernelGGL(k_graph_verify, dim3(nb), dim3(256), 0, c->stream, g->pout.as<PieceOut>(),
                            g->head.as<GraphHead>(), g->mcrc.as<uint32_t>(), g->n, g->status.as<int32_t>(),
                            g->olen.as<uint64_t>(), g->ocrc.as<uint32_t>());

@@ -263,7 +263,7 @@ struct gzgpu_ctx {
     gzgpu_tuning tune{};
     std::string err;
     DevBuf d_in, d_out, d_pieces, d_pout, d_order, d_counters, d_surv, d_hits, d_slab, d_crc_tab, d_seg_off,
-        d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len, d_bf_ranges;
+        d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len, d_bf_ranges, d_seg_tail;
     std::vector<DevBuf> extra_slabs;
     PinnedArena pin;
     hipEvent_t ev[10]{};
@@ -887,7 +887,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
 
     // ---- CRC32 per member
-    std::vector<uint64_t> seg_off;
+    std::vector<uint64_t> seg_off, seg_tail;
     std::vector<uint32_t> seg_len, mseg;
     std::vector<uint64_t> mlen;
     std::vector<uint32_t> crc_members;
@@ -901,8 +901,10 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         mseg.push_back((uint32_t)seg_off.size());
         mlen.push_back(M.out_len);
         for (uint64_t o = 0; o < M.out_len; o += CRC_SEG) {
+            const uint64_t l = std::min<uint64_t>(CRC_SEG, M.out_len - o);
             seg_off.push_back((uint64_t)(M.out - base) + o);
-            seg_len.push_back((uint32_t)std::min<uint64_t>(CRC_SEG, M.out_len - o));
+            seg_len.push_back((uint32_t)l);
+            seg_tail.push_back(M.out_len - o - l);
         }
     }
     if (!crc_members.empty()) {
@@ -917,9 +919,11 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         if (nseg) {
             upload(c, c->d_seg_off, seg_off.data(), seg_off.size() * 8);
             upload(c, c->d_seg_len, seg_len.data(), seg_len.size() * 4);
+            upload(c, c->d_seg_tail, seg_tail.data(), seg_tail.size() * 8);
             hipLaunchKernelGGL(k_crc_segments, dim3(std::min<uint32_t>(nseg, (uint32_t)c->num_cus * 8)), dim3(256), 0,
-                               c->stream, base, c->d_seg_off.as<uint64_t>(), c->d_seg_len.as<uint32_t>(), nseg,
-                               c->d_crc_tab.as<uint32_t>(), c->d_seg_crc.as<uint32_t>());
+                               c->stream, base, c->d_seg_off.as<uint64_t>(), c->d_seg_len.as<uint32_t>(),
+                               c->d_seg_tail.as<uint64_t>(), nseg, c->d_crc_tab.as<uint32_t>(),
+                               c->d_seg_crc.as<uint32_t>());
             HIPCHK(hipGetLastError());
             c->st.launches++;
         }
@@ -928,8 +932,8 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         upload(c, c->d_mlen, mlen.data(), mlen.size() * 8);
         c->d_mcrc.ensure(nm * 4);
         hipLaunchKernelGGL(k_crc_members, dim3(std::min<uint32_t>(nm, (uint32_t)c->num_cus * 8)), dim3(256), 0, c->stream,
-                           c->d_seg_crc.as<uint32_t>(), c->d_seg_len.as<uint32_t>(), c->d_mseg.as<uint32_t>(),
-                           c->d_mlen.as<uint64_t>(), nm, c->d_mcrc.as<uint32_t>());
+                           c->d_seg_crc.as<uint32_t>(), c->d_mseg.as<uint32_t>(), c->d_mlen.as<uint64_t>(), nm,
+                           c->d_mcrc.as<uint32_t>());
         HIPCHK(hipGetLastError());
         c->st.launches++;
         const uint32_t *crcs = download<uint32_t>(c, c->d_mcrc.p, nm);
@@ -1143,7 +1147,7 @@ void gzgpu_ctx_destroy(gzgpu_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf *b : {&c->d_in, &c->d_out, &c->d_pieces, &c->d_pout, &c->d_order, &c->d_counters, &c->d_surv, &c->d_hits,
                       &c->d_slab, &c->d_crc_tab, &c->d_seg_off, &c->d_seg_len, &c->d_seg_crc, &c->d_mseg, &c->d_mlen,
-                      &c->d_mcrc, &c->d_rp, &c->d_tiles, &c->d_chains, &c->d_env, &c->d_env_off, &c->d_env_len, &c->d_bf_ranges})
+                      &c->d_mcrc, &c->d_rp, &c->d_tiles, &c->d_chains, &c->d_env, &c->d_env_off, &c->d_env_len, &c->d_bf_ranges, &c->d_seg_tail})
         b->release();
     for (auto &b : c->extra_slabs) b.release();
     c->pin.release();
@@ -1416,21 +1420,24 @@ int gzgpu_crc32(gzgpu_ctx *c, const void *d_data, size_t len, uint32_t *crc) {
             c->d_crc_tab.ensure(sizeof(cc.tab));
             HIPCHK(hipMemcpy(c->d_crc_tab.p, cc.tab, sizeof(cc.tab), hipMemcpyHostToDevice));
         }
-        std::vector<uint64_t> so;
+        std::vector<uint64_t> so, st;
         std::vector<uint32_t> sl;
         for (uint64_t o = 0; o < len; o += CRC_SEG) {
+            const uint64_t l = std::min<uint64_t>(CRC_SEG, len - o);
             so.push_back(o);
-            sl.push_back((uint32_t)std::min<uint64_t>(CRC_SEG, len - o));
+            sl.push_back((uint32_t)l);
+            st.push_back(len - o - l);
         }
         uint32_t nseg = (uint32_t)so.size();
         c->d_seg_crc.ensure(std::max<size_t>(nseg, 1) * 4);
         if (nseg) {
             upload(c, c->d_seg_off, so.data(), so.size() * 8);
             upload(c, c->d_seg_len, sl.data(), sl.size() * 4);
+            upload(c, c->d_seg_tail, st.data(), st.size() * 8);
             hipLaunchKernelGGL(k_crc_segments, dim3(std::min<uint32_t>(nseg, (uint32_t)c->num_cus * 8)), dim3(256), 0,
                                c->stream, (const uint8_t *)d_data, c->d_seg_off.as<uint64_t>(),
-                               c->d_seg_len.as<uint32_t>(), nseg, c->d_crc_tab.as<uint32_t>(),
-                               c->d_seg_crc.as<uint32_t>());
+                               c->d_seg_len.as<uint32_t>(), c->d_seg_tail.as<uint64_t>(), nseg,
+                               c->d_crc_tab.as<uint32_t>(), c->d_seg_crc.as<uint32_t>());
             HIPCHK(hipGetLastError());
         }
         uint32_t ms[2] = {0, nseg};
@@ -1439,8 +1446,7 @@ int gzgpu_crc32(gzgpu_ctx *c, const void *d_data, size_t len, uint32_t *crc) {
         upload(c, c->d_mlen, &ml, 8);
         c->d_mcrc.ensure(4);
         hipLaunchKernelGGL(k_crc_members, dim3(1), dim3(256), 0, c->stream, c->d_seg_crc.as<uint32_t>(),
-                           c->d_seg_len.as<uint32_t>(), c->d_mseg.as<uint32_t>(), c->d_mlen.as<uint64_t>(), 1u,
-                           c->d_mcrc.as<uint32_t>());
+                           c->d_mseg.as<uint32_t>(), c->d_mlen.as<uint64_t>(), 1u, c->d_mcrc.as<uint32_t>());
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(crc, c->d_mcrc.p, 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));

@@ -8,8 +8,10 @@
 // lane l's CRC is multiplied by the constant x^(8 * 256 * (255 - l)) and the
 // 256 products are XORed.  A segment shorter than 64 KiB is placed at the end
 // of the 64 KiB frame behind zeros, which a raw CRC register ignores.
-// Stage 2 (k_crc_members): per member, fold its segment CRCs the same way
-// and apply the init/final inversion: crc = ~(x^(8D) * 0xFFFFFFFF ^ raw).
+// Each segment's CRC leaves stage 1 already shifted over the member bytes
+// after it (seg_tail), so stage 2 (k_crc_members) only XORs a member's
+// segment values and applies the init/final inversion:
+// crc = ~(x^(8D) * 0xFFFFFFFF ^ raw).
 #pragma once
 #include "gz_common.h"
 
@@ -57,7 +59,8 @@ __device__ __forceinline__ uint32_t crc_shift(uint32_t c, uint64_t len) { return
 // segment is staged with 16-byte loads; any other with dword loads realigned
 // by v_alignbit (only dwords that hold segment bytes are read).
 __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict__ base, const uint64_t *__restrict__ seg_off,
-                                                      const uint32_t *__restrict__ seg_len, uint32_t nseg,
+                                                      const uint32_t *__restrict__ seg_len,
+                                                      const uint64_t *__restrict__ seg_tail, uint32_t nseg,
                                                       const uint32_t *__restrict__ tables, uint32_t *__restrict__ seg_crc) {
     __shared__ uint32_t t[8][256];
     __shared__ uint32_t buf[256 * 65];  // frame row r (bytes [256 r, 256 r + 256)) at dword 65 r (conflict-free reads)
@@ -116,47 +119,25 @@ __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict_
         }
         if (c) atomicXor(&red, c);
         __syncthreads();
-        if (threadIdx.x == 0) seg_crc[s] = red;
+        if (threadIdx.x == 0) seg_crc[s] = crc_shift(red, seg_tail[s]);
         __syncthreads();
     }
 }
 
-// Per member m: segments [mseg[m], mseg[m+1]) of lengths seg_len; total
-// bytes mlen[m].  One workgroup (256 lanes) per member.
+// Per member m: segments [mseg[m], mseg[m+1]), total bytes mlen[m].  One
+// workgroup per member.
 __global__ void __launch_bounds__(256) k_crc_members(const uint32_t *__restrict__ seg_crc,
-                                                     const uint32_t *__restrict__ seg_len,
                                                      const uint32_t *__restrict__ mseg, const uint64_t *__restrict__ mlen,
                                                      uint32_t nmem, uint32_t *__restrict__ crc_out) {
-    __shared__ uint32_t part[256];
-    __shared__ uint64_t plen[256];
+    __shared__ uint32_t red;
     for (uint32_t m = blockIdx.x; m < nmem; m += gridDim.x) {
-        const uint32_t s0 = mseg[m], s1 = mseg[m + 1];
-        const uint32_t ns = s1 - s0;
-        const uint32_t per = (ns + 255) / 256;
-        uint32_t a = s0 + threadIdx.x * per, b = a + per;
-        if (a > s1) a = s1;
-        if (b > s1) b = s1;
-        uint32_t c = 0;
-        uint64_t l = 0;
-        for (uint32_t s = a; s < b; s++) {
-            c = crc_shift(c, seg_len[s]) ^ seg_crc[s];
-            l += seg_len[s];
-        }
-        part[threadIdx.x] = c;
-        plen[threadIdx.x] = l;
+        if (threadIdx.x == 0) red = 0;
         __syncthreads();
-        for (uint32_t w = 1; w < 256; w <<= 1) {
-            uint32_t i = threadIdx.x;
-            if ((i & (2 * w - 1)) == 0 && i + w < 256) {
-                uint64_t rl = plen[i + w];
-                if (rl) {
-                    part[i] = crc_shift(part[i], rl) ^ part[i + w];
-                    plen[i] += rl;
-                }
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) crc_out[m] = ~(crc_shift(0xFFFFFFFFu, mlen[m]) ^ part[0]);
+        uint32_t c = 0;
+        for (uint32_t s = mseg[m] + threadIdx.x; s < mseg[m + 1]; s += blockDim.x) c ^= seg_crc[s];
+        if (c) atomicXor(&red, c);
+        __syncthreads();
+        if (threadIdx.x == 0) crc_out[m] = ~(crc_shift(0xFFFFFFFFu, mlen[m]) ^ red);
         __syncthreads();
     }
 }
