@@ -331,32 +331,30 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
                            c->d_bf_ranges.as<BfRange>(), (uint32_t)table.size(), nchunk, c->d_surv.as<uint64_t>(),
                            surv_cap, cnt + 0, c->d_hits.as<gzgpu_boundary>(), hit_cap, cnt + 1);
         HIPCHK(hipGetLastError());
-        uint32_t *h_cnt = download<uint32_t>(c, cnt, 2);
-        HIPCHK(hipStreamSynchronize(c->stream));
         c->st.launches++;
-        if (h_cnt[0] > surv_cap) {
-            surv_cap = h_cnt[0] + 1024;
+        // stage B reads the survivor count on the device; one read-back of
+        // both counters and the first hits (pinned), the rest only if needed
+        const uint32_t bb = std::min<uint32_t>(surv_cap / 256 + 1, (uint32_t)c->num_cus * 16);
+        hipLaunchKernelGGL(k_bf_stage_b, dim3(bb), dim3(256), 0, c->stream, d_in, in_len, c->d_surv.as<uint64_t>(),
+                           cnt + 0, surv_cap, c->d_hits.as<gzgpu_boundary>(), hit_cap, cnt + 1);
+        HIPCHK(hipGetLastError());
+        c->st.launches++;
+        const uint32_t first = std::min<uint32_t>(hit_cap, 8192);
+        const uint32_t *h_cnt = download<uint32_t>(c, cnt, 2);
+        const gzgpu_boundary *h_hits = download<gzgpu_boundary>(c, c->d_hits.p, first);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (h_cnt[0] > surv_cap || h_cnt[1] > hit_cap) {
+            if (h_cnt[0] > surv_cap) surv_cap = h_cnt[0] + 1024;
+            if (h_cnt[1] > hit_cap) hit_cap = h_cnt[1] + 1024;
             continue;
         }
-        uint32_t nsurv = h_cnt[0];
-        if (nsurv) {
-            uint32_t bb = std::min<uint32_t>((nsurv + 255) / 256, (uint32_t)c->num_cus * 16);
-            hipLaunchKernelGGL(k_bf_stage_b, dim3(bb), dim3(256), 0, c->stream, d_in, in_len, c->d_surv.as<uint64_t>(),
-                               nsurv, c->d_hits.as<gzgpu_boundary>(), hit_cap, cnt + 1);
-            HIPCHK(hipGetLastError());
-            c->st.launches++;
+        const uint32_t nh = h_cnt[1];
+        out.assign(h_hits, h_hits + std::min(nh, first));
+        if (nh > first) {
+            const gzgpu_boundary *rest = download<gzgpu_boundary>(c, c->d_hits.as<gzgpu_boundary>() + first, nh - first);
+            HIPCHK(hipStreamSynchronize(c->stream));
+            out.insert(out.end(), rest, rest + (nh - first));
         }
-        h_cnt = download<uint32_t>(c, cnt, 2);
-        HIPCHK(hipStreamSynchronize(c->stream));
-        if (h_cnt[1] > hit_cap) {
-            hit_cap = h_cnt[1] + 1024;
-            continue;
-        }
-        out.resize(h_cnt[1]);
-        if (h_cnt[1])
-            HIPCHK(hipMemcpyAsync(out.data(), c->d_hits.p, out.size() * sizeof(gzgpu_boundary), hipMemcpyDeviceToHost,
-                                  c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
         auto lt = [](const gzgpu_boundary &a, const gzgpu_boundary &b) {
             return a.bit_offset != b.bit_offset ? a.bit_offset < b.bit_offset : a.seek_bit < b.seek_bit;
         };
@@ -391,6 +389,13 @@ constexpr double PIECES_PER_SLOT = 2.0;
 
 uint64_t marker_slab_bytes(uint64_t expect_entries) { return round_up(2 * (expect_entries + 64 * KIB) + 512, 256); }
 
+double host_ms() {
+    return (double)std::chrono::duration_cast<std::chrono::microseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+               .count() /
+           1e3;
+}
+
 bool debug_enabled() {
     static int v = -1;
     if (v < 0) {
@@ -412,12 +417,14 @@ struct InflateLaunch {
 template <typename RT>
 InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, const std::vector<PieceDesc> &descs,
                               const std::vector<uint64_t> &cost) {
+    const double t0 = debug_enabled() ? host_ms() : 0;
     InflateLaunch L;
     L.n = (uint32_t)descs.size();
     L.u16 = sizeof(RT) == 2;
     std::vector<uint32_t> order(L.n);
     for (uint32_t i = 0; i < L.n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    std::sort(order.begin(), order.end(),
+              [&](uint32_t a, uint32_t b) { return cost[a] != cost[b] ? cost[a] > cost[b] : a < b; });
     upload(c, c->d_pieces, descs.data(), descs.size() * sizeof(PieceDesc));
     upload(c, c->d_order, order.data(), order.size() * 4);
     c->d_pout.ensure(descs.size() * sizeof(PieceOut));
@@ -427,6 +434,7 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     uint32_t per_cu = L.u16 ? U16_WAVES_PER_CU : U8_WAVES_PER_CU;
     if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
     L.grid = (uint32_t)std::min<uint64_t>(L.n, (uint64_t)c->num_cus * per_cu);
+    if (debug_enabled()) fprintf(stderr, "[gzgpu] host: inflate setup %.3f ms\n", host_ms() - t0);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     hipLaunchKernelGGL(k_inflate<RT>, dim3(L.grid), dim3(64), 0, c->stream, d_in, in_len, c->d_pieces.as<PieceDesc>(),
                        c->d_order.as<uint32_t>(), L.n, c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
@@ -580,7 +588,9 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     std::vector<std::pair<uint64_t, uint64_t>> ranges;
     for (auto &M : mem)
         if (splits(M)) ranges.push_back({M.ds + 1, M.de});
+    const double th0 = debug_enabled() ? host_ms() : 0;
     const std::vector<gzgpu_boundary> all_hits = find_hits(c, d_in, in_len, ranges);
+    const double th1 = debug_enabled() ? host_ms() : 0;
     c->st.candidates += (uint32_t)all_hits.size();
     auto hit_lt = [](const gzgpu_boundary &a, uint64_t bit) { return a.bit_offset < bit; };
     std::vector<PieceHost> P;
@@ -676,6 +686,9 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         }
     }
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    if (debug_enabled())
+        fprintf(stderr, "[gzgpu] host: find %.3f ms (%zu hits), plan %.3f ms (%zu pieces)\n", th1 - th0, all_hits.size(),
+                host_ms() - th1, P.size());
 
     // slab pool for the first launch
     uint64_t pool = 0;

@@ -362,12 +362,16 @@ __device__ bool validate_dynamic(const uint8_t *__restrict__ in, uint64_t n, uin
     return ok_ll && ok_d;
 }
 
+// Survivor count from stage A's counter (capped at the buffer), so that the
+// host need not read it between the stages.
 __global__ void __launch_bounds__(256) k_bf_stage_b(const uint8_t *__restrict__ in, uint64_t n,
-                                                    const uint64_t *__restrict__ surv, uint32_t nsurv,
-                                                    gzgpu_boundary *__restrict__ hits, uint32_t hit_cap,
-                                                    uint32_t *hit_count) {
+                                                    const uint64_t *__restrict__ surv, const uint32_t *surv_count,
+                                                    uint32_t surv_cap, gzgpu_boundary *__restrict__ hits,
+                                                    uint32_t hit_cap, uint32_t *hit_count) {
     __shared__ __align__(16) uint8_t luts[256][128];
     const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t counted = (uint32_t)__builtin_amdgcn_readfirstlane((int)*surv_count);
+    const uint32_t nsurv = counted < surv_cap ? counted : surv_cap;
     for (uint32_t base = blockIdx.x * blockDim.x; base < nsurv; base += stride) {
         uint32_t i = base + threadIdx.x;
         bool ok = false;
