@@ -31,7 +31,7 @@ enum : uint32_t {
 // error sub-codes (PieceOut::err)
 enum : uint32_t {
     E_NONE = 0, E_BTYPE3, E_TRUNC, E_STORED_LEN, E_HDR_COUNTS, E_PRECODE, E_CODELEN, E_EOB_ZERO,
-    E_LL_TABLE, E_D_TABLE, E_BAD_SYM, E_BAD_DIST_SYM, E_DIST_FAR,
+    E_LL_TABLE, E_D_TABLE, E_BAD_SYM, E_BAD_DIST_SYM, E_DIST_FAR, E_SYNC,
 };
 
 // LUT entry (u32): [0,4) code length (0 = resolve on the slow path),

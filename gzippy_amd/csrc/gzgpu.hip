@@ -381,11 +381,37 @@ uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 constexpr uint32_t U8_WAVES_PER_CU = 16, U16_WAVES_PER_CU = 16;
 // planner: smallest piece (compressed KiB; the reference's block finder
 // keeps 32 KiB, gzip_block_finder.rs:109) and pieces per resident wavefront.
-// Pieces start at block boundaries, so their sizes vary with the blocks';
-// two per wavefront, pulled longest first, even the load out (measured on a
-// 1 GiB text member: 8.75 -> 8.4 ms; a 10k-member batch: 52 -> 37 ms).
-constexpr uint64_t MIN_SPACING_KIB = 8;
-constexpr double PIECES_PER_SLOT = 2.0;
+// With starts inside blocks a member's pieces come out equal, so one launch
+// round with one piece per resident wavefront (less a 1/32 margin: a piece
+// more than the slots waits a whole piece time) beats several: every piece
+// also pays a marker (u16) phase of its own, ~160k entries on text at half
+// the speed of u8 decoding.  A member whose cost per compressed bit is high
+// (member_cost) may go down to MIN_SPACING_FLOOR_KIB so that no piece costs
+// much more than MIN_PIECE_OUT.
+constexpr uint64_t MIN_SPACING_KIB = 2, MIN_SPACING_FLOOR_KIB = 1;
+constexpr uint64_t MIN_PIECE_OUT = 64 * 1024;
+constexpr double PIECES_PER_SLOT = 1.0, SLOT_FRACTION = 31.0 / 32.0;
+// a start inside a block: at least MID_HEAD bits after the block's header
+// hit (a dynamic header is rarely longer; a start inside it fails its sync
+// and costs a re-decode) and MID_TAIL bits before the next hit
+constexpr uint64_t MID_HEAD = 2048, MID_TAIL = 2048;
+
+// Decode cost of a member in output-entry units.  A wavefront's decode time
+// is about max(40 ns per compressed bit, 11 ns per output entry) at full
+// occupancy: a decode round covers at most 64 input bits, and a round's
+// copies take time in proportion to their bytes.  Measured per piece: text
+// (91k bits, 300k entries) 3.3 ms, long-copy data (247k bits, 1.77M entries)
+// 19.6 ms, u32 deltas (707k bits, 263k entries) 31 ms.  So a Huffman-coded
+// member costs max(4 x bits, ISIZE).  Stored data (ratio below 1.15, the
+// reference's StoredParallel cut, decompress/mod.rs:143-151) moves at copy
+// speed, ~830 MB/s for a lone wavefront against ~230 MB/s of entries from
+// long copies, and is weighted 1/4 of its bytes.  ISIZE is the size mod 2^32; a
+// member whose true size wrapped looks stored-like here, which only changes
+// how it is split.
+uint64_t member_cost(uint64_t cbits, uint32_t exp_isize) {
+    const bool stored_like = (uint64_t)exp_isize * 100 < (cbits / 8) * 115;
+    return std::max<uint64_t>(1, stored_like ? cbits / 32 : std::max<uint64_t>(4 * cbits, exp_isize));
+}
 
 uint64_t marker_slab_bytes(uint64_t expect_entries) { return round_up(2 * (expect_entries + 64 * KIB) + 512, 256); }
 
@@ -396,11 +422,33 @@ double host_ms() {
            1e3;
 }
 
-bool debug_enabled() {
+// GZGPU_DEBUG=1: per-launch timing lines on stderr; 2: also the planner's
+// per-member split decisions
+int debug_level() {
     static int v = -1;
     if (v < 0) {
         const char *e = getenv("GZGPU_DEBUG");
-        v = e && *e && *e != '0';
+        v = e && *e ? atoi(e) : 0;
+    }
+    return v;
+}
+bool debug_enabled() { return debug_level() >= 1; }
+
+double env_double(const char *name, double dflt) {
+    const char *e = getenv(name);
+    return e && *e ? atof(e) : dflt;
+}
+uint64_t env_u64(const char *name, uint64_t dflt) {
+    const char *e = getenv(name);
+    return e && *e ? strtoull(e, nullptr, 10) : dflt;
+}
+
+// Starts inside blocks (MIDBLOCK pieces); GZGPU_MIDBLOCK=0 turns them off.
+bool midblock_enabled() {
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("GZGPU_MIDBLOCK");
+        v = !(e && *e == '0');
     }
     return v == 1;
 }
@@ -489,11 +537,15 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
     auto stop_of = [&](const PieceDesc &d) { return d.stop_bit == NO_STOP ? mem[d.member].de : d.stop_bit; };
     if (todo.empty()) return;
     std::vector<PieceDesc> d;
-    std::vector<uint64_t> cost;
+    std::vector<uint64_t> cost;  // the member's cost per compressed bit x the piece's bits
     bool any_marker = false;
     for (uint32_t pi : todo) {
         d.push_back(P[pi].d);
-        cost.push_back(stop_of(P[pi].d) - P[pi].d.start_bit);
+        const Member &M = mem[P[pi].d.member];
+        const uint64_t mbits = std::max<uint64_t>(1, M.de - M.ds);
+        const double per_bit = (double)member_cost(mbits, M.exp_isize) / (double)mbits;
+        const uint64_t bits = stop_of(P[pi].d) - P[pi].d.start_bit;
+        cost.push_back((uint64_t)(per_bit * 256.0 * (double)bits));
         any_marker |= (P[pi].d.flags & PF_MARKER) != 0;
     }
     float ms = 0;
@@ -551,32 +603,34 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
 
 void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
     const uint64_t chunk_bits = (c->tune.chunk_bytes ? c->tune.chunk_bytes : 512 * KIB) * 8;
-    const uint64_t min_spacing = 8 * MIN_SPACING_KIB * KIB;
+    const uint64_t min_spacing = 8 * env_u64("GZGPU_MIN_SPACING_KIB", MIN_SPACING_KIB) * KIB;
+    auto min_spacing_of = [&](const Member &M) -> uint64_t {
+        const uint64_t cbits = M.de - M.ds;
+        const double bits = (double)env_u64("GZGPU_MIN_PIECE", MIN_PIECE_OUT) * (double)cbits /
+                            (double)member_cost(cbits, M.exp_isize);
+        return std::min<uint64_t>(min_spacing, std::max<uint64_t>(8 * MIN_SPACING_FLOOR_KIB * KIB, (uint64_t)bits));
+    };
     const uint64_t slots = (uint64_t)c->num_cus *  // resident u8-ring inflate wavefronts
                            (c->tune.waves_per_cu ? std::min<uint32_t>(c->tune.waves_per_cu, U8_WAVES_PER_CU)
                                                  : U8_WAVES_PER_CU);
     // Piece grid.  Explicit pieces_per_chunk: chunk_bits / ppc compressed bits.
-    // Default: balance decode time.  A wavefront decodes Huffman-coded input
-    // at a roughly constant rate of compressed bits, whatever the ratio;
-    // stored data (ratio below 1.15, the reference's StoredParallel cut,
-    // decompress/mod.rs:143-151) moves at copy speed, weighted 1/10.  Every
-    // piece aims at one resident slot's share of the call's total cost.
-    auto cost_of = [](const Member &M) -> uint64_t {
-        const uint64_t cbits = M.de - M.ds;
-        const bool stored_like = (uint64_t)M.exp_isize * 100 < (cbits / 8) * 115;  // ISIZE mod 2^32: low only if so
-        return stored_like ? cbits / 10 : cbits;
-    };
+    // Default: balance decode time (member_cost); every piece aims at one
+    // resident slot's share of the call's total cost, and a member's pieces
+    // split its compressed bits evenly.
+    auto cost_of = [](const Member &M) -> uint64_t { return member_cost(M.de - M.ds, M.exp_isize); };
     uint64_t total_cost = 0;
     for (auto &M : mem)
         if (M.status == GZGPU_OK) total_cost += cost_of(M);
-    const uint64_t target_cost =
-        std::max<uint64_t>(min_spacing, (uint64_t)((double)total_cost / (PIECES_PER_SLOT * (double)(slots - slots / 16))));
+    const double pps = env_double("GZGPU_PPS", PIECES_PER_SLOT);
+    const double slot_frac = env_double("GZGPU_SLOT_FRAC", SLOT_FRACTION);
+    const uint64_t target_cost = std::max<uint64_t>(
+        env_u64("GZGPU_MIN_PIECE", MIN_PIECE_OUT), (uint64_t)((double)total_cost / (pps * slot_frac * (double)slots)));
     const uint64_t ppc = c->tune.pieces_per_chunk;
     auto spacing_of = [&](const Member &M) -> uint64_t {
         if (ppc) return std::max<uint64_t>(chunk_bits / ppc, min_spacing);
         const uint64_t npieces = (cost_of(M) + target_cost - 1) / target_cost;
         if (npieces <= 1) return ~0ull;
-        return std::max<uint64_t>((M.de - M.ds) / npieces, min_spacing);
+        return std::max<uint64_t>((M.de - M.ds) / npieces, min_spacing_of(M));
     };
 
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -617,9 +671,13 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         const std::vector<gzgpu_boundary> hits(
             std::lower_bound(all_hits.begin(), all_hits.end(), M.ds + 1, hit_lt),
             std::lower_bound(all_hits.begin(), all_hits.end(), M.de, hit_lt));
-        // piece starts: per grid point the hit nearest to it within half a
-        // spacing either way (pieces come out near equal, and the last one
-        // to finish sets the launch time).  Preference: a dynamic-block hit;
+        // piece starts: per grid point a hit within an eighth of a spacing
+        // of it, else a start inside the block that holds the grid point
+        // (MIDBLOCK: the piece reads that block's header, at the last hit
+        // before the point, and starts at the point's sync point), else the
+        // hit nearest to it within half a spacing either way (pieces come out
+        // near equal, and the last one to finish sets the launch time).
+        // Preference among hits: a dynamic-block hit;
         // else a stored hit whose payload ends where another hit's accepted
         // range begins (a final stored block's range starts one bit late, past
         // its BFINAL bit) or at the member's end.  That rules out LEN/~NLEN
@@ -633,7 +691,12 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         const uint64_t rest = span / np;
         size_t h = 0;
         uint64_t last = M.ds;
-        std::vector<size_t> chosen;
+        struct Start {
+            uint64_t start, acc_lo, acc_hi, hdr;
+            bool mid;
+        };
+        std::vector<Start> chosen;
+        auto after = [](uint64_t v, const gzgpu_boundary &a) { return v < a.bit_offset; };
         auto chained = [&](size_t i) {
             const uint64_t o = hits[i].seek_bit + 3;  // the LEN field
             const uint64_t next = (o / 8 + 4 + (hits[i].info >> 16)) * 8;
@@ -658,28 +721,62 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                         best = j;
                     }
                 }
+            if (midblock_enabled() && (best == hits.size() || bestd > rest / 8)) {
+                // the block holding g starts at the last hit at or before g
+                // (the member's first block when there is none); a start
+                // needs MID_HEAD bits after that header hit, for the header
+                // itself, and MID_TAIL bits before the next hit.  Stored hits
+                // that are not chained are LEN/NLEN coincidences and skipped.
+                auto real = [&](size_t j) { return hits[j].kind == 2 || chained(j); };
+                size_t k = std::upper_bound(hits.begin(), hits.end(), g, after) - hits.begin();
+                size_t prev = k;
+                while (prev > 0 && !real(prev - 1)) prev--;
+                while (k < hits.size() && !real(k)) k++;
+                const uint64_t hdr = prev ? hits[prev - 1].seek_bit : M.ds;
+                const bool dyn = prev == 0 || hits[prev - 1].kind == 2;
+                const uint64_t next = k < hits.size() ? hits[k].bit_offset : M.de;
+                if (dyn && g >= hdr + MID_HEAD && g >= last + MID_HEAD && next >= g + MID_TAIL) {
+                    chosen.push_back({g, g, g, hdr, true});
+                    last = g;
+                    continue;
+                }
+            }
             if (best == hits.size()) continue;  // no boundary near this grid point
             if (hits[best].seek_bit >= M.de) break;
-            chosen.push_back(best);
+            chosen.push_back({hits[best].seek_bit, hits[best].bit_offset, hits[best].seek_bit, 0, false});
             last = hits[best].bit_offset;
             h = best + 1;
         }
         const uint64_t total_bits = M.de - M.ds;
+        if (debug_level() >= 2)
+            fprintf(stderr,
+                    "[gzgpu] plan member %u: bits %llu isize %u cost %llu target %llu spacing %llu grid %llu hits %zu "
+                    "chosen %zu\n",
+                    m, (unsigned long long)span, M.exp_isize, (unsigned long long)cost_of(M),
+                    (unsigned long long)target_cost, (unsigned long long)spacing, (unsigned long long)np, hits.size(),
+                    chosen.size());
         for (size_t k = 0; k < chosen.size(); k++) {
-            const gzgpu_boundary &b = hits[chosen[k]];
-            P[M.pieces.back()].d.stop_bit = b.bit_offset;
+            const Start &b = chosen[k];
+            PieceDesc &prev = P[M.pieces.back()].d;
+            prev.stop_bit = b.acc_lo;
+            if (b.mid) {
+                prev.flags |= PF_STOPMID;
+                prev.stop_hdr = b.hdr;
+            }
             PieceHost q;
-            q.d.start_bit = b.seek_bit;
+            q.d.start_bit = b.start;
             q.d.stop_bit = NO_STOP;
             q.d.limit_bit = M.limit;
             q.d.window = nullptr;
-            q.d.flags = PF_MARKER;
+            q.d.flags = PF_MARKER | (b.mid ? PF_MIDBLOCK : 0u);
+            q.d.hdr_bit = b.mid ? b.hdr : 0;
             q.d.member = m;
             q.member = m;
-            q.acc_lo = b.bit_offset;
-            q.acc_hi = b.seek_bit;
-            uint64_t end = k + 1 < chosen.size() ? hits[chosen[k + 1]].bit_offset : M.de;
-            double frac = (double)(end - b.bit_offset) / (double)total_bits;
+            c->st.midblock_pieces += b.mid ? 1u : 0u;
+            q.acc_lo = b.acc_lo;  // MIDBLOCK: the sync point, known after the launch
+            q.acc_hi = b.acc_hi;
+            uint64_t end = k + 1 < chosen.size() ? chosen[k + 1].acc_lo : M.de;
+            double frac = (double)(end - b.acc_lo) / (double)total_bits;
             q.d.cap = marker_slab_bytes((uint64_t)(1.3 * frac * (double)M.exp_isize));
             M.pieces.push_back((uint32_t)P.size());
             P.push_back(q);
@@ -731,6 +828,13 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 uint32_t pi = M.pieces[k];
                 PieceHost &p = P[pi];
                 if (p.dropped) break;
+                if (p.d.flags & PF_MIDBLOCK) {
+                    // the start is the sync point the piece found; none: unlinkable
+                    const PieceOut &me = R[pi];
+                    const bool synced = !(me.status == PS_ERROR && me.err == E_SYNC);
+                    p.acc_lo = synced ? me.start_bit : 1;
+                    p.acc_hi = synced ? me.start_bit : 0;
+                }
                 const PieceOut &pr = R[M.pieces[k - 1]];
                 if (pr.status == PS_FINAL) {
                     if (linked[M.pieces[k - 1]])
@@ -744,6 +848,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                     continue;
                 }
                 if (pr.end_bit >= p.acc_lo && pr.end_bit <= p.acc_hi) {
+                    if ((p.d.flags & PF_MIDBLOCK) && !linked[pi]) c->st.midblock_linked++;
                     linked[pi] = 1;
                 } else {
                     linked[pi] = 0;
@@ -753,6 +858,8 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                                     (unsigned long long)p.acc_lo, (unsigned long long)p.acc_hi,
                                     (unsigned long long)pr.end_bit);
                         p.d.start_bit = p.acc_lo = p.acc_hi = pr.end_bit;
+                        p.d.flags &= ~PF_MIDBLOCK;
+                        p.d.hdr_bit = 0;
                         todo.push_back(pi);
                     }
                 }

@@ -36,7 +36,12 @@
 
 namespace gz {
 
-enum : uint32_t { PF_DIRECT = 1, PF_MARKER = 2, PF_WINDOW = 4 };
+// MIDBLOCK: the piece starts inside the block whose header is at hdr_bit, at
+// the sync point of start_bit (sync_point).  STOPMID: the piece stops at the
+// sync point of stop_bit inside the block whose header is at stop_hdr, when
+// its decode lands there exactly; else at the first non-final block header
+// at or after stop_bit, as without the flag.
+enum : uint32_t { PF_DIRECT = 1, PF_MARKER = 2, PF_WINDOW = 4, PF_MIDBLOCK = 8, PF_STOPMID = 16 };
 
 struct PieceDesc {
     uint64_t start_bit;      // first block header to decode
@@ -47,9 +52,12 @@ struct PieceDesc {
     const uint8_t *window;   // WINDOW: 32768 bytes, oldest first
     uint32_t flags;
     uint32_t member;
+    uint64_t hdr_bit;        // MIDBLOCK: header of the block holding start_bit
+    uint64_t stop_hdr;       // STOPMID: header of the block holding stop_bit
 };
 
 struct PieceOut {
+    uint64_t start_bit;  // first symbol decoded (MIDBLOCK: the sync point), else start_bit
     uint64_t end_bit;
     uint64_t n16;       // u16 marker-mode entries
     uint64_t n8;        // u8 entries (DIRECT/WINDOW, or after the flip)
@@ -363,6 +371,29 @@ __device__ __forceinline__ void copy_chunk_out(RT *ring, const OutSeg &o, uint32
     }
 }
 
+// Where entries more than a ring back are read from (see InflateShared).
+struct FarSrc {
+    uint64_t out16, out8, n16, win;
+    uint32_t kind;
+};
+template <typename SH>
+__device__ __forceinline__ FarSrc far_src(SH &sh) {
+    return {uni64(sh.far_out16), uni64(sh.far_out8), uni64(sh.far_n16), uni64(sh.far_win), uni(sh.far_win_kind)};
+}
+// The entry at piece position r (negative: the window before the piece).
+// Entries more than 32768 before the piece do not exist (a distance the
+// window cannot serve): they read as 0 without a memory access.
+template <typename RT>
+__device__ __forceinline__ uint32_t far_fetch(const FarSrc &f, int64_t r) {
+    if (r >= 0) {
+        if (sizeof(RT) == 2 && (uint64_t)r < f.n16) return to_global((const uint16_t *)f.out16)[r];
+        return to_global((const uint8_t *)f.out8)[(uint64_t)r - f.n16];
+    }
+    if (r >= -(int64_t)WIN && f.kind == 1) return to_global((const uint8_t *)f.win)[WIN + r];
+    if (r >= -(int64_t)WIN && f.kind == 3) return MARKER_BASE + (uint32_t)(WIN + r);
+    return 0u;
+}
+
 template <typename RT, typename SH>
 __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o, uint32_t pos, uint32_t g0,
                                                          uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
@@ -374,30 +405,14 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
         // 8192 > len): the piece's output in HBM, written by this wavefront
         // at least a ring earlier, or the window before the piece.  A
         // wavefront's vector memory operations reach the CU's L1 in order
-        // and the loads' wait covers its earlier stores.  Entries more than
-        // 32768 before the piece do not exist (a distance the window cannot
-        // serve): they read as 0 without a memory access.
-        const uint64_t out16 = uni64(sh.far_out16), out8 = uni64(sh.far_out8), n16 = uni64(sh.far_n16);
-        const uint64_t win = uni64(sh.far_win);
-        const uint32_t wk = uni(sh.far_win_kind);
+        // and the loads' wait covers its earlier stores.
+        const FarSrc f = far_src(sh);
         const int64_t rel = (int64_t)o.seg0 + (int64_t)pos - (int64_t)dist;  // piece position of entry 0
         uint32_t v[5];
 #pragma unroll
         for (uint32_t k = 0; k < 5; k++) {
             const uint32_t i = lane + WAVE * k;
-            const int64_t r = rel + (int64_t)i;
-            uint32_t x = 0;
-            if (i < len) {
-                if (r >= 0) {
-                    if (sizeof(RT) == 2 && (uint64_t)r < n16) x = to_global((const uint16_t *)out16)[r];
-                    else x = to_global((const uint8_t *)out8)[(uint64_t)r - n16];
-                } else if (r >= -(int64_t)WIN && wk == 1) {
-                    x = to_global((const uint8_t *)win)[WIN + r];
-                } else if (r >= -(int64_t)WIN && wk == 3) {
-                    x = MARKER_BASE + (uint32_t)(WIN + r);
-                }
-            }
-            v[k] = x;
+            v[k] = i < len ? far_fetch<RT>(f, rel + (int64_t)i) : 0u;
         }
         compiler_fence();
 #pragma unroll
@@ -455,7 +470,84 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
     compiler_fence();
 }
 
-enum : uint32_t { R_EOB = 0, R_FLIP = 1, R_OVERFLOW = 2, R_ERROR = 3, R_CONT = 4 };
+// A round whose output fits one wavefront (run <= 64 entries) and holds
+// copies, in one pass: lane j produces entry j.  Each taken symbol leaves a
+// descriptor (literal value, or copy distance) at its first output slot in
+// LDS scratch; a ballot of the slots gives every lane its symbol.  A copy
+// entry's source j - dist inside the round is chased by pointer jumping
+// over ds_bpermute (each step doubles the hop, at most six); a source before
+// the round is read from the ring or, more than a ring back, from HBM.  All
+// reads precede the round's one ring write and one store, so no cut rule
+// applies (sources inside the round never reach the ring or HBM).  Scratch:
+// the code-length array, which only the header reader uses.
+template <typename RT, typename SH>
+__device__ __attribute__((always_inline)) void fused_round(SH &sh, const OutSeg &o, uint32_t pos, uint32_t g0,
+                                                           uint32_t run, bool taken, bool is_lit, uint32_t opos,
+                                                           uint32_t e, uint32_t dist, uint32_t olen,
+                                                           uint64_t &last_marker_end) {
+    static_assert(sizeof(sh.lens) >= 4 * WAVE, "scratch");
+    const uint32_t lane = lane_id();
+    uint32_t *scr = reinterpret_cast<uint32_t *>(sh.lens);
+    compiler_fence();
+    constexpr uint32_t VALID = 0x80000000u, LIT = 0x40000000u;
+    // lanes exchange values through scr: the fences keep the compiler from
+    // treating each lane's store and load as its own (forwarding the 0)
+    scr[lane] = 0u;
+    compiler_fence();
+    if (taken && olen != 0) scr[opos] = VALID | (is_lit ? LIT | ((e >> 16) & 0xFFu) : dist);
+    compiler_fence();
+    const uint32_t d0 = scr[lane];
+    const uint64_t S = ballot(d0 != 0u);  // first output slots of the round's symbols
+    const uint64_t below = S & (lane == WAVE - 1 ? ~0ull : (2ull << lane) - 1);
+    const uint32_t st = below ? 63u - (uint32_t)__builtin_clzll(below) : 0u;
+    const uint32_t d = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(st << 2), (int)d0);
+    const bool live = lane < run;
+    uint32_t val = 0, ptr = 0;
+    bool res = true;
+    bool far = false;
+    int32_t src = 0;
+    if (!(d & LIT)) {
+        src = (int32_t)lane - (int32_t)(d & 0xFFFFu);
+        if (src >= 0) {
+            res = false;
+            ptr = (uint32_t)src;
+        } else if ((uint32_t)(-src) <= RING<RT>) {
+            val = sh.ring[(g0 + pos + (uint32_t)src) & RMASK<RT>];
+        } else {
+            far = live;
+        }
+    } else {
+        val = d & 0xFFu;
+    }
+    if (ballot(far)) {
+        const FarSrc f = far_src(sh);
+        if (far) val = far_fetch<RT>(f, (int64_t)o.seg0 + (int64_t)pos + (int64_t)src);
+    }
+    for (uint64_t pend = ballot(!res && live); pend; pend = ballot(!res && live)) {
+        const uint32_t w = res ? (VALID | val) : ptr;
+        const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(ptr << 2), (int)w);
+        if (!res) {
+            if (x & VALID) {
+                val = x & 0xFFFFu;
+                res = true;
+            } else {
+                ptr = x;
+            }
+        }
+    }
+    compiler_fence();
+    if (live) {
+        sh.ring[(g0 + pos + lane) & RMASK<RT>] = (RT)val;
+        put<RT>(o, pos + lane, val);
+    }
+    if (is_wide<RT>(o)) {
+        const uint64_t m = ballot(live && val >= 256);
+        if (m) last_marker_end = o.seg0 + pos + 64 - (uint64_t)__clzll(m);
+    }
+    compiler_fence();
+}
+
+enum : uint32_t { R_EOB = 0, R_FLIP = 1, R_OVERFLOW = 2, R_ERROR = 3, R_CONT = 4, R_STOPMID = 5 };
 
 // 128 bits of input starting at absolute bit P, as two wave-uniform words.
 // Reads six 32-bit words through the scalar data cache (SMEM loads, counted
@@ -620,6 +712,58 @@ __device__ __attribute__((always_inline)) void lookahead(SH &sh, const uint8_t *
     la.step = nx | (go_on ? 0u : 0x100u);
 }
 
+// ---- a symbol boundary inside a block, from any bit ---------------------
+constexpr uint64_t NO_SYNC = ~0ull;
+constexpr uint32_t SYNC_WINDOWS = 64;  // give up after 4096 bits
+
+// The first bit S >= X at which every decode of the current block's codes
+// started at X, X+1, ..., X+47 has a symbol boundary, or NO_SYNC.  A symbol
+// (code plus extra bits, a length with its distance) spans at most 48 bits,
+// so the true symbol sequence has a boundary in [X, X+48) and passes
+// through S too.  Paths are followed as a set of pending positions over
+// 64-bit lookahead windows; a path at an invalid code or at end-of-block is
+// dropped (a start is planned well before the block's end).  The result
+// depends only on the block's tables and
+// the input, so a piece stopping at X and the piece starting at X find the
+// same S; the stopping piece only stops there if its own decode lands on S,
+// which is what makes a start at S exact.  Huffman codes resynchronise
+// within a few symbols, so S is usually within 100 bits of X.  X below the
+// block's first symbol bit `first` is refused.
+template <typename SH>
+__device__ __attribute__((noinline)) uint64_t sync_point(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
+                                                         uint64_t X, uint64_t first, uint64_t limit_bit) {
+    X = uni64(X);
+    if (X < uni64(first)) return NO_SYNC;
+    uint64_t P = X;
+    uint64_t A = (1ull << 48) - 1;  // pending path positions, relative to P
+    for (uint32_t w = 0; w < SYNC_WINDOWS; w++) {
+        if (P + 128 > uni64(limit_bit)) return NO_SYNC;
+        Lookahead la;
+        lookahead(sh, in, n, P, la);
+        const uint64_t CONT = uni64(la.CONT);
+        uint64_t C = 0;  // pending positions in the next window
+        while (A) {
+            A = uni64(A);
+            C = uni64(C);
+            const uint32_t i = uni((uint32_t)__builtin_ctzll(A));
+            if (C == 0 && (A & (A - 1)) == 0) return P + i;  // every path is at i
+            A &= A - 1;
+            // end-of-block or an invalid code: the path stops here.  A
+            // misaligned path meets the end-of-block code now and then;
+            // the true one only where the block ends, and then the
+            // stopping piece does not land on S
+            if (!((CONT >> i) & 1)) continue;
+            const uint32_t t = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)i));
+            if (t < 64) A |= 1ull << t;
+            else C |= 1ull << (t - 64);
+        }
+        A = uni64(C);
+        if (A == 0) return NO_SYNC;
+        P += 64;
+    }
+    return NO_SYNC;
+}
+
 // Decode the Huffman symbols of the current block until end-of-block, from
 // bit P.  Multi-symbol rounds: lane j looks up the litlen (and, for a length
 // code, the distance) LUT entry of a symbol starting at bit P + j, all 64 in
@@ -635,7 +779,8 @@ __device__ __attribute__((always_inline)) void lookahead(SH &sh, const uint8_t *
 template <typename RT, typename SH>
 __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
                                                                   uint64_t &P, OutSeg &o, uint64_t limit_bit,
-                                                                  uint64_t &last_marker_end, uint32_t &err) {
+                                                                  uint64_t &last_marker_end, uint32_t &err,
+                                                                  uint64_t &stop_at) {
     const uint32_t lane = lane_id();
     // every control value is re-asserted wave-uniform (readfirstlane): the
     // compiler's divergence analysis cannot prove it for values that came
@@ -660,6 +805,14 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
         P = uni64(P);
         pos = uni(pos);
         last_marker_end = uni64(last_marker_end);
+        stop_at = uni64(stop_at);
+        if (P >= stop_at) {
+            if (P == stop_at) {
+                r = R_STOPMID;
+                break;
+            }
+            stop_at = NO_SYNC;  // the decode went past it (single steps)
+        }
         if (P > limit_bit) {
             err = E_TRUNC;
             r = R_ERROR;
@@ -684,6 +837,19 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             eob = 1;
         }
         M = uni64(M);
+        // ---- a mid-block stop inside this round: keep the symbols before it
+        // when one of them ends exactly there, else the stream does not pass
+        // through it and it is dropped
+        if (stop_at - P < (uint64_t)s) {
+            const uint32_t rel = uni((uint32_t)(stop_at - P));
+            if (rel < 64 && ((M >> rel) & 1)) {
+                M = uni64(M & ((1ull << rel) - 1));
+                s = rel;
+                eob = 0;
+            } else {
+                stop_at = NO_SYNC;
+            }
+        }
         // ---- output offsets of the taken symbols
         const uint32_t v = ((M >> lane) & 1) ? la.olen : 0u;
         const uint32_t incl = wave_scan_incl(v);
@@ -746,20 +912,25 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             const uint32_t c_e = la.e, c_dist = la.dist, c_olen = la.olen;
             P += s;
             if (!eob) lookahead(sh, in, n, P, la);
-            // ---- literals of the round: one LDS write and one HBM store
-            if ((lits >> lane) & 1) {
-                const uint32_t q = pos + opos;
-                sh.ring[(g0 + q) & RMASK<RT>] = (RT)(c_e >> 16);
-                put<RT>(o, q, c_e >> 16);
-            }
-            compiler_fence();
-            // ---- copies, in stream order
-            for (uint64_t mm = matches; mm; mm = uni64(mm & (mm - 1))) {
-                const uint32_t mi = uni((uint32_t)__builtin_ctzll(mm));
-                const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)opos, (int)mi);
-                const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)c_dist, (int)mi);
-                const uint32_t ln = (uint32_t)__builtin_amdgcn_readlane((int)c_olen, (int)mi);
-                wave_copy<RT>(sh, o, pos + m, g0, ln, d, last_marker_end);
+            if (run <= WAVE && matches) {
+                fused_round<RT>(sh, o, pos, g0, run, (M >> lane) & 1, (lits >> lane) & 1, opos, c_e, c_dist, c_olen,
+                                last_marker_end);
+            } else {
+                // ---- literals of the round: one LDS write and one HBM store
+                if ((lits >> lane) & 1) {
+                    const uint32_t q = pos + opos;
+                    sh.ring[(g0 + q) & RMASK<RT>] = (RT)(c_e >> 16);
+                    put<RT>(o, q, c_e >> 16);
+                }
+                compiler_fence();
+                // ---- copies, in stream order
+                for (uint64_t mm = matches; mm; mm = uni64(mm & (mm - 1))) {
+                    const uint32_t mi = uni((uint32_t)__builtin_ctzll(mm));
+                    const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)opos, (int)mi);
+                    const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)c_dist, (int)mi);
+                    const uint32_t ln = (uint32_t)__builtin_amdgcn_readlane((int)c_olen, (int)mi);
+                    wave_copy<RT>(sh, o, pos + m, g0, ln, d, last_marker_end);
+                }
             }
             pos += run;
             if (eob) {
@@ -888,8 +1059,13 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
     }
     __syncthreads();
 
+    // A MIDBLOCK piece reads its block's header first and starts at the sync
+    // point of start_bit; stop_at is the exact stop inside the block at
+    // stop_hdr (STOPMID), once that block's tables are built.
+    bool mid = (pd.flags & PF_MIDBLOCK) != 0;
+    uint64_t start_bit = pd.start_bit, stop_at = NO_SYNC;
     BitReader br;
-    br.init(in, in_len, pd.start_bit);
+    br.init(in, in_len, mid ? pd.hdr_bit : pd.start_bit);
     for (;;) {
         br.make_uniform();
         br.refill();
@@ -915,6 +1091,13 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
         }
         nblocks++;
         uint32_t r = R_EOB;
+        stop_at = NO_SYNC;
+        if (btype == 0 && mid) {
+            status = PS_ERROR;  // no symbols to synchronise on
+            err = E_SYNC;
+            end_bit = pd.start_bit;
+            break;
+        }
         if (btype == 0) {
             uint32_t pad = (uint32_t)((8 - (br.pos() & 7)) & 7);
             br.consume(pad);
@@ -961,8 +1144,29 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
                 }
             }
             uint64_t P = br.pos();
-            r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err);
+            const uint64_t first = P;
+            if (mid) {
+                mid = false;
+                P = sync_point(sh, in, in_len, pd.start_bit, first, pd.limit_bit);
+                if (P == NO_SYNC) {
+                    status = PS_ERROR;
+                    err = E_SYNC;
+                    end_bit = pd.start_bit;
+                    break;
+                }
+                start_bit = P;
+            }
+            if ((pd.flags & PF_STOPMID) && hbit == pd.stop_hdr) {
+                stop_at = sync_point(sh, in, in_len, pd.stop_bit, first, pd.limit_bit);
+                if (stop_at != NO_SYNC && stop_at <= P) stop_at = NO_SYNC;
+            }
+            r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err, stop_at);
             br.init(in, in_len, P);
+        }
+        if (r == R_STOPMID) {
+            status = PS_STOPPED;
+            end_bit = br.pos();
+            break;
         }
         if (r == R_FLIP) {
             // the ring already holds the last entries, all literal: continue
@@ -985,12 +1189,17 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
                 // a flip inside a block resumes that block's symbols
                 if (btype != 0) {
                     uint64_t P = br.pos();
-                    r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err);
+                    r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err, stop_at);
                     br.init(in, in_len, P);
                 } else {
                     r = R_EOB;
                 }
             }
+        }
+        if (r == R_STOPMID) {
+            status = PS_STOPPED;
+            end_bit = br.pos();
+            break;
         }
         if (r == R_OVERFLOW) {
             status = PS_OVERFLOW;
@@ -1017,12 +1226,13 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
     __syncthreads();
     if (lane == 0) {
         const bool flipped = marker && o.wide == 0;
+        po->start_bit = start_bit;
         po->end_bit = end_bit;
         const uint64_t total = o.seg0 + o.pos;
         po->n16 = marker ? (flipped ? n16 : total) : 0;
         po->n8 = marker ? (flipped ? total - n16 : 0) : total;
         po->u8_off = flipped ? u8_off : 0;
-        po->bits_done = br.pos() - pd.start_bit;
+        po->bits_done = br.pos() > start_bit ? br.pos() - start_bit : 0;
         po->t_begin = t_begin;
         po->t_end = wall_clock64();
         po->status = status;

@@ -119,17 +119,17 @@ def trailer(gz):
     return crc, isize
 
 
-# ---------------------------------------------------------------- configs
+# ---------------------------------------------------------------- workloads
 
 def text_member(n, seed, level=6):
-    """Config-0/1 shape: `n` bytes of the text generator, one member at `level`."""
+    """Single-member shape: `n` bytes of the text generator, one member at `level`."""
     raw = generate("text", n, seed)
     return raw, gzip_compress(raw, level)
 
 
 def batch_members(count, seed, min_size=1 << 10, max_size=64 << 20, median=64 << 10, sigma=1.5,
                   levels=(1, 6, 9), threads=8):
-    """Config-3 shape: `count` independent members with lognormal sizes, content
+    """Batch shape: `count` independent members with lognormal sizes, content
     uniform over the four kinds and level uniform over `levels`.
     Returns list of (raw, gz, kind, level)."""
     rng = np.random.default_rng(seed)

@@ -82,6 +82,8 @@ typedef struct {
     uint64_t clean_out_bytes;   /* bytes written by the u8-ring kernel */
     uint64_t marker_in_bytes;   /* compressed bytes consumed by the u16-ring kernel */
     uint64_t marker_out_bytes;  /* bytes written by the u16-ring kernel (u16 entries count 2) */
+    uint32_t midblock_pieces;   /* pieces planned to start inside a block (at a sync point) */
+    uint32_t midblock_linked;   /* of those, linked to their predecessor's exact stop */
 } gzgpu_stats;
 
 /* ---- context ---------------------------------------------------------- */

@@ -30,13 +30,36 @@ def fine_grid(gpu):
     ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
 
 
-def test_config0_text_16mib(gpu):
+def test_text_member_16mib(gpu):
     raw, gz = W.text_member(16 << 20, seed=0x675A0000)
-    say("config0", raw.size, gz.size)
+    say("text member", raw.size, gz.size)
     st = {}
     out = gpu.decompress(gz, stats=st)
     assert out == raw.tobytes()
     say("stats", st)
+
+
+@pytest.mark.parametrize("kind,level", [("text", 6), ("text", 1), ("u32", 1), ("runs", 9)])
+def test_midblock_starts(gpu, kind, level):
+    """Pieces that start inside a block (at the sync point of a grid bit) and
+    predecessors that stop there exactly: the output stays byte-exact and
+    nearly every such start links on the first launch."""
+    ctx = gpu.context()
+    old = ctx.tuning()
+    try:
+        # 2 KiB compressed per piece: most grid points fall inside blocks
+        ctx.set_tuning(chunk_bytes=64 << 10, pieces_per_chunk=32)
+        raw = W.generate(kind, 6 << 20, seed=11 * level + len(kind))
+        gz = W.gzip_compress(raw, level)
+        st = {}
+        out = gpu.decompress(gz, stats=st)
+        say("midblock", kind, level, gz.size, {k: st[k] for k in ("pieces", "midblock_pieces", "midblock_linked",
+                                                                  "redecoded")})
+        assert out == raw.tobytes()
+        assert st["midblock_pieces"] > 0
+        assert st["midblock_linked"] >= 0.9 * st["midblock_pieces"]
+    finally:
+        ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
 
 
 @pytest.mark.parametrize("kind", ["text", "random", "runs", "u32"])

@@ -45,7 +45,7 @@ def test_gunzip_matches_generator(oracle, kind, level):
         assert zlib.decompress(bytes(gz), 31) == raw.tobytes()
 
 
-def test_gunzip_config0_shape(oracle):
+def test_gunzip_text_member_shape(oracle):
     raw, gz = W.text_member(16 << 20, seed=0x675A0000)
     st, out, _ = oracle.gunzip(gz)
     assert st == oracle.OK and np.array_equal(out, raw)
