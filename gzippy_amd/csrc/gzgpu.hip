@@ -262,6 +262,7 @@ struct gzgpu_ctx {
     int num_cus = 256;
     gzgpu_tuning tune{};
     std::string err;
+    DevBuf d_lists;  // resolve and CRC lists of the current call, one upload
     DevBuf d_in, d_out, d_pieces, d_pout, d_order, d_counters, d_surv, d_hits, d_slab, d_crc_tab, d_seg_off,
         d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len, d_bf_ranges, d_seg_tail;
     std::vector<DevBuf> extra_slabs;
@@ -299,6 +300,54 @@ T *download(gzgpu_ctx *c, const void *d, size_t count) {
     T *h = (T *)c->pin.alloc(count * sizeof(T));
     if (count) HIPCHK(hipMemcpyAsync(h, d, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
     return h;
+}
+
+double host_ms() {
+    return (double)std::chrono::duration_cast<std::chrono::microseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+               .count() /
+           1e3;
+}
+
+// GZGPU_DEBUG=1: per-launch timing lines on stderr; 2: also the planner's
+// per-member split decisions
+int debug_level() {
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("GZGPU_DEBUG");
+        v = e && *e ? atoi(e) : 0;
+    }
+    return v;
+}
+bool debug_enabled() { return debug_level() >= 1; }
+
+double env_double(const char *name, double dflt) {
+    const char *e = getenv(name);
+    return e && *e ? atof(e) : dflt;
+}
+uint64_t env_u64(const char *name, uint64_t dflt) {
+    const char *e = getenv(name);
+    return e && *e ? strtoull(e, nullptr, 10) : dflt;
+}
+
+// Indices of `keys` in ascending key order, equal keys by index (LSD radix,
+// 11-bit digits over the bits the largest key uses): the host sorts a few
+// thousand keys per call, where std::sort cost 0.1 ms.
+std::vector<uint32_t> radix_order(const std::vector<uint64_t> &keys) {
+    const size_t n = keys.size();
+    std::vector<uint32_t> a(n), b(n);
+    for (size_t i = 0; i < n; i++) a[i] = (uint32_t)i;
+    uint64_t mx = 0;
+    for (uint64_t k : keys) mx |= k;
+    uint32_t cnt[2049];
+    for (uint32_t sh = 0; sh < 64 && (mx >> sh); sh += 11) {
+        memset(cnt, 0, sizeof(cnt));
+        for (size_t i = 0; i < n; i++) cnt[((keys[a[i]] >> sh) & 2047) + 1]++;
+        for (uint32_t d = 0; d < 2048; d++) cnt[d + 1] += cnt[d];
+        for (size_t i = 0; i < n; i++) b[cnt[(keys[a[i]] >> sh) & 2047]++] = a[i];
+        a.swap(b);
+    }
+    return a;
 }
 
 // ---- block finder over ranges [start_bit, end_bit) of d_in ----------------
@@ -348,6 +397,7 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
             if (h_cnt[1] > hit_cap) hit_cap = h_cnt[1] + 1024;
             continue;
         }
+        const double ts = debug_enabled() ? host_ms() : 0;
         const uint32_t nh = h_cnt[1];
         out.assign(h_hits, h_hits + std::min(nh, first));
         if (nh > first) {
@@ -355,16 +405,31 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
             HIPCHK(hipStreamSynchronize(c->stream));
             out.insert(out.end(), rest, rest + (nh - first));
         }
-        auto lt = [](const gzgpu_boundary &a, const gzgpu_boundary &b) {
-            return a.bit_offset != b.bit_offset ? a.bit_offset < b.bit_offset : a.seek_bit < b.seek_bit;
-        };
-        std::sort(out.begin(), out.end(), lt);
+        // by (bit_offset, seek_bit): radix on bit_offset, then the rare runs
+        // of equal bit_offset by seek_bit
+        {
+            std::vector<uint64_t> keys(out.size());
+            for (size_t i = 0; i < out.size(); i++) keys[i] = out[i].bit_offset;
+            const std::vector<uint32_t> ord = radix_order(keys);
+            std::vector<gzgpu_boundary> sorted(out.size());
+            for (size_t i = 0; i < ord.size(); i++) sorted[i] = out[ord[i]];
+            out.swap(sorted);
+            for (size_t i = 0; i < out.size();) {
+                size_t j = i + 1;
+                while (j < out.size() && out[j].bit_offset == out[i].bit_offset) j++;
+                if (j - i > 1)
+                    std::sort(out.begin() + i, out.begin() + j,
+                              [](const gzgpu_boundary &x, const gzgpu_boundary &y) { return x.seek_bit < y.seek_bit; });
+                i = j;
+            }
+        }
         if (table.size() > 1)
             out.erase(std::unique(out.begin(), out.end(),
                                   [](const gzgpu_boundary &a, const gzgpu_boundary &b) {
                                       return a.bit_offset == b.bit_offset && a.seek_bit == b.seek_bit;
                                   }),
                       out.end());
+        if (debug_enabled()) fprintf(stderr, "[gzgpu] host: hits read-back + sort %.3f ms\n", host_ms() - ts);
         return out;
     }
     throw StatusError{GZGPU_OUT_OF_DEVICE_MEMORY, "block finder: candidate buffers kept overflowing"};
@@ -415,34 +480,6 @@ uint64_t member_cost(uint64_t cbits, uint32_t exp_isize) {
 
 uint64_t marker_slab_bytes(uint64_t expect_entries) { return round_up(2 * (expect_entries + 64 * KIB) + 512, 256); }
 
-double host_ms() {
-    return (double)std::chrono::duration_cast<std::chrono::microseconds>(
-               std::chrono::steady_clock::now().time_since_epoch())
-               .count() /
-           1e3;
-}
-
-// GZGPU_DEBUG=1: per-launch timing lines on stderr; 2: also the planner's
-// per-member split decisions
-int debug_level() {
-    static int v = -1;
-    if (v < 0) {
-        const char *e = getenv("GZGPU_DEBUG");
-        v = e && *e ? atoi(e) : 0;
-    }
-    return v;
-}
-bool debug_enabled() { return debug_level() >= 1; }
-
-double env_double(const char *name, double dflt) {
-    const char *e = getenv(name);
-    return e && *e ? atof(e) : dflt;
-}
-uint64_t env_u64(const char *name, uint64_t dflt) {
-    const char *e = getenv(name);
-    return e && *e ? strtoull(e, nullptr, 10) : dflt;
-}
-
 // Starts inside blocks (MIDBLOCK pieces); GZGPU_MIDBLOCK=0 turns them off.
 bool midblock_enabled() {
     static int v = -1;
@@ -469,12 +506,21 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     InflateLaunch L;
     L.n = (uint32_t)descs.size();
     L.u16 = sizeof(RT) == 2;
-    std::vector<uint32_t> order(L.n);
-    for (uint32_t i = 0; i < L.n; i++) order[i] = i;
-    std::sort(order.begin(), order.end(),
-              [&](uint32_t a, uint32_t b) { return cost[a] != cost[b] ? cost[a] > cost[b] : a < b; });
+    // longest first: radix order of the complement of each cost's top 16
+    // significant bits (a 6-bit exponent and 10-bit mantissa; LPT needs no
+    // finer order), equal keys by index
+    std::vector<uint64_t> keys(L.n);
+    for (uint32_t i = 0; i < L.n; i++) {
+        const uint64_t v = cost[i];
+        const uint32_t e = v ? 63u - (uint32_t)__builtin_clzll(v) : 0u;
+        const uint64_t q = e < 10 ? v : ((uint64_t)(e - 9) << 10) | ((v >> (e - 10)) & 1023);
+        keys[i] = 0xFFFFu - (q & 0xFFFFu);
+    }
+    const std::vector<uint32_t> order = radix_order(keys);
+    const double t1 = debug_enabled() ? host_ms() : 0;
     upload(c, c->d_pieces, descs.data(), descs.size() * sizeof(PieceDesc));
     upload(c, c->d_order, order.data(), order.size() * 4);
+    if (debug_enabled()) fprintf(stderr, "[gzgpu] host: sort %.3f ms upload %.3f ms\n", t1 - t0, host_ms() - t1);
     c->d_pout.ensure(descs.size() * sizeof(PieceOut));
     c->d_counters.ensure(64);
     HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
@@ -514,6 +560,12 @@ std::vector<PieceOut> finish_inflate(gzgpu_ctx *c, const InflateLaunch &L, const
         size_t slow = 0;
         for (size_t i = 1; i < ro.size(); i++)
             if (ro[i].t_end - ro[i].t_begin > ro[slow].t_end - ro[slow].t_begin) slow = i;
+        if (debug_level() >= 3)
+            for (size_t i = 0; i < ro.size(); i++)
+                fprintf(stderr, "[gzgpu] piece %zu n16 %llu n8 %llu bits %llu us %.1f begin %.1f flags %u\n", i,
+                        (unsigned long long)ro[i].n16, (unsigned long long)ro[i].n8,
+                        (unsigned long long)ro[i].bits_done, (double)(ro[i].t_end - ro[i].t_begin) / 100.0,
+                        (double)(ro[i].t_begin - t0) / 100.0, descs[i].flags);
         std::sort(us.begin(), us.end());
         fprintf(stderr,
                 "[gzgpu] inflate<%s> pieces=%u grid=%u kernel=%.3f ms t0=%.1f us span=%.3f ms piece_us min=%.1f p50=%.1f "
@@ -536,6 +588,7 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
                     const std::vector<Member> &mem, std::vector<uint32_t> &todo, std::vector<PieceOut> &R) {
     auto stop_of = [&](const PieceDesc &d) { return d.stop_bit == NO_STOP ? mem[d.member].de : d.stop_bit; };
     if (todo.empty()) return;
+    const double tl = debug_enabled() ? host_ms() : 0;
     std::vector<PieceDesc> d;
     std::vector<uint64_t> cost;  // the member's cost per compressed bit x the piece's bits
     bool any_marker = false;
@@ -549,6 +602,7 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
         any_marker |= (P[pi].d.flags & PF_MARKER) != 0;
     }
     float ms = 0;
+    if (debug_enabled()) fprintf(stderr, "[gzgpu] host: launch lists %.3f ms\n", host_ms() - tl);
     std::vector<PieceOut> r = any_marker ? finish_inflate(c, enqueue_inflate<uint16_t>(c, d_in, in_len, d, cost), d, ms)
                                          : finish_inflate(c, enqueue_inflate<uint8_t>(c, d_in, in_len, d, cost), d, ms);
     uint64_t in_bytes = 0, out_bytes = 0;
@@ -648,6 +702,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     c->st.candidates += (uint32_t)all_hits.size();
     auto hit_lt = [](const gzgpu_boundary &a, uint64_t bit) { return a.bit_offset < bit; };
     std::vector<PieceHost> P;
+    P.reserve(mem.size() + (size_t)std::min<uint64_t>(total_cost / target_cost, 1u << 24) + 16);
     for (uint32_t m = 0; m < mem.size(); m++) {
         Member &M = mem[m];
         if (M.status != GZGPU_OK) continue;
@@ -668,9 +723,14 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         P.push_back(p0);
         if (!splits(M)) continue;
         const uint64_t spacing = spacing_of(M);
-        const std::vector<gzgpu_boundary> hits(
-            std::lower_bound(all_hits.begin(), all_hits.end(), M.ds + 1, hit_lt),
-            std::lower_bound(all_hits.begin(), all_hits.end(), M.de, hit_lt));
+        struct {
+            const gzgpu_boundary *b, *e;
+            size_t size() const { return (size_t)(e - b); }
+            const gzgpu_boundary &operator[](size_t i) const { return b[i]; }
+            const gzgpu_boundary *begin() const { return b; }
+            const gzgpu_boundary *end() const { return e; }
+        } hits{all_hits.data() + (std::lower_bound(all_hits.begin(), all_hits.end(), M.ds + 1, hit_lt) - all_hits.begin()),
+               all_hits.data() + (std::lower_bound(all_hits.begin(), all_hits.end(), M.de, hit_lt) - all_hits.begin())};
         // piece starts: per grid point a hit within an eighth of a spacing
         // of it, else a start inside the block that holds the grid point
         // (MIDBLOCK: the piece reads that block's header, at the last hit
@@ -787,6 +847,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         fprintf(stderr, "[gzgpu] host: find %.3f ms (%zu hits), plan %.3f ms (%zu pieces)\n", th1 - th0, all_hits.size(),
                 host_ms() - th1, P.size());
 
+    const double tslab = debug_enabled() ? host_ms() : 0;
     // slab pool for the first launch
     uint64_t pool = 0;
     for (auto &p : P)
@@ -801,6 +862,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             }
     }
 
+    if (debug_enabled()) fprintf(stderr, "[gzgpu] host: slabs %.3f ms\n", host_ms() - tslab);
     std::vector<PieceOut> R(P.size());
     std::vector<uint32_t> todo(P.size());
     for (uint32_t i = 0; i < P.size(); i++) todo[i] = i;
@@ -984,29 +1046,9 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         }
         M.out_len = pos;
     }
-    if (!rps.empty()) {
-        chains.push_back((uint32_t)rps.size());
-        upload(c, c->d_rp, rps.data(), rps.size() * sizeof(ResolvePiece));
-        upload(c, c->d_chains, chains.data(), chains.size() * 4);
-        uint32_t nch = (uint32_t)chains.size() - 1;
-        HIPCHK(hipEventRecord(c->ev[7], c->stream));
-        hipLaunchKernelGGL(k_chain, dim3(std::min<uint32_t>(nch, (uint32_t)c->num_cus * 4)), dim3(1024), 0, c->stream,
-                           c->d_rp.as<ResolvePiece>(), c->d_chains.as<uint32_t>(), nch);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(c->ev[8], c->stream));
-        c->st.launches++;
-        if (!tiles.empty()) {
-            upload(c, c->d_tiles, tiles.data(), tiles.size() * sizeof(ResolveTile));
-            hipLaunchKernelGGL(k_resolve_body, dim3((uint32_t)std::min<size_t>(tiles.size(), (size_t)c->num_cus * 32)),
-                               dim3(256), 0, c->stream, c->d_rp.as<ResolvePiece>(), c->d_tiles.as<ResolveTile>(),
-                               (uint32_t)tiles.size());
-            HIPCHK(hipGetLastError());
-            c->st.launches++;
-        }
-    }
-    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    if (!rps.empty()) chains.push_back((uint32_t)rps.size());
 
-    // ---- CRC32 per member
+    // ---- CRC32 lists (every member's output size is known now)
     std::vector<uint64_t> seg_off, seg_tail;
     std::vector<uint32_t> seg_len, mseg;
     std::vector<uint64_t> mlen;
@@ -1027,32 +1069,69 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             seg_tail.push_back(M.out_len - o - l);
         }
     }
+    if (!crc_members.empty()) mseg.push_back((uint32_t)seg_off.size());
+    const uint32_t nseg = (uint32_t)seg_off.size(), nm = (uint32_t)crc_members.size();
+
+    // ---- every resolve and CRC list in one staging buffer and one copy:
+    // small copies between the kernels each left the GPU idle for a while
+    struct Part {
+        const void *src;
+        size_t bytes, off;
+    };
+    Part parts[] = {{rps.data(), rps.size() * sizeof(ResolvePiece), 0}, {chains.data(), chains.size() * 4, 0},
+                    {tiles.data(), tiles.size() * sizeof(ResolveTile), 0}, {seg_off.data(), seg_off.size() * 8, 0},
+                    {seg_len.data(), seg_len.size() * 4, 0}, {seg_tail.data(), seg_tail.size() * 8, 0},
+                    {mseg.data(), mseg.size() * 4, 0}, {mlen.data(), mlen.size() * 8, 0}};
+    size_t lists_bytes = 0;
+    for (auto &pt : parts) {
+        pt.off = lists_bytes;
+        lists_bytes += (pt.bytes + 255) & ~(size_t)255;
+    }
+    if (lists_bytes) {
+        c->d_lists.ensure(lists_bytes);
+        uint8_t *h = (uint8_t *)c->pin.alloc(lists_bytes);
+        for (auto &pt : parts)
+            if (pt.bytes) memcpy(h + pt.off, pt.src, pt.bytes);
+        HIPCHK(hipMemcpyAsync(c->d_lists.p, h, lists_bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    auto part = [&](int i) { return (void *)(c->d_lists.as<uint8_t>() + parts[i].off); };
+    if (!rps.empty()) {
+        uint32_t nch = (uint32_t)chains.size() - 1;
+        HIPCHK(hipEventRecord(c->ev[7], c->stream));
+        hipLaunchKernelGGL(k_chain, dim3(std::min<uint32_t>(nch, (uint32_t)c->num_cus * 4)), dim3(1024), 0, c->stream,
+                           (const ResolvePiece *)part(0), (const uint32_t *)part(1), nch);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[8], c->stream));
+        c->st.launches++;
+        if (!tiles.empty()) {
+            hipLaunchKernelGGL(k_resolve_body, dim3((uint32_t)std::min<size_t>(tiles.size(), (size_t)c->num_cus * 32)),
+                               dim3(256), 0, c->stream, (const ResolvePiece *)part(0), (const ResolveTile *)part(2),
+                               (uint32_t)tiles.size());
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+        }
+    }
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+
+    // ---- CRC32 per member
     if (!crc_members.empty()) {
-        mseg.push_back((uint32_t)seg_off.size());
         const CrcConst &cc = crc_const();
         if (c->d_crc_tab.cap == 0) {
             c->d_crc_tab.ensure(sizeof(cc.tab));
             HIPCHK(hipMemcpy(c->d_crc_tab.p, cc.tab, sizeof(cc.tab), hipMemcpyHostToDevice));
         }
-        uint32_t nseg = (uint32_t)seg_off.size();
         c->d_seg_crc.ensure(std::max<size_t>(nseg, 1) * 4);
         if (nseg) {
-            upload(c, c->d_seg_off, seg_off.data(), seg_off.size() * 8);
-            upload(c, c->d_seg_len, seg_len.data(), seg_len.size() * 4);
-            upload(c, c->d_seg_tail, seg_tail.data(), seg_tail.size() * 8);
             hipLaunchKernelGGL(k_crc_segments, dim3(std::min<uint32_t>(nseg, (uint32_t)c->num_cus * 8)), dim3(256), 0,
-                               c->stream, base, c->d_seg_off.as<uint64_t>(), c->d_seg_len.as<uint32_t>(),
-                               c->d_seg_tail.as<uint64_t>(), nseg, c->d_crc_tab.as<uint32_t>(),
+                               c->stream, base, (const uint64_t *)part(3), (const uint32_t *)part(4),
+                               (const uint64_t *)part(5), nseg, c->d_crc_tab.as<uint32_t>(),
                                c->d_seg_crc.as<uint32_t>());
             HIPCHK(hipGetLastError());
             c->st.launches++;
         }
-        uint32_t nm = (uint32_t)crc_members.size();
-        upload(c, c->d_mseg, mseg.data(), mseg.size() * 4);
-        upload(c, c->d_mlen, mlen.data(), mlen.size() * 8);
         c->d_mcrc.ensure(nm * 4);
         hipLaunchKernelGGL(k_crc_members, dim3(std::min<uint32_t>(nm, (uint32_t)c->num_cus * 8)), dim3(256), 0, c->stream,
-                           c->d_seg_crc.as<uint32_t>(), c->d_mseg.as<uint32_t>(), c->d_mlen.as<uint64_t>(), nm,
+                           c->d_seg_crc.as<uint32_t>(), (const uint32_t *)part(6), (const uint64_t *)part(7), nm,
                            c->d_mcrc.as<uint32_t>());
         HIPCHK(hipGetLastError());
         c->st.launches++;
