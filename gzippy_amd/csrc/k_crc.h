@@ -49,6 +49,14 @@ __constant__ uint32_t c_kseg[256];
 // shift a raw CRC register over `len` zero bytes
 __device__ __forceinline__ uint32_t crc_shift(uint32_t c, uint64_t len) { return len ? gf2_mulmod(xpow_bytes(len), c) : c; }
 
+// undo the shift over n zero bytes: divide by x^(8n) mod P.  One step of
+// multiplication by x is c -> (c >> 1) ^ (P if c & 1); P has bit 31 set, so
+// the top bit of the product tells the low bit of c, and the step inverts.
+__device__ __forceinline__ uint32_t crc_unshift(uint32_t c, uint32_t n) {
+    for (uint32_t i = 0; i < 8 * n; i++) c = (c & 0x80000000u) ? (((c ^ CRC_POLY) << 1) | 1u) : (c << 1);
+    return c;
+}
+
 
 // Segments: seg_off[i] bytes into `base`, seg_len[i] <= CRC_SEG.  Every
 // segment is folded as the last seg_len bytes of a 64 KiB frame whose first
@@ -70,8 +78,48 @@ __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict_
     for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
         const uint8_t *p = base + seg_off[s];
         const uint32_t len = seg_len[s];
-        const uint32_t f0 = CRC_SEG - len;  // frame byte of the segment's first byte
-        if (len == CRC_SEG && (((uintptr_t)p) & 15) == 0) {
+        uint32_t f0 = CRC_SEG - len;  // frame byte of the segment's first byte
+        // A segment whose 16-byte chunks fit the frame (lead bytes before it
+        // in its first chunk, pad zero bytes after it to the next 16-byte
+        // boundary) is staged with 16-byte loads, masked at both ends; the
+        // frame then holds the segment followed by `pad` zeros, whose shift
+        // is undone at the end.
+        const uint32_t lead = (uint32_t)((uintptr_t)p & 15), pad = (uint32_t)(-(uintptr_t)(p + len) & 15);
+        const bool chunked = len && lead + len + pad <= CRC_SEG && !(len == CRC_SEG && lead == 0);
+        if (chunked) f0 = CRC_SEG - len - pad;
+        if (chunked) {
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            const uint32_t fc = f0 - lead;  // frame byte of the first chunk (16-byte aligned)
+            const uint32_t nch = (lead + len + pad) >> 4;
+            gptr<const u32x4> src = (gptr<const u32x4>)to_global(p - lead);
+            for (uint32_t k = threadIdx.x; k < nch; k += blockDim.x) {
+                u32x4 v = src[k];
+                if (k == 0 || k == nch - 1) {
+                    // zero the bytes outside [lead, lead + len) of the chunk run
+                    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+#pragma unroll
+                        for (int b = 0; b < 4; b++) {
+                            const uint32_t q = 16 * k + 4 * j + b;  // byte of the chunk run
+                            if (q < lead || q >= lead + len) w[j] &= ~(0xFFu << (8 * b));
+                        }
+                    v.x = w[0];
+                    v.y = w[1];
+                    v.z = w[2];
+                    v.w = w[3];
+                }
+                const uint32_t o = fc + 16 * k;
+                uint32_t *d = &buf[65 * (o >> 8) + ((o & 255) >> 2)];
+                d[0] = v.x;
+                d[1] = v.y;
+                d[2] = v.z;
+                d[3] = v.w;
+            }
+            // the first data row's bytes before the chunk run read as zero
+            const uint32_t r0 = fc >> 8;
+            for (uint32_t D = r0 * 64 + threadIdx.x; D < (fc >> 2); D += blockDim.x) buf[65 * (D >> 6) + (D & 63)] = 0;
+        } else if (len == CRC_SEG && (((uintptr_t)p) & 15) == 0) {
             typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
             gptr<const u32x4> src = (gptr<const u32x4>)to_global(p);
 #pragma unroll 4
@@ -119,7 +167,7 @@ __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict_
         }
         if (c) atomicXor(&red, c);
         __syncthreads();
-        if (threadIdx.x == 0) seg_crc[s] = crc_shift(red, seg_tail[s]);
+        if (threadIdx.x == 0) seg_crc[s] = crc_shift(chunked ? crc_unshift(red, pad) : red, seg_tail[s]);
         __syncthreads();
     }
 }
