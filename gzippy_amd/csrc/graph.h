@@ -86,9 +86,9 @@ __global__ void k_graph_prep(const uint8_t *__restrict__ in, uint8_t *out, const
     head[i] = h;
 }
 
-// Segment s of member seg_member[s] covers its bytes [seg_start[s],
-// seg_start[s] + 64 KiB), clipped to what the member decoded; seg_tail is
-// what the member decoded after it.
+// Segment s of member seg_member[s] covers its bytes from seg_start[s] to
+// the next segment's start (64 KiB for its last), clipped to what the member
+// decoded; seg_tail is what the member decoded after it.
 __global__ void k_graph_seglen(const PieceOut *__restrict__ pout, const uint32_t *__restrict__ seg_member,
                                const uint64_t *__restrict__ seg_start, uint32_t nseg, uint32_t n,
                                uint32_t *__restrict__ seg_len, uint64_t *__restrict__ seg_tail,
@@ -98,7 +98,8 @@ __global__ void k_graph_seglen(const PieceOut *__restrict__ pout, const uint32_t
         const PieceOut &r = pout[seg_member[i]];
         const uint64_t got = r.status == PS_FINAL ? r.n8 : 0;
         const uint64_t s = seg_start[i];
-        const uint32_t l = got > s ? (uint32_t)(got - s < CRC_SEG ? got - s : CRC_SEG) : 0u;
+        const uint64_t cap = i + 1 < nseg && seg_member[i + 1] == seg_member[i] ? seg_start[i + 1] - s : CRC_SEG;
+        const uint32_t l = got > s ? (uint32_t)(got - s < cap ? got - s : cap) : 0u;
         seg_len[i] = l;
         seg_tail[i] = got > s + l ? got - s - l : 0;
     }
@@ -170,7 +171,10 @@ int gzgpu_batch_graph_create(gzgpu_ctx *c, const void *d_in, const uint64_t *in_
             gm[i] = {ai.shift + in_off[i], in_len[i], out_off[i], out_cap[i]};
             in_total = std::max<uint64_t>(in_total, ai.shift + in_off[i] + in_len[i]);
             mseg.push_back((uint32_t)seg_off.size());
-            for (uint64_t o = 0; o < out_cap[i]; o += CRC_SEG) {
+            // segment boundaries on 16-byte addresses (full aligned frames
+            // but for a member's first and last segment)
+            const uint64_t lead = ((uintptr_t)d_out + out_off[i]) & 15;
+            for (uint64_t o = 0; o < out_cap[i]; o += o ? CRC_SEG : CRC_SEG - lead) {
                 seg_member.push_back((uint32_t)i);
                 seg_start.push_back(o);
                 seg_off.push_back(out_off[i] + o);

@@ -1652,11 +1652,14 @@ int gzgpu_crc32(gzgpu_ctx *c, const void *d_data, size_t len, uint32_t *crc) {
         }
         std::vector<uint64_t> so, st;
         std::vector<uint32_t> sl;
-        for (uint64_t o = 0; o < len; o += CRC_SEG) {
-            const uint64_t l = std::min<uint64_t>(CRC_SEG, len - o);
+        // segment boundaries on 16-byte addresses
+        const uint64_t lead = (uintptr_t)d_data & 15;
+        for (uint64_t o = 0; o < len;) {
+            const uint64_t l = std::min<uint64_t>(o ? CRC_SEG : CRC_SEG - lead, len - o);
             so.push_back(o);
             sl.push_back((uint32_t)l);
             st.push_back(len - o - l);
+            o += l;
         }
         uint32_t nseg = (uint32_t)so.size();
         c->d_seg_crc.ensure(std::max<size_t>(nseg, 1) * 4);

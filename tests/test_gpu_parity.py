@@ -40,12 +40,17 @@ def test_crc32_device(gpu, n):
 
 
 def test_crc32_unaligned(gpu):
+    """Unaligned starts and ends: a segment that starts or ends inside a
+    16-byte chunk is staged with masked 16-byte loads (down to one chunk
+    holding the whole segment) and the trailing pad's shift divided out."""
     rng = np.random.default_rng(7)
     d = rng.integers(0, 256, 300_001, dtype=np.uint8)
     t = dev(d)
-    for off in (1, 2, 3, 5):
+    for off in (1, 2, 3, 5, 8, 15):
         sub = t[off:]
         assert gpu.crc32(sub, d.size - off) == zlib.crc32(d[off:].tobytes())
+        for n in (1, 7, 14, 16, 17, 4095, 65535 - off, 65536, 65537 + off):
+            assert gpu.crc32(sub, n) == zlib.crc32(d[off:off + n].tobytes()), (off, n)
 
 
 # ---------------------------------------------------------------- block finder
