@@ -209,11 +209,7 @@ int gzgpu_batch_graph_create(gzgpu_ctx *c, const void *d_in, const uint64_t *in_
         g->status.ensure(n * 4);
         g->olen.ensure(n * 8);
         g->ocrc.ensure(n * 4);
-        const CrcConst &cc = crc_const();
-        if (c->d_crc_tab.cap == 0) {
-            c->d_crc_tab.ensure(sizeof(cc.tab));
-            HIPCHK(hipMemcpy(c->d_crc_tab.p, cc.tab, sizeof(cc.tab), hipMemcpyHostToDevice));
-        }
+        ensure_crc_tables(c);
         HIPCHK(hipStreamSynchronize(c->stream));
 
         const uint32_t grid = (uint32_t)std::min<uint64_t>(n, (uint64_t)c->num_cus * U8_WAVES_PER_CU);

@@ -96,6 +96,7 @@ struct DevBuf {
 // ---------------------------------------------------------------- host CRC tables
 struct CrcConst {
     uint32_t tab[8][256];
+    uint32_t mx[4][256];  // follows tab: c -> c * x^(8 * 4096) mod P, one table per byte of c
     uint32_t xbyte[16][16];
     uint32_t kseg[256];  // x^(8 * 256 * (255 - l)) mod P
     CrcConst() {
@@ -125,6 +126,14 @@ struct CrcConst {
                 if (nbytes & 15) p = mulmod(xbyte[k][nbytes & 15], p);
             kseg[l] = p;
         }
+        {
+            uint64_t nbytes = 4096;
+            uint32_t X = 1u << 31;
+            for (int k = 0; nbytes; k++, nbytes >>= 4)
+                if (nbytes & 15) X = mulmod(xbyte[k][nbytes & 15], X);
+            for (int k = 0; k < 4; k++)
+                for (uint32_t b = 0; b < 256; b++) mx[k][b] = mulmod(X, b << (8 * k));
+        }
     }
     static uint32_t mulmod(uint32_t a, uint32_t b) {
         uint32_t r = 0;
@@ -139,6 +148,7 @@ const CrcConst &crc_const() {
     static CrcConst c;
     return c;
 }
+
 
 // RFC 1952 header length (gzip_format.rs:47-131).  0 = ok.
 int parse_header(const uint8_t *p, size_t n, size_t *hdr_len) {
@@ -262,7 +272,8 @@ struct gzgpu_ctx {
     int num_cus = 256;
     gzgpu_tuning tune{};
     std::string err;
-    DevBuf d_lists;  // resolve and CRC lists of the current call, one upload
+    DevBuf d_lists;     // resolve and CRC lists of the current call, one upload
+    DevBuf d_tile_crc;  // CRC32 of each resolve tile, from k_resolve_body
     DevBuf d_in, d_out, d_pieces, d_pout, d_order, d_counters, d_surv, d_hits, d_slab, d_crc_tab, d_seg_off,
         d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len, d_bf_ranges, d_seg_tail;
     std::vector<DevBuf> extra_slabs;
@@ -328,6 +339,16 @@ double env_double(const char *name, double dflt) {
 uint64_t env_u64(const char *name, uint64_t dflt) {
     const char *e = getenv(name);
     return e && *e ? strtoull(e, nullptr, 10) : dflt;
+}
+
+// The slicing tables and the x^(8 * 4096) multiply tables, contiguous on the
+// device (uploaded once per context).
+void ensure_crc_tables(gzgpu_ctx *c) {
+    if (c->d_crc_tab.cap) return;
+    const CrcConst &cc = crc_const();
+    static_assert(offsetof(CrcConst, mx) == sizeof(CrcConst::tab), "tab, mx contiguous");
+    c->d_crc_tab.ensure(sizeof(cc.tab) + sizeof(cc.mx));
+    HIPCHK(hipMemcpy(c->d_crc_tab.p, cc.tab, sizeof(cc.tab) + sizeof(cc.mx), hipMemcpyHostToDevice));
 }
 
 // Indices of `keys` in ascending key order, equal keys by index (LSD radix,
@@ -1002,9 +1023,11 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     std::vector<ResolvePiece> rps;
     std::vector<uint32_t> chains;
     std::vector<ResolveTile> tiles;
+    std::vector<uint32_t> tile_lo(mem.size(), 0), tile_hi(mem.size(), 0);  // each member's tiles
     for (uint32_t m = 0; m < mem.size(); m++) {
         Member &M = mem[m];
         if (M.status != GZGPU_OK) continue;
+        tile_lo[m] = tile_hi[m] = (uint32_t)tiles.size();
         uint64_t pos = 0;
         bool final_seen = false;
         bool chain_open = false;
@@ -1050,6 +1073,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                         tl.pad = 0;
                         tl.t0 = t;
                         tl.t1 = std::min<uint64_t>(t + tile, L - WIN);
+                        tl.tail = 0;  // set once the member's size is known
                         tiles.push_back(tl);
                     }
                 }
@@ -1068,13 +1092,21 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             continue;
         }
         M.out_len = pos;
+        tile_hi[m] = (uint32_t)tiles.size();
+        for (uint32_t i = tile_lo[m]; i < tile_hi[m]; i++) {
+            ResolveTile &tl = tiles[i];
+            tl.tail = M.out_len - (rps[tl.piece].out_pos + tl.t1);
+        }
     }
     if (!rps.empty()) chains.push_back((uint32_t)rps.size());
 
     const double tasm = debug_enabled() ? host_ms() : 0;
     // ---- CRC32 lists (every member's output size is known now)
+    // k_resolve_body returns the CRC of every tile it writes; the segments
+    // here cover the rest of each member (pieces decoded straight to the
+    // output, and each marker piece's last 32 KiB, written by k_chain)
     std::vector<uint64_t> seg_off, seg_tail;
-    std::vector<uint32_t> seg_len, mseg;
+    std::vector<uint32_t> seg_len, mseg, mtile;
     std::vector<uint64_t> mlen;
     std::vector<uint32_t> crc_members;
     const uint8_t *base = nullptr;
@@ -1085,17 +1117,30 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         if (M.status != GZGPU_OK) continue;
         crc_members.push_back(m);
         mseg.push_back((uint32_t)seg_off.size());
+        mtile.push_back(tile_lo[m]);
+        mtile.push_back(tile_hi[m]);
         mlen.push_back(M.out_len);
-        // segment boundaries on 16-byte addresses: every segment but a
-        // member's first and last is a full, aligned 64 KiB frame
-        const uint64_t lead = (uint64_t)((uintptr_t)M.out & 15);
-        for (uint64_t o = 0; o < M.out_len;) {
-            const uint64_t l = std::min<uint64_t>(o ? CRC_SEG : CRC_SEG - lead, M.out_len - o);
-            seg_off.push_back((uint64_t)(M.out - base) + o);
-            seg_len.push_back((uint32_t)l);
-            seg_tail.push_back(M.out_len - o - l);
-            o += l;
+        // output bytes [a, b) as segments with boundaries on 16-byte
+        // addresses: every segment but a range's first and last is a full,
+        // aligned 64 KiB frame
+        auto add_range = [&](uint64_t a, uint64_t b) {
+            const uint64_t lead = (uint64_t)((uintptr_t)(M.out + a) & 15);
+            for (uint64_t o = a; o < b;) {
+                const uint64_t l = std::min<uint64_t>(o == a ? CRC_SEG - lead : CRC_SEG, b - o);
+                seg_off.push_back((uint64_t)(M.out - base) + o);
+                seg_len.push_back((uint32_t)l);
+                seg_tail.push_back(M.out_len - o - l);
+                o += l;
+            }
+        };
+        // the gaps between the member's tiles (tiles run in output order)
+        uint64_t at = 0;
+        for (uint32_t i = tile_lo[m]; i < tile_hi[m]; i++) {
+            const uint64_t t0 = rps[tiles[i].piece].out_pos + tiles[i].t0;
+            add_range(at, t0);
+            at = rps[tiles[i].piece].out_pos + tiles[i].t1;
         }
+        add_range(at, M.out_len);
     }
     if (!crc_members.empty()) mseg.push_back((uint32_t)seg_off.size());
     const uint32_t nseg = (uint32_t)seg_off.size(), nm = (uint32_t)crc_members.size();
@@ -1109,7 +1154,8 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     Part parts[] = {{rps.data(), rps.size() * sizeof(ResolvePiece), 0}, {chains.data(), chains.size() * 4, 0},
                     {tiles.data(), tiles.size() * sizeof(ResolveTile), 0}, {seg_off.data(), seg_off.size() * 8, 0},
                     {seg_len.data(), seg_len.size() * 4, 0}, {seg_tail.data(), seg_tail.size() * 8, 0},
-                    {mseg.data(), mseg.size() * 4, 0}, {mlen.data(), mlen.size() * 8, 0}};
+                    {mseg.data(), mseg.size() * 4, 0}, {mlen.data(), mlen.size() * 8, 0},
+                    {mtile.data(), mtile.size() * 4, 0}};
     size_t lists_bytes = 0;
     for (auto &pt : parts) {
         pt.off = lists_bytes;
@@ -1135,9 +1181,11 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         HIPCHK(hipEventRecord(c->ev[8], c->stream));
         c->st.launches++;
         if (!tiles.empty()) {
+            ensure_crc_tables(c);
+            c->d_tile_crc.ensure(tiles.size() * 4);
             hipLaunchKernelGGL(k_resolve_body, dim3((uint32_t)std::min<size_t>(tiles.size(), (size_t)c->num_cus * 32)),
                                dim3(256), 0, c->stream, (const ResolvePiece *)part(0), (const ResolveTile *)part(2),
-                               (uint32_t)tiles.size());
+                               (uint32_t)tiles.size(), c->d_crc_tab.as<uint32_t>(), c->d_tile_crc.as<uint32_t>());
             HIPCHK(hipGetLastError());
             c->st.launches++;
         }
@@ -1146,11 +1194,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
 
     // ---- CRC32 per member
     if (!crc_members.empty()) {
-        const CrcConst &cc = crc_const();
-        if (c->d_crc_tab.cap == 0) {
-            c->d_crc_tab.ensure(sizeof(cc.tab));
-            HIPCHK(hipMemcpy(c->d_crc_tab.p, cc.tab, sizeof(cc.tab), hipMemcpyHostToDevice));
-        }
+        ensure_crc_tables(c);
         c->d_seg_crc.ensure(std::max<size_t>(nseg, 1) * 4);
         if (nseg) {
             hipLaunchKernelGGL(k_crc_segments, dim3(std::min<uint32_t>(nseg, (uint32_t)c->num_cus * 8)), dim3(256), 0,
@@ -1161,9 +1205,10 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             c->st.launches++;
         }
         c->d_mcrc.ensure(nm * 4);
+        c->d_tile_crc.ensure(std::max<size_t>(tiles.size(), 1) * 4);
         hipLaunchKernelGGL(k_crc_members, dim3(std::min<uint32_t>(nm, (uint32_t)c->num_cus * 8)), dim3(256), 0, c->stream,
                            c->d_seg_crc.as<uint32_t>(), (const uint32_t *)part(6), (const uint64_t *)part(7), nm,
-                           c->d_mcrc.as<uint32_t>());
+                           c->d_mcrc.as<uint32_t>(), (const uint32_t *)c->d_tile_crc.p, (const uint32_t *)part(8));
         HIPCHK(hipGetLastError());
         c->st.launches++;
         const uint32_t *crcs = download<uint32_t>(c, c->d_mcrc.p, nm);
@@ -1645,11 +1690,7 @@ int gzgpu_crc32(gzgpu_ctx *c, const void *d_data, size_t len, uint32_t *crc) {
     if (!c || (!d_data && len) || !crc) return GZGPU_INVALID_ARGUMENT;
     return guarded(c, [&]() {
         HIPCHK(hipSetDevice(c->device));
-        const CrcConst &cc = crc_const();
-        if (c->d_crc_tab.cap == 0) {
-            c->d_crc_tab.ensure(sizeof(cc.tab));
-            HIPCHK(hipMemcpy(c->d_crc_tab.p, cc.tab, sizeof(cc.tab), hipMemcpyHostToDevice));
-        }
+        ensure_crc_tables(c);
         std::vector<uint64_t> so, st;
         std::vector<uint32_t> sl;
         // segment boundaries on 16-byte addresses

@@ -58,6 +58,22 @@ __device__ __forceinline__ uint32_t crc_unshift(uint32_t c, uint32_t n) {
 }
 
 
+// Raw CRC (from a zero register) of 16 bytes given as four little-endian
+// dwords: two slicing-by-8 steps over the tables t[8][256].
+__device__ __forceinline__ uint32_t crc16b(const uint32_t (*t)[256], uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3) {
+    uint32_t c = t[7][w0 & 0xFF] ^ t[6][(w0 >> 8) & 0xFF] ^ t[5][(w0 >> 16) & 0xFF] ^ t[4][w0 >> 24] ^ t[3][w1 & 0xFF] ^
+                 t[2][(w1 >> 8) & 0xFF] ^ t[1][(w1 >> 16) & 0xFF] ^ t[0][w1 >> 24];
+    const uint32_t lo = c ^ w2;
+    return t[7][lo & 0xFF] ^ t[6][(lo >> 8) & 0xFF] ^ t[5][(lo >> 16) & 0xFF] ^ t[4][lo >> 24] ^ t[3][w3 & 0xFF] ^
+           t[2][(w3 >> 8) & 0xFF] ^ t[1][(w3 >> 16) & 0xFF] ^ t[0][w3 >> 24];
+}
+
+// c * x^(8 * 4096) mod P through the byte tables mx[4][256] (mx[k][b] is the
+// product for byte b in position k of c; the map is linear over GF(2)).
+__device__ __forceinline__ uint32_t mul_x4k(const uint32_t (*mx)[256], uint32_t c) {
+    return mx[0][c & 0xFF] ^ mx[1][(c >> 8) & 0xFF] ^ mx[2][(c >> 16) & 0xFF] ^ mx[3][c >> 24];
+}
+
 // Segments: seg_off[i] bytes into `base`, seg_len[i] <= CRC_SEG.  Every
 // segment is folded as the last seg_len bytes of a 64 KiB frame whose first
 // bytes are zero: leading zeros leave a raw CRC register at 0, so the frame's
@@ -172,17 +188,23 @@ __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict_
     }
 }
 
-// Per member m: segments [mseg[m], mseg[m+1]), total bytes mlen[m].  One
-// workgroup per member.
+// Per member m: segments [mseg[m], mseg[m+1]) and, when tile_crc is given,
+// resolve tiles [mtile[2m], mtile[2m+1]) (their CRCs come from
+// k_resolve_body, shifted the same way), total bytes mlen[m].  One workgroup
+// per member.
 __global__ void __launch_bounds__(256) k_crc_members(const uint32_t *__restrict__ seg_crc,
                                                      const uint32_t *__restrict__ mseg, const uint64_t *__restrict__ mlen,
-                                                     uint32_t nmem, uint32_t *__restrict__ crc_out) {
+                                                     uint32_t nmem, uint32_t *__restrict__ crc_out,
+                                                     const uint32_t *__restrict__ tile_crc = nullptr,
+                                                     const uint32_t *__restrict__ mtile = nullptr) {
     __shared__ uint32_t red;
     for (uint32_t m = blockIdx.x; m < nmem; m += gridDim.x) {
         if (threadIdx.x == 0) red = 0;
         __syncthreads();
         uint32_t c = 0;
         for (uint32_t s = mseg[m] + threadIdx.x; s < mseg[m + 1]; s += blockDim.x) c ^= seg_crc[s];
+        if (tile_crc)
+            for (uint32_t s = mtile[2 * m] + threadIdx.x; s < mtile[2 * m + 1]; s += blockDim.x) c ^= tile_crc[s];
         if (c) atomicXor(&red, c);
         __syncthreads();
         if (threadIdx.x == 0) crc_out[m] = ~(crc_shift(0xFFFFFFFFu, mlen[m]) ^ red);

@@ -18,6 +18,7 @@
 // output.
 #pragma once
 #include "gz_common.h"
+#include "k_crc.h"
 
 namespace gz {
 
@@ -32,6 +33,7 @@ struct ResolveTile {
     uint32_t piece;
     uint32_t pad;
     uint64_t t0, t1;
+    uint64_t tail;  // member output bytes after the tile
 };
 
 __device__ __forceinline__ uint8_t piece_entry(const ResolvePiece &r, uint64_t t, const uint8_t *win) {
@@ -108,9 +110,29 @@ __device__ __forceinline__ uint32_t resolve16(uint32_t e, gptr<const uint8_t> wi
 //   * all u16 entries: nine aligned dword loads, markers resolved from the
 //     window just before the piece;
 //   * tile edges and the u16/u8 seam: entry by entry.
+// The tile's CRC32 comes out of the same pass (the bytes are not read
+// again): over the tile's 16-byte blocks [A0, A1) with the bytes outside
+// the tile taken as zero, block i contributes crc(block) * x^(8 (A1 - end of
+// block)).  A thread's blocks are 4096 bytes apart, so it folds them in
+// order with acc = acc * x^(8 * 4096) ^ crc(block) and shifts once to A1;
+// the threads' values XOR together, the trailing zeros (A1 - d1) are divided
+// out and the result is shifted over the member bytes after the tile, as
+// k_crc_segments does for its segments.
 __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__restrict__ rp,
-                                                      const ResolveTile *__restrict__ tiles, uint32_t ntiles) {
+                                                      const ResolveTile *__restrict__ tiles, uint32_t ntiles,
+                                                      const uint32_t *__restrict__ crc_tables,
+                                                      uint32_t *__restrict__ tile_crc) {
+    __shared__ uint32_t t8[8][256];
+    __shared__ uint32_t mx[4][256];
+    __shared__ uint32_t red;
+    for (uint32_t i = threadIdx.x; i < 8 * 256; i += blockDim.x) t8[i >> 8][i & 255] = crc_tables[i];
+    for (uint32_t i = threadIdx.x; i < 4 * 256; i += blockDim.x) mx[i >> 8][i & 255] = crc_tables[8 * 256 + i];
+    __syncthreads();
     for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+        if (threadIdx.x == 0) red = 0;
+        __syncthreads();
+        uint32_t acc = 0;
+        int64_t my_end = -1;  // end of this thread's last block
         const ResolveTile tl = tiles[b];
         const ResolvePiece r = rp[tl.piece];
         gptr<uint8_t> mout = to_global(r.mout);
@@ -137,6 +159,7 @@ __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__rest
                 o.z = __builtin_amdgcn_alignbit(x3, x2, sh);
                 o.w = __builtin_amdgcn_alignbit(x4, x3, sh);
                 *(gptr<u32x4>)(mout + blk) = o;
+                acc = mul_x4k(mx, acc) ^ crc16b(t8, o.x, o.y, o.z, o.w);
             } else if (full && t + 16 <= r.n16) {
                 // u16 entries [t, t + 16): 32 bytes at slab offset 2t
                 const uint64_t so = 2 * t;
@@ -164,13 +187,28 @@ __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__rest
                 o.z = ob[2];
                 o.w = ob[3];
                 *(gptr<u32x4>)(mout + blk) = o;
+                acc = mul_x4k(mx, acc) ^ crc16b(t8, o.x, o.y, o.z, o.w);
             } else {
+                uint32_t w[4] = {0, 0, 0, 0};
                 for (int64_t k = 0; k < 16; k++) {
                     const int64_t d = blk + k;
-                    if (d >= d0 && d < d1) mout[d] = body_entry(r, (uint64_t)(d - (int64_t)r.out_pos));
+                    if (d >= d0 && d < d1) {
+                        const uint8_t v = body_entry(r, (uint64_t)(d - (int64_t)r.out_pos));
+                        mout[d] = v;
+                        w[k >> 2] |= (uint32_t)v << (8 * (k & 3));
+                    }
                 }
+                acc = mul_x4k(mx, acc) ^ crc16b(t8, w[0], w[1], w[2], w[3]);
             }
+            my_end = blk + 16;
         }
+        // the tile's blocks end at A1 (16-byte aligned past d1)
+        const int64_t A1 = first + ((d1 - first + 15) & ~(int64_t)15);
+        if (my_end >= 0) acc = crc_shift(acc, (uint64_t)(A1 - my_end));
+        if (acc) atomicXor(&red, acc);
+        __syncthreads();
+        if (threadIdx.x == 0) tile_crc[b] = crc_shift(crc_unshift(red, (uint32_t)(A1 - d1)), tl.tail);
+        __syncthreads();
     }
 }
 
