@@ -256,7 +256,21 @@ struct Member {
     uint64_t out_len = 0;
     uint64_t end_bit = 0;
     uint32_t crc = 0;
-    std::vector<uint32_t> pieces;  // in stream order
+    // the member's pieces: consecutive indices into the call's piece list
+    // (no allocation per member)
+    struct PieceRange {
+        uint32_t first = 0, n = 0;
+        size_t size() const { return n; }
+        bool empty() const { return n == 0; }
+        uint32_t operator[](size_t k) const { return first + (uint32_t)k; }
+        uint32_t back() const { return first + n - 1; }
+        void clear() { n = 0; }
+        void push_back(uint32_t i) {
+            if (n == 0) first = i;
+            else if (i != first + n) throw StatusError{GZGPU_DECODE_FAILED, "member pieces not consecutive"};
+            n++;
+        }
+    } pieces;  // in stream order
 };
 
 struct PieceHost {
