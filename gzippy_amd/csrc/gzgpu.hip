@@ -785,15 +785,13 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         // predecessor's end.
         const uint64_t span = M.de - M.ds;
         const uint64_t np = std::max<uint64_t>(2, (span + spacing / 2) / spacing);
-        const uint64_t rest = span / np;
-        size_t h = 0;
+        size_t h = 0, hk = 0;  // hk: the first hit past the current grid point
         uint64_t last = M.ds;
         struct Start {
             uint64_t start, acc_lo, acc_hi, hdr;
             bool mid;
         };
         std::vector<Start> chosen;
-        auto after = [](uint64_t v, const gzgpu_boundary &a) { return v < a.bit_offset; };
         auto chained = [&](size_t i) {
             const uint64_t o = hits[i].seek_bit + 3;  // the LEN field
             const uint64_t next = (o / 8 + 4 + (hits[i].info >> 16)) * 8;
@@ -843,7 +841,8 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 // itself, and MID_TAIL bits before the next hit.  Stored hits
                 // that are not chained are LEN/NLEN coincidences and skipped.
                 auto real = [&](size_t j) { return hits[j].kind == 2 || chained(j); };
-                size_t k = std::upper_bound(hits.begin(), hits.end(), g, after) - hits.begin();
+                while (hk < hits.size() && hits[hk].bit_offset <= g) hk++;  // grid points increase
+                size_t k = hk;
                 size_t prev = k;
                 while (prev > 0 && !real(prev - 1)) prev--;
                 while (k < hits.size() && !real(k)) k++;
