@@ -289,7 +289,7 @@ struct gzgpu_ctx {
     DevBuf d_lists;     // resolve and CRC lists of the current call, one upload
     DevBuf d_tile_crc;  // CRC32 of each resolve tile, from k_resolve_body
     DevBuf d_in, d_out, d_pieces, d_pout, d_order, d_counters, d_surv, d_hits, d_slab, d_crc_tab, d_seg_off,
-        d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_rp, d_tiles, d_chains, d_env, d_env_off, d_env_len, d_bf_ranges, d_seg_tail;
+        d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_env, d_env_off, d_env_len, d_bf_ranges, d_seg_tail;
     std::vector<DevBuf> extra_slabs;
     PinnedArena pin;
     hipEvent_t ev[10]{};
@@ -1035,12 +1035,15 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     // ---- assemble: offsets, sizes, resolve lists
     std::vector<ResolvePiece> rps;
     std::vector<uint32_t> chains;
-    std::vector<ResolveTile> tiles;
+    std::vector<uint32_t> tile_first;  // per marker piece: its first body tile (k_resolve_body)
+    uint32_t ntiles = 0;
     std::vector<uint32_t> tile_lo(mem.size(), 0), tile_hi(mem.size(), 0);  // each member's tiles
+    std::vector<uint32_t> rp_lo(mem.size(), 0), rp_hi(mem.size(), 0);
     for (uint32_t m = 0; m < mem.size(); m++) {
         Member &M = mem[m];
         if (M.status != GZGPU_OK) continue;
-        tile_lo[m] = tile_hi[m] = (uint32_t)tiles.size();
+        tile_lo[m] = tile_hi[m] = ntiles;
+        rp_lo[m] = (uint32_t)rps.size();
         uint64_t pos = 0;
         bool final_seen = false;
         bool chain_open = false;
@@ -1076,20 +1079,10 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 rp.n16 = r.n16;
                 rp.n8 = r.n8;
                 rp.u8_off = r.u8_off;
-                uint32_t idx = (uint32_t)rps.size();
+                rp.tail = 0;  // set once the member's size is known
                 rps.push_back(rp);
-                if (L > WIN) {
-                    const uint64_t tile = 64 * KIB;
-                    for (uint64_t t = 0; t < L - WIN; t += tile) {
-                        ResolveTile tl;
-                        tl.piece = idx;
-                        tl.pad = 0;
-                        tl.t0 = t;
-                        tl.t1 = std::min<uint64_t>(t + tile, L - WIN);
-                        tl.tail = 0;  // set once the member's size is known
-                        tiles.push_back(tl);
-                    }
-                }
+                tile_first.push_back(ntiles);
+                if (L > WIN) ntiles += (uint32_t)((L - WIN + RESOLVE_TILE - 1) / RESOLVE_TILE);
             }
             pos += L;
             if (r.status == PS_FINAL) {
@@ -1105,13 +1098,13 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             continue;
         }
         M.out_len = pos;
-        tile_hi[m] = (uint32_t)tiles.size();
-        for (uint32_t i = tile_lo[m]; i < tile_hi[m]; i++) {
-            ResolveTile &tl = tiles[i];
-            tl.tail = M.out_len - (rps[tl.piece].out_pos + tl.t1);
-        }
+        tile_hi[m] = ntiles;
+        rp_hi[m] = (uint32_t)rps.size();
+        for (uint32_t i = rp_lo[m]; i < rp_hi[m]; i++)
+            rps[i].tail = M.out_len - (rps[i].out_pos + rps[i].n16 + rps[i].n8);
     }
     if (!rps.empty()) chains.push_back((uint32_t)rps.size());
+    tile_first.push_back(ntiles);
 
     const double tasm = debug_enabled() ? host_ms() : 0;
     // ---- CRC32 lists (every member's output size is known now)
@@ -1146,12 +1139,14 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 o += l;
             }
         };
-        // the gaps between the member's tiles (tiles run in output order)
+        // the gaps between the member's tiles: a marker piece's tiles cover
+        // its output but for the last 32768 entries
         uint64_t at = 0;
-        for (uint32_t i = tile_lo[m]; i < tile_hi[m]; i++) {
-            const uint64_t t0 = rps[tiles[i].piece].out_pos + tiles[i].t0;
-            add_range(at, t0);
-            at = rps[tiles[i].piece].out_pos + tiles[i].t1;
+        for (uint32_t i = rp_lo[m]; i < rp_hi[m]; i++) {
+            const uint64_t L = rps[i].n16 + rps[i].n8;
+            if (L <= WIN) continue;
+            add_range(at, rps[i].out_pos);
+            at = rps[i].out_pos + L - WIN;
         }
         add_range(at, M.out_len);
     }
@@ -1165,7 +1160,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         size_t bytes, off;
     };
     Part parts[] = {{rps.data(), rps.size() * sizeof(ResolvePiece), 0}, {chains.data(), chains.size() * 4, 0},
-                    {tiles.data(), tiles.size() * sizeof(ResolveTile), 0}, {seg_off.data(), seg_off.size() * 8, 0},
+                    {tile_first.data(), tile_first.size() * 4, 0}, {seg_off.data(), seg_off.size() * 8, 0},
                     {seg_len.data(), seg_len.size() * 4, 0}, {seg_tail.data(), seg_tail.size() * 8, 0},
                     {mseg.data(), mseg.size() * 4, 0}, {mlen.data(), mlen.size() * 8, 0},
                     {mtile.data(), mtile.size() * 4, 0}};
@@ -1193,12 +1188,12 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->ev[8], c->stream));
         c->st.launches++;
-        if (!tiles.empty()) {
+        if (ntiles) {
             ensure_crc_tables(c);
-            c->d_tile_crc.ensure(tiles.size() * 4);
-            hipLaunchKernelGGL(k_resolve_body, dim3((uint32_t)std::min<size_t>(tiles.size(), (size_t)c->num_cus * 32)),
-                               dim3(256), 0, c->stream, (const ResolvePiece *)part(0), (const ResolveTile *)part(2),
-                               (uint32_t)tiles.size(), c->d_crc_tab.as<uint32_t>(), c->d_tile_crc.as<uint32_t>());
+            c->d_tile_crc.ensure((size_t)ntiles * 4);
+            hipLaunchKernelGGL(k_resolve_body, dim3((uint32_t)std::min<size_t>(ntiles, (size_t)c->num_cus * 32)), dim3(256),
+                               0, c->stream, (const ResolvePiece *)part(0), (const uint32_t *)part(2),
+                               (uint32_t)rps.size(), c->d_crc_tab.as<uint32_t>(), c->d_tile_crc.as<uint32_t>());
             HIPCHK(hipGetLastError());
             c->st.launches++;
         }
@@ -1218,7 +1213,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             c->st.launches++;
         }
         c->d_mcrc.ensure(nm * 4);
-        c->d_tile_crc.ensure(std::max<size_t>(tiles.size(), 1) * 4);
+        c->d_tile_crc.ensure(std::max<size_t>(ntiles, 1) * 4);
         hipLaunchKernelGGL(k_crc_members, dim3(std::min<uint32_t>(nm, (uint32_t)c->num_cus * 8)), dim3(256), 0, c->stream,
                            c->d_seg_crc.as<uint32_t>(), (const uint32_t *)part(6), (const uint64_t *)part(7), nm,
                            c->d_mcrc.as<uint32_t>(), (const uint32_t *)c->d_tile_crc.p, (const uint32_t *)part(8));
@@ -1246,7 +1241,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     }
     if (debug_enabled() && !rps.empty())
         fprintf(stderr, "[gzgpu] resolve: chain %.3f ms (%zu pieces), body %.3f ms (%zu tiles)\n",
-                ev_ms(c->ev[7], c->ev[8]), rps.size(), ev_ms(c->ev[8], c->ev[3]), tiles.size());
+                ev_ms(c->ev[7], c->ev[8]), rps.size(), ev_ms(c->ev[8], c->ev[3]), (size_t)ntiles);
     c->st.ms_find += ev_ms(c->ev[0], c->ev[1]);
     c->st.ms_inflate += ev_ms(c->ev[1], c->ev[2]);
     c->st.ms_resolve += ev_ms(c->ev[2], c->ev[3]);
@@ -1435,7 +1430,8 @@ void gzgpu_ctx_destroy(gzgpu_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf *b : {&c->d_in, &c->d_out, &c->d_pieces, &c->d_pout, &c->d_order, &c->d_counters, &c->d_surv, &c->d_hits,
                       &c->d_slab, &c->d_crc_tab, &c->d_seg_off, &c->d_seg_len, &c->d_seg_crc, &c->d_mseg, &c->d_mlen,
-                      &c->d_mcrc, &c->d_rp, &c->d_tiles, &c->d_chains, &c->d_env, &c->d_env_off, &c->d_env_len, &c->d_bf_ranges, &c->d_seg_tail})
+                      &c->d_mcrc, &c->d_env, &c->d_env_off, &c->d_env_len, &c->d_bf_ranges, &c->d_seg_tail, &c->d_lists,
+                      &c->d_tile_crc})
         b->release();
     for (auto &b : c->extra_slabs) b.release();
     c->pin.release();

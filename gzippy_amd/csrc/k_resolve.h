@@ -27,14 +27,13 @@ struct ResolvePiece {
     uint8_t *mout;      // member output base
     uint64_t out_pos;   // piece offset in the member output
     uint64_t n16, n8, u8_off;
+    uint64_t tail;      // member output bytes after the piece
 };
 
-struct ResolveTile {
-    uint32_t piece;
-    uint32_t pad;
-    uint64_t t0, t1;
-    uint64_t tail;  // member output bytes after the tile
-};
+// Body tiles: entries [RESOLVE_TILE k, RESOLVE_TILE (k+1)) of a piece's
+// first L - 32768 entries; tile_first[i] is the index of piece i's first
+// tile (a prefix sum, tile_first[npieces] = all tiles).
+constexpr uint64_t RESOLVE_TILE = 64 * 1024;
 
 __device__ __forceinline__ uint8_t piece_entry(const ResolvePiece &r, uint64_t t, const uint8_t *win) {
     if (t < r.n16) {
@@ -119,9 +118,10 @@ __device__ __forceinline__ uint32_t resolve16(uint32_t e, gptr<const uint8_t> wi
 // out and the result is shifted over the member bytes after the tile, as
 // k_crc_segments does for its segments.
 __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__restrict__ rp,
-                                                      const ResolveTile *__restrict__ tiles, uint32_t ntiles,
+                                                      const uint32_t *__restrict__ tile_first, uint32_t npieces,
                                                       const uint32_t *__restrict__ crc_tables,
                                                       uint32_t *__restrict__ tile_crc) {
+    const uint32_t ntiles = tile_first[npieces];
     __shared__ uint32_t t8[8][256];
     __shared__ uint32_t mx[4][256];
     __shared__ uint32_t red;
@@ -133,12 +133,22 @@ __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__rest
         __syncthreads();
         uint32_t acc = 0;
         int64_t my_end = -1;  // end of this thread's last block
-        const ResolveTile tl = tiles[b];
-        const ResolvePiece r = rp[tl.piece];
+        // the tile's piece: the last one whose first tile is at or before b
+        uint32_t lo = 0, hi = npieces;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (tile_first[mid] <= b) lo = mid;
+            else hi = mid;
+        }
+        const ResolvePiece r = rp[lo];
+        const uint64_t body = r.n16 + r.n8 - WIN;
+        const uint64_t tl_t0 = (uint64_t)(b - tile_first[lo]) * RESOLVE_TILE;
+        const uint64_t tl_t1 = tl_t0 + RESOLVE_TILE < body ? tl_t0 + RESOLVE_TILE : body;
+        const uint64_t tl_tail = r.tail + (r.n16 + r.n8 - tl_t1);
         gptr<uint8_t> mout = to_global(r.mout);
         gptr<const uint8_t> win = to_global((const uint8_t *)r.mout) + r.out_pos;  // window = win[-WIN .. 0)
         const bool near_start = r.out_pos < WIN;
-        const int64_t d0 = (int64_t)(r.out_pos + tl.t0), d1 = (int64_t)(r.out_pos + tl.t1);  // output range
+        const int64_t d0 = (int64_t)(r.out_pos + tl_t0), d1 = (int64_t)(r.out_pos + tl_t1);  // output range
         // blocks are 16-byte aligned in the address space (the member's
         // output base need not be aligned); offsets relative to mout
         const int64_t ba = (int64_t)(uintptr_t)r.mout;
@@ -207,7 +217,7 @@ __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__rest
         if (my_end >= 0) acc = crc_shift(acc, (uint64_t)(A1 - my_end));
         if (acc) atomicXor(&red, acc);
         __syncthreads();
-        if (threadIdx.x == 0) tile_crc[b] = crc_shift(crc_unshift(red, (uint32_t)(A1 - d1)), tl.tail);
+        if (threadIdx.x == 0) tile_crc[b] = crc_shift(crc_unshift(red, (uint32_t)(A1 - d1)), tl_tail);
         __syncthreads();
     }
 }
