@@ -346,15 +346,6 @@ int debug_level() {
 }
 bool debug_enabled() { return debug_level() >= 1; }
 
-double env_double(const char *name, double dflt) {
-    const char *e = getenv(name);
-    return e && *e ? atof(e) : dflt;
-}
-uint64_t env_u64(const char *name, uint64_t dflt) {
-    const char *e = getenv(name);
-    return e && *e ? strtoull(e, nullptr, 10) : dflt;
-}
-
 // The slicing tables and the x^(8 * 4096) multiply tables, contiguous on the
 // device (uploaded once per context).
 void ensure_crc_tables(gzgpu_ctx *c) {
@@ -694,10 +685,10 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
 
 void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
     const uint64_t chunk_bits = (c->tune.chunk_bytes ? c->tune.chunk_bytes : 512 * KIB) * 8;
-    const uint64_t min_spacing = 8 * env_u64("GZGPU_MIN_SPACING_KIB", MIN_SPACING_KIB) * KIB;
+    const uint64_t min_spacing = 8 * MIN_SPACING_KIB * KIB;
     auto min_spacing_of = [&](const Member &M) -> uint64_t {
         const uint64_t cbits = M.de - M.ds;
-        const double bits = (double)env_u64("GZGPU_MIN_PIECE", MIN_PIECE_OUT) * (double)cbits /
+        const double bits = (double)MIN_PIECE_OUT * (double)cbits /
                             (double)member_cost(cbits, M.exp_isize);
         return std::min<uint64_t>(min_spacing, std::max<uint64_t>(8 * MIN_SPACING_FLOOR_KIB * KIB, (uint64_t)bits));
     };
@@ -712,10 +703,8 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     uint64_t total_cost = 0;
     for (auto &M : mem)
         if (M.status == GZGPU_OK) total_cost += cost_of(M);
-    const double pps = env_double("GZGPU_PPS", PIECES_PER_SLOT);
-    const double slot_frac = env_double("GZGPU_SLOT_FRAC", SLOT_FRACTION);
     const uint64_t target_cost = std::max<uint64_t>(
-        env_u64("GZGPU_MIN_PIECE", MIN_PIECE_OUT), (uint64_t)((double)total_cost / (pps * slot_frac * (double)slots)));
+        MIN_PIECE_OUT, (uint64_t)((double)total_cost / (PIECES_PER_SLOT * SLOT_FRACTION * (double)slots)));
     const uint64_t ppc = c->tune.pieces_per_chunk;
     auto spacing_of = [&](const Member &M) -> uint64_t {
         if (ppc) return std::max<uint64_t>(chunk_bits / ppc, min_spacing);
@@ -812,7 +801,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         // phase, and the window chain and marker resolution grow with the
         // piece count.)  Measured on that member: inflate kernel 4.33 ms
         // flat, 4.20 ms at a slope of 0.22-0.3.
-        const double tilt = std::min(0.9, std::max(0.0, env_double("GZGPU_TILT", TILT)));
+        const double tilt = TILT;
         auto grid_at = [&](uint64_t k) {
             const double x = (double)k / (double)np;
             return M.ds + (uint64_t)((double)span * (x - 0.5 * tilt * x * x) / (1.0 - 0.5 * tilt));
