@@ -313,13 +313,27 @@ __device__ bool validate_dynamic(const uint8_t *__restrict__ in, uint64_t n, uin
     const uint32_t hl = hlit + 257, total_codes = hl + hdist + 1;
     uint32_t i = 0, prev = 0, len256 = 0;
     uint32_t sum_ll = 0, sum_d = 0, gt1_ll = 0, gt1_d = 0;
+    // Streaming bit buffer: `buf` holds the nb >= 32 bits from pos on and the
+    // next input word is already loaded, so the dependent load of each step
+    // (a code-length symbol takes at most 14 bits) is a word ahead instead of
+    // a full read per symbol.
+    uint64_t wi = pos >> 5;
+    uint64_t buf = ((uint64_t)load_word(in, n, wi) | (uint64_t)load_word(in, n, wi + 1) << 32) >> (pos & 31);
+    uint32_t nb = 64 - (uint32_t)(pos & 31);
+    uint32_t nxt = load_word(in, n, wi + 2);
+    wi += 3;
     while (i < total_codes) {
         if (pos >= total) return false;  // is_eof
-        uint64_t w = peek64(in, n, pos);
+        if (nb < 32) {
+            buf |= (uint64_t)nxt << nb;
+            nb += 32;
+            nxt = load_word(in, n, wi++);
+        }
+        uint64_t w = buf;
         uint32_t e = lut[w & 127];
         uint32_t L = e >> 5, sym = e & 31;
         if (L == 0) return false;
-        pos += L;
+        uint32_t used = L;
         w >>= L;
         uint32_t val, rep;
         if (sym < 16) {
@@ -329,16 +343,19 @@ __device__ bool validate_dynamic(const uint8_t *__restrict__ in, uint64_t n, uin
             if (i == 0) return false;
             val = prev;
             rep = 3 + (uint32_t)(w & 3);
-            pos += 2;
+            used += 2;
         } else if (sym == 17) {
             val = 0;
             rep = 3 + (uint32_t)(w & 7);
-            pos += 3;
+            used += 3;
         } else {
             val = 0;
             rep = 11 + (uint32_t)(w & 127);
-            pos += 7;
+            used += 7;
         }
+        pos += used;
+        buf >>= used;
+        nb -= used;
         if (i + rep > total_codes) return false;
         if (i <= 256 && 256 < i + rep) len256 = val;
         uint32_t n_ll = i < hl ? (hl - i < rep ? hl - i : rep) : 0;
