@@ -307,8 +307,8 @@ def main():
                          "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); sharding members or pieces "
+              "across GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
