@@ -1028,14 +1028,20 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     uint32_t ntiles = 0;
     std::vector<uint32_t> tile_lo(mem.size(), 0), tile_hi(mem.size(), 0);  // each member's tiles
     std::vector<uint32_t> rp_lo(mem.size(), 0), rp_hi(mem.size(), 0);
+    struct Out {
+        uint32_t pi;
+        uint64_t pos, L;
+    };
+    std::vector<Out> outs;  // one member's pieces with output, in order
     for (uint32_t m = 0; m < mem.size(); m++) {
         Member &M = mem[m];
         if (M.status != GZGPU_OK) continue;
         tile_lo[m] = tile_hi[m] = ntiles;
-        rp_lo[m] = (uint32_t)rps.size();
+        rp_lo[m] = rp_hi[m] = (uint32_t)rps.size();
+        // pass 1: sizes, offsets, status
+        outs.clear();
         uint64_t pos = 0;
         bool final_seen = false;
-        bool chain_open = false;
         for (size_t k = 0; k < M.pieces.size(); k++) {
             uint32_t pi = M.pieces[k];
             if (P[pi].dropped) break;
@@ -1045,34 +1051,13 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 break;
             }
             const PieceOut &r = R[pi];
-            uint64_t L = r.n16 + r.n8;
-            if (P[pi].d.flags & PF_MARKER) {
-                if (pos + L > M.out_cap) {
-                    M.status = GZGPU_OUTPUT_TOO_SMALL;
-                    M.msg = "output capacity exceeded";
-                    break;
-                }
-                // a piece that flipped to u8 holds at least 32768 entries and
-                // its last 32768 are marker-free: its end window needs no
-                // predecessor, so a new chain starts there (chains run in
-                // parallel, one workgroup each)
-                const bool self_windowed = r.u8_off != 0 && L >= WIN;
-                if (!chain_open || self_windowed) {
-                    chains.push_back((uint32_t)rps.size());
-                    chain_open = true;
-                }
-                ResolvePiece rp;
-                rp.slab = P[pi].d.dst;
-                rp.mout = M.out;
-                rp.out_pos = pos;
-                rp.n16 = r.n16;
-                rp.n8 = r.n8;
-                rp.u8_off = r.u8_off;
-                rp.tail = 0;  // set once the member's size is known
-                rps.push_back(rp);
-                tile_first.push_back(ntiles);
-                if (L > WIN) ntiles += (uint32_t)((L - WIN + RESOLVE_TILE - 1) / RESOLVE_TILE);
+            const uint64_t L = r.n16 + r.n8;
+            if ((P[pi].d.flags & PF_MARKER) && pos + L > M.out_cap) {
+                M.status = GZGPU_OUTPUT_TOO_SMALL;
+                M.msg = "output capacity exceeded";
+                break;
             }
+            outs.push_back({pi, pos, L});
             pos += L;
             if (r.status == PS_FINAL) {
                 final_seen = true;
@@ -1087,13 +1072,72 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             continue;
         }
         M.out_len = pos;
+        // pass 2: resolve lists.  A marker piece that flipped to u8 has final
+        // bytes as its last 32768 entries.  When all of those are u8 entries
+        // and the next piece with output holds 32768 entries or more (so no
+        // later window reaches back into it), k_resolve_body resolves it
+        // whole and the pieces up to that next one read their window from
+        // its slab.  Every other marker piece is in a chain: k_chain writes
+        // its last 32 KiB before k_resolve_body runs, so every window read
+        // from the output is final by then.  A chain starts anew at a piece
+        // that flipped (its end window needs no predecessor); chains run in
+        // parallel, one workgroup each.
+        bool chain_open = false;
+        const uint8_t *flip_end = nullptr;  // the slab-resolved piece windows come from
+        for (size_t k = 0; k < outs.size(); k++) {
+            const uint32_t pi = outs[k].pi;
+            const PieceOut &r = R[pi];
+            const uint64_t L = outs[k].L;
+            if (!(P[pi].d.flags & PF_MARKER)) {
+                if (L) flip_end = nullptr;
+                continue;
+            }
+            uint64_t nextL = WIN;  // the next piece with output (none: nothing reads back)
+            for (size_t j = k + 1; j < outs.size(); j++)
+                if (outs[j].L) {
+                    nextL = outs[j].L;
+                    break;
+                }
+            const bool self_windowed = r.u8_off != 0 && L >= WIN;
+            const bool whole = self_windowed && r.n8 >= WIN && nextL >= WIN;
+            if (whole) {
+                if (chain_open) chains.push_back((uint32_t)rps.size());
+                chain_open = false;
+            } else if (!chain_open || self_windowed) {
+                if (chain_open) chains.push_back((uint32_t)rps.size());
+                chains.push_back((uint32_t)rps.size());
+                chain_open = true;
+            }
+            ResolvePiece rp;
+            rp.slab = P[pi].d.dst;
+            rp.mout = M.out;
+            rp.out_pos = outs[k].pos;
+            rp.n16 = r.n16;
+            rp.n8 = r.n8;
+            rp.u8_off = r.u8_off;
+            rp.tail = M.out_len - (outs[k].pos + L);
+            rp.body = whole ? L : (L > WIN ? L - WIN : 0);
+            rp.win_end = flip_end ? flip_end : M.out + outs[k].pos;
+            rp.win_out = flip_end ? 0u : 1u;
+            rp.pad = 0;
+            rps.push_back(rp);
+            tile_first.push_back(ntiles);
+            ntiles += (uint32_t)((rp.body + RESOLVE_TILE - 1) / RESOLVE_TILE);
+            if (L) flip_end = whole ? P[pi].d.dst + r.u8_off + r.n8 : nullptr;
+        }
+        if (chain_open) chains.push_back((uint32_t)rps.size());
         tile_hi[m] = ntiles;
         rp_hi[m] = (uint32_t)rps.size();
-        for (uint32_t i = rp_lo[m]; i < rp_hi[m]; i++)
-            rps[i].tail = M.out_len - (rps[i].out_pos + rps[i].n16 + rps[i].n8);
     }
-    if (!rps.empty()) chains.push_back((uint32_t)rps.size());
     tile_first.push_back(ntiles);
+    if (debug_level() >= 3) {
+        for (size_t i = 0; i < rps.size(); i++)
+            fprintf(stderr, "[gzgpu] rp %zu out %llu n16 %llu n8 %llu u8_off %llu body %llu win_out %u\n", i,
+                    (unsigned long long)rps[i].out_pos, (unsigned long long)rps[i].n16, (unsigned long long)rps[i].n8,
+                    (unsigned long long)rps[i].u8_off, (unsigned long long)rps[i].body, rps[i].win_out);
+        for (size_t i = 0; i + 1 < chains.size(); i += 2)
+            fprintf(stderr, "[gzgpu] chain %u..%u\n", chains[i], chains[i + 1]);
+    }
 
     const double tasm = debug_enabled() ? host_ms() : 0;
     // ---- CRC32 lists (every member's output size is known now)
@@ -1129,13 +1173,12 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             }
         };
         // the gaps between the member's tiles: a marker piece's tiles cover
-        // its output but for the last 32768 entries
+        // its first `body` entries
         uint64_t at = 0;
         for (uint32_t i = rp_lo[m]; i < rp_hi[m]; i++) {
-            const uint64_t L = rps[i].n16 + rps[i].n8;
-            if (L <= WIN) continue;
+            if (rps[i].body == 0) continue;
             add_range(at, rps[i].out_pos);
-            at = rps[i].out_pos + L - WIN;
+            at = rps[i].out_pos + rps[i].body;
         }
         add_range(at, M.out_len);
     }
@@ -1170,13 +1213,15 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         fprintf(stderr, "[gzgpu] host: link %.3f ms, assemble %.3f ms, crc lists + stage %.3f ms (%zu KB)\n",
                 tlink - tpost, tasm - tlink, host_ms() - tasm, lists_bytes >> 10);
     if (!rps.empty()) {
-        uint32_t nch = (uint32_t)chains.size() - 1;
+        const uint32_t nch = (uint32_t)(chains.size() / 2);  // (first, end) pairs
         HIPCHK(hipEventRecord(c->ev[7], c->stream));
-        hipLaunchKernelGGL(k_chain, dim3(std::min<uint32_t>(nch, (uint32_t)c->num_cus * 4)), dim3(1024), 0, c->stream,
-                           (const ResolvePiece *)part(0), (const uint32_t *)part(1), nch);
-        HIPCHK(hipGetLastError());
+        if (nch) {
+            hipLaunchKernelGGL(k_chain, dim3(std::min<uint32_t>(nch, (uint32_t)c->num_cus * 4)), dim3(1024), 0,
+                               c->stream, (const ResolvePiece *)part(0), (const uint32_t *)part(1), nch);
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+        }
         HIPCHK(hipEventRecord(c->ev[8], c->stream));
-        c->st.launches++;
         if (ntiles) {
             ensure_crc_tables(c);
             c->d_tile_crc.ensure((size_t)ntiles * 4);

@@ -8,14 +8,16 @@
 // with window = the 32768 final output bytes before the piece (zero-padded
 // at the member start, WindowMap{0: zeros}).
 //
-// k_chain walks a chain of marker pieces in order, keeping the rolling 32 KiB
+// A piece that flipped to u8 entries has final bytes as its last 32768
+// entries.  When those are all u8 entries and no later window reaches back
+// into them, k_resolve_body resolves that piece whole and its successor
+// reads its window from the piece's slab.  Every other marker piece belongs
+// to a chain: k_chain walks each chain in order, keeping the rolling 32 KiB
 // window in LDS, and writes each piece's last 32 KiB (the end window its
-// successor needs: get_last_window, chunk_data.rs:1295).  A chain starts at
-// the member's first marker piece and at every piece that flipped to u8
-// entries (its last 32 KiB are literal, so its end window depends on nothing
-// before it); chains are independent workgroups.  k_resolve_body then
-// resolves every other entry in parallel, reading windows from the now final
-// output.
+// successor needs: get_last_window, chunk_data.rs:1295); a chain starts anew
+// at a flipped piece, and chains are independent workgroups.  k_resolve_body
+// then resolves every other entry in parallel, reading each piece's window
+// from the output (final by then) or from that flipped predecessor's slab.
 #pragma once
 #include "gz_common.h"
 #include "k_crc.h"
@@ -28,11 +30,19 @@ struct ResolvePiece {
     uint64_t out_pos;   // piece offset in the member output
     uint64_t n16, n8, u8_off;
     uint64_t tail;      // member output bytes after the piece
+    uint64_t body;      // entries [0, body) are k_resolve_body's, the rest k_chain's
+    // the byte after the 32 KiB window before the piece: in the member
+    // output (win_out, positions before the member start read 0), or the
+    // end of the predecessor's u8 entries in its slab when that piece
+    // flipped (its last 32768 entries are final bytes already)
+    const uint8_t *win_end;
+    uint32_t win_out, pad;
 };
 
 // Body tiles: entries [RESOLVE_TILE k, RESOLVE_TILE (k+1)) of a piece's
-// first L - 32768 entries; tile_first[i] is the index of piece i's first
-// tile (a prefix sum, tile_first[npieces] = all tiles).
+// first `body` entries (all of a piece that flipped, else all but the last
+// 32768); tile_first[i] is the index of piece i's first tile (a prefix sum,
+// tile_first[npieces] = all tiles).
 constexpr uint64_t RESOLVE_TILE = 64 * 1024;
 
 __device__ __forceinline__ uint8_t piece_entry(const ResolvePiece &r, uint64_t t, const uint8_t *win) {
@@ -47,17 +57,18 @@ __global__ void __launch_bounds__(1024) k_chain(const ResolvePiece *__restrict__
                                                 uint32_t nchains) {
     __shared__ uint8_t win[2][WIN];
     for (uint32_t c = blockIdx.x; c < nchains; c += gridDim.x) {
-        const uint32_t p0 = chains[c], p1 = chains[c + 1];
+        const uint32_t p0 = chains[2 * c], p1 = chains[2 * c + 1];
         {
+            // a chain that starts at a flipped piece (u8 entries present,
+            // >= 32768 entries) never reads the window before it: leave it
+            // unloaded, since that output may still be being written by
+            // another chain
             const ResolvePiece f = rp[p0];
-            // a flipped first piece (u8 entries present, >= 32768 entries)
-            // never reads the window before it: leave it unloaded, since that
-            // output may still be being written by another chain
             if (!(f.u8_off != 0 && f.n16 + f.n8 >= WIN))
-            for (uint32_t j = threadIdx.x; j < WIN; j += blockDim.x) {
-                int64_t q = (int64_t)f.out_pos - (int64_t)WIN + j;
-                win[0][j] = q >= 0 ? to_global(f.mout)[q] : 0;
-            }
+                for (uint32_t j = threadIdx.x; j < WIN; j += blockDim.x) {
+                    const int64_t q = (int64_t)j - (int64_t)WIN;
+                    win[0][j] = f.win_out && (int64_t)f.out_pos + q < 0 ? 0 : to_global(f.win_end)[q];
+                }
         }
         __syncthreads();
         uint32_t cur = 0;
@@ -83,14 +94,13 @@ __global__ void __launch_bounds__(1024) k_chain(const ResolvePiece *__restrict__
 }
 
 // Value of entry t of a piece for the body pass: window bytes come from the
-// final output just before the piece (zeros before the member start).
+// 32 KiB before the piece (win_end; zeros before the member start).
 __device__ __forceinline__ uint8_t body_entry(const ResolvePiece &r, uint64_t t) {
-    gptr<const uint8_t> mout = to_global((const uint8_t *)r.mout);
     if (t < r.n16) {
         const uint32_t e = ((gptr<const uint16_t>)to_global(r.slab))[t];
         if (e < 256) return (uint8_t)e;
-        const int64_t q = (int64_t)r.out_pos + (int64_t)(e - MARKER_BASE) - (int64_t)WIN;
-        return q < 0 ? 0 : mout[q];
+        const int64_t q = (int64_t)(e - MARKER_BASE) - (int64_t)WIN;
+        return r.win_out && (int64_t)r.out_pos + q < 0 ? 0 : to_global(r.win_end)[q];
     }
     return to_global(r.slab)[r.u8_off + (t - r.n16)];
 }
@@ -141,13 +151,13 @@ __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__rest
             else hi = mid;
         }
         const ResolvePiece r = rp[lo];
-        const uint64_t body = r.n16 + r.n8 - WIN;
+        const uint64_t body = r.body;
         const uint64_t tl_t0 = (uint64_t)(b - tile_first[lo]) * RESOLVE_TILE;
         const uint64_t tl_t1 = tl_t0 + RESOLVE_TILE < body ? tl_t0 + RESOLVE_TILE : body;
         const uint64_t tl_tail = r.tail + (r.n16 + r.n8 - tl_t1);
         gptr<uint8_t> mout = to_global(r.mout);
-        gptr<const uint8_t> win = to_global((const uint8_t *)r.mout) + r.out_pos;  // window = win[-WIN .. 0)
-        const bool near_start = r.out_pos < WIN;
+        gptr<const uint8_t> win = to_global(r.win_end);  // window = win[-WIN .. 0)
+        const bool near_start = r.win_out && r.out_pos < WIN;
         const int64_t d0 = (int64_t)(r.out_pos + tl_t0), d1 = (int64_t)(r.out_pos + tl_t1);  // output range
         // blocks are 16-byte aligned in the address space (the member's
         // output base need not be aligned); offsets relative to mout
