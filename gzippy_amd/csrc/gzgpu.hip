@@ -475,8 +475,9 @@ constexpr uint32_t U8_WAVES_PER_CU = 16, U16_WAVES_PER_CU = 16;
 // With starts inside blocks a member's pieces come out equal, so one launch
 // round with one piece per resident wavefront (less a 1/32 margin: a piece
 // more than the slots waits a whole piece time) beats several: every piece
-// also pays a marker (u16) phase of its own, ~160k entries on text at half
-// the speed of u8 decoding.  A member whose cost per compressed bit is high
+// also pays a marker (u16) phase of its own (~160k entries on text, each
+// ~20% slower than after the flip), and the window chain and marker
+// resolution grow with the piece count.  A member whose cost per compressed bit is high
 // (member_cost) may go down to MIN_SPACING_FLOOR_KIB so that no piece costs
 // much more than MIN_PIECE_OUT.
 constexpr uint64_t MIN_SPACING_KIB = 2, MIN_SPACING_FLOOR_KIB = 1;
