@@ -969,6 +969,19 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                         p.d.flags &= ~PF_MIDBLOCK;
                         p.d.hdr_bit = 0;
                         todo.push_back(pi);
+                        // a redo that starts at or past its own stop bit stops
+                        // at once (the header there is non-final: the
+                        // predecessor stopped before it), so the piece after
+                        // it starts at that same bit, in this launch too
+                        while (k + 1 < M.pieces.size() && P[M.pieces[k]].d.stop_bit <= pr.end_bit &&
+                               !P[M.pieces[k + 1]].dropped) {
+                            PieceHost &q = P[M.pieces[++k]];
+                            q.d.start_bit = q.acc_lo = q.acc_hi = pr.end_bit;
+                            q.d.flags &= ~PF_MIDBLOCK;
+                            q.d.hdr_bit = 0;
+                            linked[M.pieces[k]] = 0;
+                            todo.push_back(M.pieces[k]);
+                        }
                     }
                 }
             }
@@ -1340,8 +1353,48 @@ void init_constants(gzgpu_ctx *c) {
     (void)c;
 }
 
+// Compressed size of the member at `p`, from its FEXTRA subfield: gzippy's
+// "GZ" (4 bytes: the whole member; 2 bytes: size - 1; gz_member_len,
+// src/decompress/bgzf.rs:2322-2357) or BGZF's "BC" (2 bytes: size - 1).
+// 0 when the member carries neither.
+uint64_t sized_member_len(const uint8_t *p, size_t avail) {
+    if (avail < 18 || p[0] != 0x1f || p[1] != 0x8b || p[2] != 8 || !(p[3] & 4)) return 0;
+    const size_t xlen = (size_t)p[10] | (size_t)p[11] << 8;
+    if (12 + xlen > avail) return 0;
+    const uint8_t *x = p + 12;
+    for (size_t pos = 0; pos + 4 <= xlen;) {
+        const size_t sl = (size_t)x[pos + 2] | (size_t)x[pos + 3] << 8;
+        const bool gz = x[pos] == 'G' && x[pos + 1] == 'Z', bc = x[pos] == 'B' && x[pos + 1] == 'C';
+        if (gz && sl == 4 && pos + 8 <= xlen) return le32(x + pos + 4);
+        if ((gz || bc) && sl == 2 && pos + 6 <= xlen) return ((uint64_t)x[pos + 4] | (uint64_t)x[pos + 5] << 8) + 1;
+        if (gz) return 0;
+        pos += 4 + sl;
+    }
+    return 0;
+}
+
+// Member starts (and the end) of an input whose members all carry their size
+// and tile it exactly (gz_coverage_is_pure, src/decompress/bgzf.rs:2295);
+// empty when any member does not.
+std::vector<uint64_t> sized_member_walk(const uint8_t *in, size_t n) {
+    std::vector<uint64_t> offs;
+    for (size_t off = 0; off < n;) {
+        const uint64_t len = sized_member_len(in + off, n - off);
+        if (len < 18 || len > n - off) return {};
+        offs.push_back(off);
+        off += len;
+    }
+    offs.push_back(n);
+    return offs;
+}
+
 // Decode the members of a host buffer one after another (multi-member walk,
 // src/decompress/mod.rs:725-741): each member runs the parallel pipeline.
+// Members that all carry their size (gzippy's parallel "GZ" format, BGZF)
+// decode together instead, in one pipeline, like a batch
+// (decompress_bgzf_parallel, src/decompress/bgzf.rs:1874); if any of them
+// fails there, the member-by-member walk decodes the input again and reports
+// the failure where it finds it.
 template <typename Sink>
 int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink &&sink, uint64_t *total,
                            gzgpu_stats *stats) {
@@ -1350,9 +1403,56 @@ int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink 
     if (in_len < 2 || in[0] != 0x1f || in[1] != 0x8b) return GZGPU_OK;  // non-gzip: Ok(0)
     c->d_in.ensure(in_len + 64);
     HIPCHK(hipMemcpyAsync(c->d_in.p, in, in_len, hipMemcpyHostToDevice, c->stream));
+    std::vector<uint8_t> host_out;
+    const std::vector<uint64_t> moff = sized_member_walk(in, in_len);
+    if (moff.size() > 2) {
+        gzgpu_tuning tn;
+        gzgpu_get_tuning(c, &tn);
+        std::vector<Member> mem(moff.size() - 1);
+        uint64_t out_total = 0;
+        bool ok = true;
+        for (size_t i = 0; i < mem.size() && ok; i++) {
+            std::string msg;
+            ok = member_from_host(in + moff[i], moff[i + 1] - moff[i], moff[i], mem[i], msg) == GZGPU_OK;
+            out_total += mem[i].exp_isize;
+        }
+        if (ok) {
+            c->d_out.ensure(out_total + 64);
+            uint64_t o = 0;
+            for (size_t i = 0; i < mem.size(); i++) {
+                Member &M = mem[i];
+                M.out = c->d_out.as<uint8_t>() + o;
+                M.out_cap = M.exp_isize;
+                M.split = tn.batch_split_bytes == 0 || moff[i + 1] - moff[i] > tn.batch_split_bytes;
+                o += M.exp_isize;
+            }
+            run_members(c, c->d_in.as<uint8_t>(), in_len, mem);
+            bool all = true;
+            for (auto &M : mem) all = all && M.status == GZGPU_OK;
+            if (debug_enabled()) {
+                size_t bad = 0;
+                while (bad < mem.size() && mem[bad].status == GZGPU_OK) bad++;
+                fprintf(stderr, "[gzgpu] sized members: %zu in one pipeline, %s%s\n", mem.size(),
+                        all ? "all ok" : "first failure: ", all ? "" : mem[bad].msg.c_str());
+            }
+            if (all) {
+                host_out.resize(out_total);
+                if (out_total)
+                    HIPCHK(hipMemcpyAsync(host_out.data(), c->d_out.p, out_total, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                int sr = sink(host_out.data(), (size_t)out_total);
+                if (sr) return sr;
+                *total = out_total;
+                c->st.compressed_bytes = in_len;
+                c->st.decompressed_bytes = out_total;
+                if (stats) *stats = c->st;
+                return GZGPU_OK;
+            }
+            c->st = gzgpu_stats{};
+        }
+    }
     size_t off = 0;
     int members = 0;
-    std::vector<uint8_t> host_out;
     while (off < in_len) {
         if (members > 0 && (in_len - off < 2 || in[off] != 0x1f || in[off + 1] != 0x8b)) break;  // trailing bytes
         Member M;

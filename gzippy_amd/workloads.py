@@ -144,3 +144,35 @@ def batch_members(count, seed, min_size=1 << 10, max_size=64 << 20, median=64 <<
 
     with ThreadPoolExecutor(threads) as ex:
         return list(ex.map(one, range(count)))
+
+
+BGZF_EOF = bytes.fromhex("1f8b08040000000000ff0600424302001b0003000000000000000000")
+
+
+def sized_member_stream(raw, block, level=6, form="gz"):
+    """Members of `block` input bytes each that carry their compressed size in
+    FEXTRA.  form "gz": gzippy's parallel format, subfield "GZ" with the 4-byte
+    whole-member size (src/compress/parallel.rs:470-499); "bgzf": subfield "BC"
+    with the 2-byte size - 1 and the 28-byte empty end-of-file member
+    (`block` <= 65280 there).  Bodies are zlib raw deflate at `level`."""
+    raw = bytes(raw)
+    parts = []
+    for off in range(0, len(raw), block):
+        chunk = raw[off:off + block]
+        co = zlib.compressobj(int(level), zlib.DEFLATED, -15)
+        body = co.compress(chunk) + co.flush()
+        tail = struct.pack("<II", zlib.crc32(chunk), len(chunk))
+        if form == "gz":
+            size = 10 + 2 + 8 + len(body) + 8
+            hdr = b"\x1f\x8b\x08\x04\0\0\0\0\0\xff" + struct.pack("<H", 8) + b"GZ" + struct.pack("<HI", 4, size)
+        elif form == "bgzf":
+            size = 10 + 2 + 6 + len(body) + 8
+            if size > 65536:
+                raise ValueError("BGZF member over 64 KiB")
+            hdr = b"\x1f\x8b\x08\x04\0\0\0\0\0\xff" + struct.pack("<H", 6) + b"BC" + struct.pack("<HH", 2, size - 1)
+        else:
+            raise ValueError(form)
+        parts.append(hdr + body + tail)
+    if form == "bgzf":
+        parts.append(BGZF_EOF)
+    return b"".join(parts)

@@ -120,6 +120,30 @@ def test_multi_member_and_trailing_garbage(gpu):
     assert gpu.decompress(gz + b"garbage!") == a.tobytes() + b.tobytes()
 
 
+@pytest.mark.parametrize("form,block", [("gz", 1 << 20), ("bgzf", 65280)])
+def test_sized_member_streams(gpu, form, block):
+    """Members that carry their size (gzippy's "GZ" subfield, BGZF's "BC")
+    decode together in one pipeline; a bad member is still reported."""
+    raw = W.generate_mixed(6 << 20, 21, seg=768 << 10)
+    gz = W.sized_member_stream(raw, block, 6, form)
+    st = {}
+    out = gpu.decompress(gz, stats=st)
+    say("sized", form, len(gz), {k: st[k] for k in ("pieces", "inflate_launches", "launches")})
+    assert out == raw.tobytes()
+    assert st["inflate_launches"] <= 4  # one pipeline, not one per member
+    sink = io.BytesIO()
+    assert gpu.decompress_to_writer(gz, sink) == raw.size and sink.getvalue() == raw.tobytes()
+    # a CRC byte of a middle member: the member-by-member walk reports it
+    bad = bytearray(gz)
+    members = len(range(0, raw.size, block))
+    mid = W.sized_member_stream(raw[: (members // 2) * block], block, 6, form)
+    mid_len = len(mid) - (len(W.BGZF_EOF) if form == "bgzf" else 0)
+    bad[mid_len - 8] ^= 0x10  # CRC32 of the member ending at mid_len
+    with pytest.raises(gpu.GzippyError) as e:
+        gpu.decompress(bytes(bad))
+    assert e.value.kind == "CrcMismatch"
+
+
 def test_corruption_is_an_error(gpu):
     raw, gz = W.text_member(1 << 20, seed=11)
     bad = gz.copy()

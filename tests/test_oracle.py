@@ -92,6 +92,18 @@ def test_header_fields(oracle):
     assert gzip.decompress(gz) == raw
 
 
+def test_sized_member_streams(oracle):
+    """gzippy's "GZ" members and BGZF members (with its empty end-of-file
+    member) are ordinary multi-member streams to the oracle."""
+    import gzip
+    raw = W.generate_mixed(2 << 20, 3, seg=256 << 10)
+    for form, block in (("gz", 300_000), ("bgzf", 65280)):
+        gz = W.sized_member_stream(raw, block, 6, form)
+        assert gzip.decompress(gz) == raw.tobytes()
+        st, out, _ = oracle.gunzip(gz)
+        assert st == oracle.OK and out.tobytes() == raw.tobytes(), form
+
+
 # ---------------------------------------------------------------- golden vectors
 
 def test_silesia_golden_vectors(oracle):
