@@ -73,6 +73,30 @@ def measured_traffic(args, kernel):
     return {"bytes_per_launch": int(got["traffic_bytes_per_launch"]), "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
 
 
+def host_leg(gz, n, ctx, runs=3):
+    """Host buffers in and out (gzgpu_decompress): H2D copy of the member,
+    the decode, D2H through pinned staging chunks into a caller buffer.
+    PCIe-inclusive; not the headline value."""
+    import ctypes
+
+    import numpy as np
+    from gzippy_amd import _native
+    a = np.ascontiguousarray(gz)
+    out = np.ones(n + 64, dtype=np.uint8)  # touched: no first-touch faults inside the timing
+    got = ctypes.c_size_t()
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        st = ctx.lib.gzgpu_decompress(ctx.h, a.ctypes.data, a.size, out.ctypes.data, out.size, ctypes.byref(got), None)
+        times.append(time.perf_counter() - t0)
+        if st != _native.OK or got.value != n:
+            log(f"host leg: status {st}, {got.value} bytes")
+            return None
+    best = min(times)
+    return {"api": "gzgpu_decompress (host in, host out)", "runs": runs, "best_ms": round(best * 1e3, 2),
+            "gbs_decompressed": round(n / best / 1e9, 2), "note": "includes H2D and D2H over the host link"}
+
+
 def batch_leg(args, dev, ctx):
     """10,000 independent members (lognormal sizes around 64 KiB, four content
     kinds, levels 1/6/9): one batched call versus one call per member."""
@@ -408,6 +432,8 @@ def main():
                      "inflate_stage": {"ms_per_step": round(infl_ms_step, 3),
                                        "achieved_c_plus_d_gbs": round(stage_gbs, 2)}},
     }
+    line["host_buffers"] = host_leg(gz, n, ctx)
+    log(f"host buffers: {line['host_buffers']}")
     if args.batch_members:
         del d_in, d_out
         torch.cuda.empty_cache()

@@ -293,6 +293,10 @@ struct gzgpu_ctx {
     std::vector<DevBuf> extra_slabs;
     PinnedArena pin;
     hipEvent_t ev[10]{};
+    // host-buffer output: two pinned staging chunks, D2H of one while the
+    // sink takes the other
+    uint8_t *drain[2]{};
+    hipEvent_t drain_ev[2]{};
     gzgpu_stats st{};
 };
 
@@ -1353,6 +1357,41 @@ void init_constants(gzgpu_ctx *c) {
     (void)c;
 }
 
+// Hand `n` bytes of device output to the sink in order, in DRAIN_CHUNK
+// pieces through two pinned staging buffers: the D2H copy of the next piece
+// runs while the sink takes the current one (in-order emission while the
+// next chunk is in flight, drain_one_pending chunk_fetcher.rs:2847).
+constexpr uint64_t DRAIN_CHUNK = 8 * MIB;
+template <typename Sink>
+int drain_to_sink(gzgpu_ctx *c, const uint8_t *d, uint64_t n, Sink &sink) {
+    if (!n) return 0;
+    for (int i = 0; i < 2; i++) {
+        if (!c->drain[i]) {
+            if (hipHostMalloc((void **)&c->drain[i], DRAIN_CHUNK, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                c->drain[i] = nullptr;
+                throw StatusError{GZGPU_GPU_UNAVAILABLE, "hipHostMalloc(output staging) failed"};
+            }
+        }
+        if (!c->drain_ev[i]) HIPCHK(hipEventCreateWithFlags(&c->drain_ev[i], hipEventDisableTiming));
+    }
+    const uint64_t chunks = (n + DRAIN_CHUNK - 1) / DRAIN_CHUNK;
+    auto enqueue = [&](uint64_t i) {
+        const uint64_t off = i * DRAIN_CHUNK, len = std::min(DRAIN_CHUNK, n - off);
+        HIPCHK(hipMemcpyAsync(c->drain[i & 1], d + off, len, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(c->drain_ev[i & 1], c->stream));
+    };
+    enqueue(0);
+    int rc = 0;
+    for (uint64_t i = 0; i < chunks; i++) {
+        // buffer (i + 1) & 1 held chunk i - 1, which the sink has finished
+        if (i + 1 < chunks) enqueue(i + 1);
+        HIPCHK(hipEventSynchronize(c->drain_ev[i & 1]));
+        if (!rc) rc = sink(c->drain[i & 1], (size_t)std::min(DRAIN_CHUNK, n - i * DRAIN_CHUNK));
+    }
+    return rc;
+}
+
 // Compressed size of the member at `p`, from its FEXTRA subfield: gzippy's
 // "GZ" (4 bytes: the whole member; 2 bytes: size - 1; gz_member_len,
 // src/decompress/bgzf.rs:2322-2357) or BGZF's "BC" (2 bytes: size - 1).
@@ -1403,7 +1442,6 @@ int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink 
     if (in_len < 2 || in[0] != 0x1f || in[1] != 0x8b) return GZGPU_OK;  // non-gzip: Ok(0)
     c->d_in.ensure(in_len + 64);
     HIPCHK(hipMemcpyAsync(c->d_in.p, in, in_len, hipMemcpyHostToDevice, c->stream));
-    std::vector<uint8_t> host_out;
     const std::vector<uint64_t> moff = sized_member_walk(in, in_len);
     if (moff.size() > 2) {
         gzgpu_tuning tn;
@@ -1436,11 +1474,7 @@ int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink 
                         all ? "all ok" : "first failure: ", all ? "" : mem[bad].msg.c_str());
             }
             if (all) {
-                host_out.resize(out_total);
-                if (out_total)
-                    HIPCHK(hipMemcpyAsync(host_out.data(), c->d_out.p, out_total, hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(hipStreamSynchronize(c->stream));
-                int sr = sink(host_out.data(), (size_t)out_total);
+                int sr = drain_to_sink(c, c->d_out.as<uint8_t>(), out_total, sink);
                 if (sr) return sr;
                 *total = out_total;
                 c->st.compressed_bytes = in_len;
@@ -1502,11 +1536,7 @@ int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink 
                 M.msg = "truncated gzip trailer";
             }
             if (M.status != GZGPU_OK) return finish(c, M.status, M.msg);
-            host_out.resize(M.out_len);
-            if (M.out_len)
-                HIPCHK(hipMemcpyAsync(host_out.data(), M.out, M.out_len, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-            int sr = sink(host_out.data(), (size_t)M.out_len);
+            int sr = drain_to_sink(c, M.out, M.out_len, sink);
             if (sr) return sr;
             *total += M.out_len;
             off = tr + 8;
@@ -1572,6 +1602,10 @@ void gzgpu_ctx_destroy(gzgpu_ctx *c) {
     c->pin.release();
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; i++) {
+        if (c->drain_ev[i]) (void)hipEventDestroy(c->drain_ev[i]);
+        if (c->drain[i]) (void)hipHostFree(c->drain[i]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

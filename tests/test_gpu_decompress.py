@@ -100,6 +100,25 @@ def test_writer_api(gpu):
     assert n == raw.size and sink.getvalue() == raw.tobytes()
 
 
+def test_writer_gets_output_in_order_chunks(gpu):
+    """A large member reaches the writer as consecutive chunks, in order."""
+    raw, gz = W.text_member(21 << 20, seed=13)
+
+    class Rec:
+        def __init__(self):
+            self.parts = []
+
+        def write(self, b):
+            self.parts.append(bytes(b))
+            return len(b)
+
+    rec = Rec()
+    n = gpu.decompress_to_writer(gz, rec)
+    say("writer calls", [len(p) for p in rec.parts])
+    assert n == raw.size and len(rec.parts) >= 3
+    assert b"".join(rec.parts) == raw.tobytes()
+
+
 def test_small_and_edge_members(gpu):
     for raw in [b"", b"a", b"ab" * 3, bytes(range(256)) * 300]:
         for level in (0, 1, 6, 9):
