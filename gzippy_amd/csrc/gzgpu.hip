@@ -288,6 +288,7 @@ struct gzgpu_ctx {
     std::string err;
     DevBuf d_lists;     // resolve and CRC lists of the current call, one upload
     DevBuf d_tile_crc;  // CRC32 of each resolve tile, from k_resolve_body
+    DevBuf d_scan;      // end windows of the parallel-prefix window chain (two buffers)
     DevBuf d_in, d_out, d_pieces, d_pout, d_order, d_counters, d_surv, d_hits, d_slab, d_crc_tab, d_seg_off,
         d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_env, d_env_off, d_env_len, d_bf_ranges, d_seg_tail;
     std::vector<DevBuf> extra_slabs;
@@ -514,6 +515,15 @@ uint64_t member_cost(uint64_t cbits, uint32_t exp_isize) {
 uint64_t marker_slab_bytes(uint64_t expect_entries) { return round_up(2 * (expect_entries + 64 * KIB) + 512, 256); }
 
 // Starts inside blocks (MIDBLOCK pieces); GZGPU_MIDBLOCK=0 turns them off.
+// Chains longer than this resolve their end windows as a parallel prefix
+// (k_chain_scan_*) instead of piece by piece (k_chain).  k_chain costs about
+// 17 us per piece of a chain; a prefix step about 0.1 ms over all slots.
+// GZGPU_CHAIN_SCAN_MIN overrides it (0: every chain, a large value: none).
+uint32_t chain_scan_min() {
+    const char *e = getenv("GZGPU_CHAIN_SCAN_MIN");
+    return e && *e ? (uint32_t)strtoul(e, nullptr, 10) : 16u;
+}
+
 bool midblock_enabled() {
     static int v = -1;
     if (v < 0) {
@@ -1203,6 +1213,19 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     if (!crc_members.empty()) mseg.push_back((uint32_t)seg_off.size());
     const uint32_t nseg = (uint32_t)seg_off.size(), nm = (uint32_t)crc_members.size();
 
+    // long chains: slots of the parallel-prefix window chain
+    std::vector<uint32_t> slot_rp, slot_pos;
+    uint32_t max_chain = 0;
+    for (size_t i = 0; i + 1 < chains.size(); i += 2) max_chain = std::max(max_chain, chains[i + 1] - chains[i]);
+    const bool chain_scan = max_chain > chain_scan_min();
+    if (chain_scan) {
+        for (size_t i = 0; i + 1 < chains.size(); i += 2)
+            for (uint32_t q = chains[i]; q < chains[i + 1]; q++) {
+                slot_rp.push_back(q);
+                slot_pos.push_back(q - chains[i]);
+            }
+    }
+
     // ---- every resolve and CRC list in one staging buffer and one copy:
     // small copies between the kernels each left the GPU idle for a while
     struct Part {
@@ -1213,7 +1236,8 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                     {tile_first.data(), tile_first.size() * 4, 0}, {seg_off.data(), seg_off.size() * 8, 0},
                     {seg_len.data(), seg_len.size() * 4, 0}, {seg_tail.data(), seg_tail.size() * 8, 0},
                     {mseg.data(), mseg.size() * 4, 0}, {mlen.data(), mlen.size() * 8, 0},
-                    {mtile.data(), mtile.size() * 4, 0}};
+                    {mtile.data(), mtile.size() * 4, 0},     {slot_rp.data(), slot_rp.size() * 4, 0},
+                    {slot_pos.data(), slot_pos.size() * 4, 0}};
     size_t lists_bytes = 0;
     for (auto &pt : parts) {
         pt.off = lists_bytes;
@@ -1233,7 +1257,28 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     if (!rps.empty()) {
         const uint32_t nch = (uint32_t)(chains.size() / 2);  // (first, end) pairs
         HIPCHK(hipEventRecord(c->ev[7], c->stream));
-        if (nch) {
+        if (nch && chain_scan) {
+            const uint32_t ns = (uint32_t)slot_rp.size();
+            c->d_scan.ensure((size_t)ns * WIN * 2 * 2);
+            uint16_t *wb[2] = {c->d_scan.as<uint16_t>(), c->d_scan.as<uint16_t>() + (size_t)ns * WIN};
+            const dim3 g(std::min<uint32_t>(ns, (uint32_t)c->num_cus * 8));
+            hipLaunchKernelGGL(k_chain_scan_init, g, dim3(256), 0, c->stream, (const ResolvePiece *)part(0),
+                               (const uint32_t *)part(9), (const uint32_t *)part(10), ns, wb[0]);
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+            int cur = 0;
+            for (uint32_t step = 1; step < max_chain; step *= 2) {
+                hipLaunchKernelGGL(k_chain_scan_step, g, dim3(256), 0, c->stream, (const uint32_t *)part(10), ns, step,
+                                   (const uint16_t *)wb[cur], wb[cur ^ 1]);
+                HIPCHK(hipGetLastError());
+                c->st.launches++;
+                cur ^= 1;
+            }
+            hipLaunchKernelGGL(k_chain_scan_out, g, dim3(256), 0, c->stream, (const ResolvePiece *)part(0),
+                               (const uint32_t *)part(9), ns, (const uint16_t *)wb[cur]);
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+        } else if (nch) {
             hipLaunchKernelGGL(k_chain, dim3(std::min<uint32_t>(nch, (uint32_t)c->num_cus * 4)), dim3(1024), 0,
                                c->stream, (const ResolvePiece *)part(0), (const uint32_t *)part(1), nch);
             HIPCHK(hipGetLastError());
@@ -1292,8 +1337,9 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     if (debug_enabled() && !rps.empty())
-        fprintf(stderr, "[gzgpu] resolve: chain %.3f ms (%zu pieces), body %.3f ms (%zu tiles)\n",
-                ev_ms(c->ev[7], c->ev[8]), rps.size(), ev_ms(c->ev[8], c->ev[3]), (size_t)ntiles);
+        fprintf(stderr, "[gzgpu] resolve: chain %.3f ms (%zu pieces, longest chain %u%s), body %.3f ms (%zu tiles)\n",
+                ev_ms(c->ev[7], c->ev[8]), rps.size(), max_chain, chain_scan ? ", parallel prefix" : "",
+                ev_ms(c->ev[8], c->ev[3]), (size_t)ntiles);
     c->st.ms_find += ev_ms(c->ev[0], c->ev[1]);
     c->st.ms_inflate += ev_ms(c->ev[1], c->ev[2]);
     c->st.ms_resolve += ev_ms(c->ev[2], c->ev[3]);
@@ -1595,7 +1641,7 @@ void gzgpu_ctx_destroy(gzgpu_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf *b : {&c->d_in, &c->d_out, &c->d_pieces, &c->d_pout, &c->d_order, &c->d_counters, &c->d_surv, &c->d_hits,
                       &c->d_slab, &c->d_crc_tab, &c->d_seg_off, &c->d_seg_len, &c->d_seg_crc, &c->d_mseg, &c->d_mlen,
-                      &c->d_mcrc, &c->d_env, &c->d_env_off, &c->d_env_len, &c->d_bf_ranges, &c->d_seg_tail, &c->d_lists,
+                      &c->d_mcrc, &c->d_env, &c->d_env_off, &c->d_env_len, &c->d_bf_ranges, &c->d_seg_tail, &c->d_lists, &c->d_scan,
                       &c->d_tile_crc})
         b->release();
     for (auto &b : c->extra_slabs) b.release();

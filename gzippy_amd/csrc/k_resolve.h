@@ -93,6 +93,81 @@ __global__ void __launch_bounds__(1024) k_chain(const ResolvePiece *__restrict__
     }
 }
 
+// ---- the window chain as a parallel prefix (long chains) -----------------
+// k_chain walks a chain piece by piece; a chain of thousands of pieces (a
+// member of a few hundred MiB, whose pieces are too short to flip) then takes
+// tens of milliseconds.  Instead: slot s holds the end window of piece
+// rp[slot_rp[s]], the slot_pos[s]-th piece of its chain (a chain's slots are
+// consecutive), as WIN u16 entries: < 256 a byte, MARKER_BASE + m byte m of
+// the window before that piece.  A chain's first piece is resolved at once
+// from its own window.  Step d (d = 1, 2, 4, ...) replaces every marker of
+// slot s (slot_pos[s] >= d) by entry m of slot s - d, the window its markers
+// refer to; after step d, slots at positions < 2d hold bytes only, so
+// ceil(log2(longest chain)) steps resolve every slot.  Double-buffered:
+// each step reads one buffer and writes the other.
+
+__global__ void __launch_bounds__(256) k_chain_scan_init(const ResolvePiece *__restrict__ rp,
+                                                         const uint32_t *__restrict__ slot_rp,
+                                                         const uint32_t *__restrict__ slot_pos, uint32_t nslots,
+                                                         uint16_t *__restrict__ W) {
+    for (uint32_t s = blockIdx.x; s < nslots; s += gridDim.x) {
+        const ResolvePiece r = rp[slot_rp[s]];
+        const uint64_t L = r.n16 + r.n8;
+        const bool first = slot_pos[s] == 0;
+        gptr<uint16_t> w = to_global(W) + (uint64_t)s * WIN;
+        for (uint32_t j = threadIdx.x; j < WIN; j += blockDim.x) {
+            const int64_t t = (int64_t)L - (int64_t)WIN + (int64_t)j;
+            uint32_t e;
+            if (t < 0) e = MARKER_BASE + (uint32_t)(L + j);  // window byte L + j (position t)
+            else if ((uint64_t)t < r.n16) e = ((gptr<const uint16_t>)to_global(r.slab))[t];
+            else e = to_global(r.slab)[r.u8_off + ((uint64_t)t - r.n16)];
+            if (first && e >= MARKER_BASE) {
+                const int64_t q = (int64_t)(e - MARKER_BASE) - (int64_t)WIN;
+                e = r.win_out && (int64_t)r.out_pos + q < 0 ? 0u : (uint32_t)to_global(r.win_end)[q];
+            }
+            w[j] = (uint16_t)e;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_chain_scan_step(const uint32_t *__restrict__ slot_pos, uint32_t nslots,
+                                                         uint32_t step, const uint16_t *__restrict__ Win,
+                                                         uint16_t *__restrict__ Wout) {
+    typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+    for (uint32_t s = blockIdx.x; s < nslots; s += gridDim.x) {
+        const bool compose = slot_pos[s] >= step;
+        gptr<const u16x8> src = (gptr<const u16x8>)(to_global(Win) + (uint64_t)s * WIN);
+        gptr<const uint16_t> prev = to_global(Win) + (uint64_t)(compose ? s - step : s) * WIN;
+        gptr<u16x8> dst = (gptr<u16x8>)(to_global(Wout) + (uint64_t)s * WIN);
+        for (uint32_t v = threadIdx.x; v < WIN / 8; v += blockDim.x) {
+            u16x8 x = src[v];
+            if (compose) {
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (x[k] >= MARKER_BASE) x[k] = prev[x[k] - MARKER_BASE];
+            }
+            dst[v] = x;
+        }
+    }
+}
+
+// Writes each slot's piece entries (the last WIN, or all of a shorter piece)
+// to the member output; every entry is a byte by now.
+__global__ void __launch_bounds__(256) k_chain_scan_out(const ResolvePiece *__restrict__ rp,
+                                                        const uint32_t *__restrict__ slot_rp, uint32_t nslots,
+                                                        const uint16_t *__restrict__ W) {
+    for (uint32_t s = blockIdx.x; s < nslots; s += gridDim.x) {
+        const ResolvePiece r = rp[slot_rp[s]];
+        const uint64_t L = r.n16 + r.n8;
+        gptr<const uint16_t> w = to_global(W) + (uint64_t)s * WIN;
+        gptr<uint8_t> mout = to_global(r.mout);
+        for (uint32_t j = threadIdx.x; j < WIN; j += blockDim.x) {
+            const int64_t t = (int64_t)L - (int64_t)WIN + (int64_t)j;
+            if (t >= 0) mout[r.out_pos + (uint64_t)t] = (uint8_t)w[j];
+        }
+    }
+}
+
 // Value of entry t of a piece for the body pass: window bytes come from the
 // 32 KiB before the piece (win_end; zeros before the member start).
 __device__ __forceinline__ uint8_t body_entry(const ResolvePiece &r, uint64_t t) {

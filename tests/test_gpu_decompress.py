@@ -93,6 +93,29 @@ def test_device_api(gpu):
     assert np.array_equal(out.cpu().numpy(), raw)
 
 
+@pytest.mark.parametrize("kind,level", [("text", 6), ("u32", 1), ("runs", 9)])
+def test_window_chain_prefix_equals_walk(gpu, kind, level):
+    """Pieces too short to flip form one long window chain: resolved as a
+    parallel prefix (every chain) and piece by piece (no chain), the output
+    is the same, byte-exact."""
+    import os
+    raw = W.generate(kind, 24 << 20, seed=17 + level)
+    gz = W.gzip_compress(raw, level)
+    old = os.environ.get("GZGPU_CHAIN_SCAN_MIN")
+    try:
+        for v in ("0", "1000000000"):
+            os.environ["GZGPU_CHAIN_SCAN_MIN"] = v
+            st = {}
+            out = gpu.decompress(gz, stats=st)
+            say("chain", kind, level, v, {k: st[k] for k in ("pieces", "launches", "ms_resolve")})
+            assert out == raw.tobytes(), v
+    finally:
+        if old is None:
+            os.environ.pop("GZGPU_CHAIN_SCAN_MIN", None)
+        else:
+            os.environ["GZGPU_CHAIN_SCAN_MIN"] = old
+
+
 def test_writer_api(gpu):
     raw, gz = W.text_member(2 << 20, seed=10)
     sink = io.BytesIO()
