@@ -1267,7 +1267,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             HIPCHK(hipGetLastError());
             c->st.launches++;
             int cur = 0;
-            for (uint32_t step = 1; step < max_chain; step *= 2) {
+            for (uint32_t step = 1; step < max_chain; step *= CHAIN_RADIX) {
                 hipLaunchKernelGGL(k_chain_scan_step, g, dim3(256), 0, c->stream, (const uint32_t *)part(10), ns, step,
                                    (const uint16_t *)wb[cur], wb[cur ^ 1]);
                 HIPCHK(hipGetLastError());

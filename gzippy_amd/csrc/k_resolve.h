@@ -100,11 +100,14 @@ __global__ void __launch_bounds__(1024) k_chain(const ResolvePiece *__restrict__
 // rp[slot_rp[s]], the slot_pos[s]-th piece of its chain (a chain's slots are
 // consecutive), as WIN u16 entries: < 256 a byte, MARKER_BASE + m byte m of
 // the window before that piece.  A chain's first piece is resolved at once
-// from its own window.  Step d (d = 1, 2, 4, ...) replaces every marker of
-// slot s (slot_pos[s] >= d) by entry m of slot s - d, the window its markers
-// refer to; after step d, slots at positions < 2d hold bytes only, so
-// ceil(log2(longest chain)) steps resolve every slot.  Double-buffered:
-// each step reads one buffer and writes the other.
+// from its own window.  Step d replaces every marker of slot s
+// (slot_pos[s] >= d) by entry m of slot s - d, the window its markers
+// refer to, then the markers left by entry m of slot s - 2d, and so on up
+// to slot s - (CHAIN_RADIX - 1) d (each window in the state before the
+// step); steps go d = 1, 4, 16, ...  After step d, slots at positions
+// < CHAIN_RADIX d hold bytes only, so ceil(log4(longest chain)) steps resolve
+// every slot.  Double-buffered: each step reads one buffer and writes the
+// other.
 
 __global__ void __launch_bounds__(256) k_chain_scan_init(const ResolvePiece *__restrict__ rp,
                                                          const uint32_t *__restrict__ slot_rp,
@@ -130,21 +133,29 @@ __global__ void __launch_bounds__(256) k_chain_scan_init(const ResolvePiece *__r
     }
 }
 
+constexpr uint32_t CHAIN_RADIX = 4;
+
 __global__ void __launch_bounds__(256) k_chain_scan_step(const uint32_t *__restrict__ slot_pos, uint32_t nslots,
                                                          uint32_t step, const uint16_t *__restrict__ Win,
                                                          uint16_t *__restrict__ Wout) {
     typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
     for (uint32_t s = blockIdx.x; s < nslots; s += gridDim.x) {
-        const bool compose = slot_pos[s] >= step;
+        // up to CHAIN_RADIX - 1 windows back: s - step, s - 2 step, ...
+        const uint32_t pos = slot_pos[s];
+        const uint32_t back = pos >= (CHAIN_RADIX - 1) * step ? CHAIN_RADIX - 1 : pos / step;
         gptr<const u16x8> src = (gptr<const u16x8>)(to_global(Win) + (uint64_t)s * WIN);
-        gptr<const uint16_t> prev = to_global(Win) + (uint64_t)(compose ? s - step : s) * WIN;
         gptr<u16x8> dst = (gptr<u16x8>)(to_global(Wout) + (uint64_t)s * WIN);
         for (uint32_t v = threadIdx.x; v < WIN / 8; v += blockDim.x) {
             u16x8 x = src[v];
-            if (compose) {
+            for (uint32_t b = 1; b <= back; b++) {
+                gptr<const uint16_t> prev = to_global(Win) + (uint64_t)(s - b * step) * WIN;
+                bool more = false;
 #pragma unroll
-                for (int k = 0; k < 8; k++)
+                for (int k = 0; k < 8; k++) {
                     if (x[k] >= MARKER_BASE) x[k] = prev[x[k] - MARKER_BASE];
+                    more |= x[k] >= MARKER_BASE;
+                }
+                if (!__any(more)) break;  // wave-uniform exit
             }
             dst[v] = x;
         }
