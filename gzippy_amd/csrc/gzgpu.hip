@@ -873,6 +873,12 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                     m, (unsigned long long)span, M.exp_isize, (unsigned long long)cost_of(M),
                     (unsigned long long)target_cost, (unsigned long long)spacing, (unsigned long long)np, hits.size(),
                     chosen.size());
+        uint64_t tail_hdr = M.ds;  // header of the member's last real hit
+        for (size_t j = hits.size(); j-- > 0;)
+            if (hits[j].kind == 2 || chained(j)) {
+                tail_hdr = hits[j].seek_bit;
+                break;
+            }
         for (size_t k = 0; k < chosen.size(); k++) {
             const Start &b = chosen[k];
             PieceDesc &prev = P[M.pieces.back()].d;
@@ -893,7 +899,14 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             c->st.midblock_pieces += b.mid ? 1u : 0u;
             q.acc_lo = b.acc_lo;  // MIDBLOCK: the sync point, known after the launch
             q.acc_hi = b.acc_hi;
-            uint64_t end = k + 1 < chosen.size() ? chosen[k + 1].acc_lo : M.de;
+            // a start inside the block of the member's last real hit may lie
+            // in the final block, which the finder never reports (it takes
+            // BFINAL = 0 headers only): its sync then fails and the piece
+            // before it runs to the member's end, so that piece's slab is
+            // sized for the rest of the member (no overflow and no second
+            // launch for it)
+            const bool tail_next = k + 1 < chosen.size() && chosen[k + 1].mid && chosen[k + 1].hdr == tail_hdr;
+            uint64_t end = k + 1 < chosen.size() && !tail_next ? chosen[k + 1].acc_lo : M.de;
             double frac = (double)(end - b.acc_lo) / (double)total_bits;
             q.d.cap = marker_slab_bytes((uint64_t)(1.3 * frac * (double)M.exp_isize));
             M.pieces.push_back((uint32_t)P.size());
