@@ -138,6 +138,30 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _check(t, name, dtypes=("uint8",), length=None):
+    """Device tensors handed to the kernels: CUDA, contiguous, of the expected
+    element type, and at least `length` bytes long (the kernels read and write
+    exactly what the lengths say)."""
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise GzippyError(f"{name} must be a CUDA tensor", kind="InvalidArgument")
+    if str(t.dtype).replace("torch.", "") not in dtypes:
+        raise GzippyError(f"{name} must be {'/'.join(dtypes)}, got {t.dtype}", kind="InvalidArgument")
+    if not t.is_contiguous():
+        raise GzippyError(f"{name} must be contiguous", kind="InvalidArgument")
+    if length is not None and (length < 0 or length > t.numel() * t.element_size()):
+        raise GzippyError(f"{name} holds {t.numel() * t.element_size()} bytes, {length} asked",
+                          kind="InvalidArgument")
+
+
+def _check_ranges(name, off, ln, t):
+    total = t.numel()
+    for o, n in zip(off, ln):
+        if int(o) + int(n) > total:
+            raise GzippyError(f"{name} range [{int(o)}, {int(o) + int(n)}) past the tensor's {total} bytes",
+                              kind="InvalidArgument")
+
+
 def _sync_torch(t):
     """The library runs on its own HIP stream: drain torch's current stream so
     work it queued on these tensors (copies, fills) is complete first."""
@@ -148,12 +172,14 @@ def _sync_torch(t):
 def decompress_device(d_in, d_out, in_len=None, ctx=None):
     """Decode one gzip member from device tensor `d_in` into `d_out`
     (uint8 CUDA tensors).  Returns (out_len, crc32, stats dict)."""
+    L = d_in.numel() if in_len is None else in_len
+    _check(d_in, "d_in", length=L)
+    _check(d_out, "d_out")
     ctx = ctx or context(d_in.device.index or 0)
     _sync_torch(d_in)
     n = ctypes.c_size_t()
     crc = ctypes.c_uint32()
     st_obj = Stats()
-    L = d_in.numel() if in_len is None else in_len
     st = ctx.lib.gzgpu_decompress_device(ctx.h, _ptr(d_in), L, _ptr(d_out), d_out.numel(), ctypes.byref(n),
                                          ctypes.byref(crc), ctypes.byref(st_obj))
     if st != _native.OK:
@@ -165,6 +191,10 @@ def decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None, ra
     """Decode n independent members in one persistent launch.
     Offsets/lengths are host integer sequences.  Returns (status[], out_len[],
     crc[], stats dict) as numpy arrays."""
+    _check(d_in, "d_in")
+    _check(d_out, "d_out")
+    _check_ranges("input", in_off, in_len, d_in)
+    _check_ranges("output", out_off, out_cap, d_out)
     ctx = ctx or context(d_in.device.index or 0)
     _sync_torch(d_in)
     n = len(in_off)
@@ -192,6 +222,10 @@ class BatchGraph:
     are decoded whole, one wavefront each."""
 
     def __init__(self, d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None):
+        _check(d_in, "d_in")
+        _check(d_out, "d_out")
+        _check_ranges("input", in_off, in_len, d_in)
+        _check_ranges("output", out_off, out_cap, d_out)
         self.ctx = ctx or context(d_in.device.index or 0)
         self._keep = (d_in, d_out)
         _sync_torch(d_in)
@@ -242,6 +276,10 @@ class BatchGraph:
 def autotune(d_in, in_off, in_len, d_out, out_off, out_cap, budget_ms=2000, ctx=None):
     """Pick the fastest tuning for these members (device time) and keep it in
     the context; returns (Tuning, ms)."""
+    _check(d_in, "d_in")
+    _check(d_out, "d_out")
+    _check_ranges("input", in_off, in_len, d_in)
+    _check_ranges("output", out_off, out_cap, d_out)
     ctx = ctx or context(d_in.device.index or 0)
     _sync_torch(d_in)
     a = [np.ascontiguousarray(np.asarray(x, dtype=np.uint64)) for x in (in_off, in_len, out_off, out_cap)]
@@ -259,9 +297,10 @@ def autotune(d_in, in_off, in_len, d_out, out_off, out_cap, budget_ms=2000, ctx=
 def find_blocks(d_in, start_bit=0, end_bit=None, in_len=None, ctx=None):
     """Block-finder hits in [start_bit, end_bit) of a device buffer, sorted:
     list of (bit_offset, seek_bit, kind)."""
+    L = d_in.numel() if in_len is None else in_len
+    _check(d_in, "d_in", length=L)
     ctx = ctx or context(d_in.device.index or 0)
     _sync_torch(d_in)
-    L = d_in.numel() if in_len is None else in_len
     if end_bit is None:
         end_bit = L * 8
     cnt = ctypes.c_size_t()
@@ -278,10 +317,12 @@ def find_blocks(d_in, start_bit=0, end_bit=None, in_len=None, ctx=None):
 
 def crc32(d_data, length=None, ctx=None):
     """gzip CRC32 of a device buffer."""
+    L = d_data.numel() * d_data.element_size() if length is None else length
+    _check(d_data, "d_data", dtypes=("uint8", "int8", "int16", "uint16", "int32", "uint32", "int64", "uint64"),
+           length=L)
     ctx = ctx or context(d_data.device.index or 0)
     _sync_torch(d_data)
     c = ctypes.c_uint32()
-    L = d_data.numel() * d_data.element_size() if length is None else length
     st = ctx.lib.gzgpu_crc32(ctx.h, _ptr(d_data), L, ctypes.byref(c))
     if st != _native.OK:
         _raise(ctx, st)
@@ -292,10 +333,12 @@ def inflate_piece(d_in, start_bit, stop_bit, d_out, window=None, in_len=None, ct
     """One wavefront decodes raw DEFLATE from start_bit with the stop rule.
     window=None: marker mode, d_out is int16/uint16-sized entries; window =
     32768 host bytes: clean decode into uint8 d_out.  Returns PieceResult."""
+    L = d_in.numel() if in_len is None else in_len
+    _check(d_in, "d_in", length=L)
+    _check(d_out, "d_out", dtypes=("uint8",) if window is not None else ("int16", "uint16"))
     ctx = ctx or context(d_in.device.index or 0)
     _sync_torch(d_in)
     res = _native.PieceResult()
-    L = d_in.numel() if in_len is None else in_len
     w = None
     if window is not None:
         w = np.ascontiguousarray(np.frombuffer(bytes(window), dtype=np.uint8))

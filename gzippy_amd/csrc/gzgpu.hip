@@ -278,6 +278,7 @@ struct PieceHost {
     uint64_t acc_lo = 0, acc_hi = 0;  // accepted predecessor end range
     uint32_t member = 0;
     bool confirmed = false, dropped = false;
+    bool merged = false;  // folded into an earlier piece's re-decode: no output of its own
 };
 
 struct gzgpu_ctx {
@@ -494,6 +495,11 @@ constexpr double PIECES_PER_SLOT = 1.0, SLOT_FRACTION = 31.0 / 32.0;
 constexpr uint64_t MID_HEAD = 2048, MID_TAIL = 2048;
 // planner: piece-size slope along a member (see the grid in run_members_aligned)
 constexpr double TILT = 0.25;
+// speculation: from this launch on a miss re-decodes the member's whole
+// remainder as one piece; a member still unfinished after SPEC_MAX_ROUNDS
+// launches fails alone (cannot happen for a valid stream: the remainder
+// piece either reaches the final block or meets a real error)
+constexpr int SPEC_FALLBACK_ROUND = 4, SPEC_MAX_ROUNDS = 16;
 
 // Decode cost of a member in output-entry units.  A wavefront's decode time
 // is about max(40 ns per compressed bit, 11 ns per output entry) at full
@@ -805,6 +811,20 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 if (k->seek_bit >= next) return true;
             return false;
         };
+        // Dynamic-header hits inside the payload of a chained stored block
+        // are data, not blocks of this stream (a gzip of deflate data: .gz,
+        // .zip, .png): every piece started there would miss.  They are
+        // shadowed for planning.
+        std::vector<uint8_t> shadow(hits.size(), 0);
+        for (size_t i = 0; i < hits.size(); i++) {
+            if (hits[i].kind != 0 || !chained(i)) continue;
+            const uint64_t o = hits[i].seek_bit + 3;
+            const uint64_t pay = (o / 8 + 4) * 8, next = (o / 8 + 4 + (hits[i].info >> 16)) * 8;
+            for (auto k = std::lower_bound(hits.begin(), hits.end(), pay, hit_lt); k != hits.end() && k->bit_offset < next;
+                 ++k)
+                if (k->kind == 2) shadow[(size_t)(k - hits.begin())] = 1;
+        }
+        auto dyn_hit = [&](size_t j) { return hits[j].kind == 2 && !shadow[j]; };
         // grid points: the starts of pieces 1 .. np-1.  Piece sizes fall
         // linearly along the member (density 1 - TILT x over x in [0, 1]):
         // pieces are pulled largest first, so the member's first pieces go
@@ -830,7 +850,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             uint64_t bestd = ~0ull;
             for (int pass = 0; pass < 2 && best == hits.size(); pass++)
                 for (size_t j = h; j < hits.size() && hits[j].bit_offset < hi; j++) {
-                    const bool ok = pass == 0 ? hits[j].kind == 2 : hits[j].kind == 0 && chained(j);
+                    const bool ok = pass == 0 ? dyn_hit(j) : hits[j].kind == 0 && chained(j);
                     if (!ok) continue;
                     const uint64_t d = hits[j].bit_offset > g ? hits[j].bit_offset - g : g - hits[j].bit_offset;
                     if (d < bestd) {
@@ -838,13 +858,13 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                         best = j;
                     }
                 }
-            if (midblock_enabled() && (best == hits.size() || bestd > rest / 8)) {
+            if (midblock_enabled() && !c->tune.miss_every && (best == hits.size() || bestd > rest / 8)) {
                 // the block holding g starts at the last hit at or before g
                 // (the member's first block when there is none); a start
                 // needs MID_HEAD bits after that header hit, for the header
                 // itself, and MID_TAIL bits before the next hit.  Stored hits
                 // that are not chained are LEN/NLEN coincidences and skipped.
-                auto real = [&](size_t j) { return hits[j].kind == 2 || chained(j); };
+                auto real = [&](size_t j) { return dyn_hit(j) || (hits[j].kind == 0 && chained(j)); };
                 while (hk < hits.size() && hits[hk].bit_offset <= g) hk++;  // grid points increase
                 size_t k = hk;
                 size_t prev = k;
@@ -875,10 +895,13 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                     chosen.size());
         uint64_t tail_hdr = M.ds;  // header of the member's last real hit
         for (size_t j = hits.size(); j-- > 0;)
-            if (hits[j].kind == 2 || chained(j)) {
+            if (dyn_hit(j) || (hits[j].kind == 0 && chained(j))) {
                 tail_hdr = hits[j].seek_bit;
                 break;
             }
+        if (c->tune.miss_every)  // test hook: every n-th start one bit late, a certain miss
+            for (size_t k = c->tune.miss_every - 1; k < chosen.size(); k += c->tune.miss_every)
+                chosen[k].start = chosen[k].acc_lo = chosen[k].acc_hi = chosen[k].start + 1;
         for (size_t k = 0; k < chosen.size(); k++) {
             const Start &b = chosen[k];
             PieceDesc &prev = P[M.pieces.back()].d;
@@ -938,31 +961,50 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     std::vector<uint32_t> todo(P.size());
     for (uint32_t i = 0; i < P.size(); i++) todo[i] = i;
     c->st.pieces = (uint32_t)P.size();
-    std::vector<uint8_t> linked(P.size(), 0);
+    // linked[k]: piece k starts where the piece before it (the nearest one
+    // not merged away) ended, by the latest results.  A member is decoded
+    // when every piece is linked: by induction from piece 0 (the deflate
+    // start) each piece then starts on the true chain of block boundaries.
+    std::vector<uint8_t> linked(P.size(), 0), in_todo(P.size(), 0);
     int rounds = 0;
     double tpost = 0;
     while (!todo.empty()) {
-        if (++rounds > 64) throw StatusError{GZGPU_DECODE_FAILED, "speculation did not converge"};
+        ++rounds;
         if (rounds > 1) c->st.redecoded += (uint32_t)todo.size();
         launch_inflate(c, d_in, in_len, P, mem, todo, R);
         if (debug_enabled()) tpost = host_ms();
+        for (uint32_t pi : todo) in_todo[pi] = 0;
         todo.clear();
+        auto push_todo = [&](uint32_t pi) {
+            if (!in_todo[pi]) {
+                in_todo[pi] = 1;
+                todo.push_back(pi);
+            }
+        };
 
-        // Speculation check.  Piece k is linked when piece k-1 stopped inside
-        // k's accepted start range.  Every unlinked piece whose predecessor is
-        // itself linked (so most likely decoded from a true boundary) is
-        // re-decoded from the predecessor's end in the next launch — all
-        // misses of a round at once.
+        // Speculation check (chunk_fetcher.rs:2506-2729 semantics: a start is
+        // accepted only where the decode before it ends).  Every piece that
+        // missed is re-decoded from its predecessor's end when that
+        // predecessor stopped cleanly, all in the next launch, together with
+        // the following pieces up to the next start that looks true: a run
+        // of false starts costs one more launch, not one per piece.  A
+        // re-decode from a wrong end is itself re-checked in the next round.
+        // From round SPEC_FALLBACK_ROUND on, a miss takes the member's whole
+        // remainder, so every valid stream finishes within a few launches.
         uint64_t extra_need = 0;
         std::vector<uint32_t> grow;
         for (uint32_t m = 0; m < mem.size(); m++) {
             Member &M = mem[m];
             if (M.status != GZGPU_OK || M.pieces.empty()) continue;
+            const size_t np = M.pieces.size();
+            // pass 1: local links against the current predecessor
             linked[M.pieces[0]] = 1;
-            for (size_t k = 1; k < M.pieces.size(); k++) {
-                uint32_t pi = M.pieces[k];
+            uint32_t prev = M.pieces[0];
+            for (size_t k = 1; k < np; k++) {
+                const uint32_t pi = M.pieces[k];
                 PieceHost &p = P[pi];
                 if (p.dropped) break;
+                if (p.merged) continue;
                 if (p.d.flags & PF_MIDBLOCK) {
                     // the start is the sync point the piece found; none: unlinkable
                     const PieceOut &me = R[pi];
@@ -970,57 +1012,94 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                     p.acc_lo = synced ? me.start_bit : 1;
                     p.acc_hi = synced ? me.start_bit : 0;
                 }
-                const PieceOut &pr = R[M.pieces[k - 1]];
-                if (pr.status == PS_FINAL) {
-                    if (linked[M.pieces[k - 1]])
-                        for (size_t j = k; j < M.pieces.size(); j++) P[M.pieces[j]].dropped = true;
-                    linked[pi] = 0;
-                    if (linked[M.pieces[k - 1]]) break;
-                    continue;
-                }
-                if (pr.status != PS_STOPPED) {
-                    linked[pi] = 0;
-                    continue;
-                }
-                if (pr.end_bit >= p.acc_lo && pr.end_bit <= p.acc_hi) {
-                    if ((p.d.flags & PF_MIDBLOCK) && !linked[pi]) c->st.midblock_linked++;
-                    linked[pi] = 1;
-                } else {
-                    linked[pi] = 0;
-                    if (linked[M.pieces[k - 1]]) {
-                        if (debug_enabled())
-                            fprintf(stderr, "[gzgpu] miss: member %u piece %zu start [%llu,%llu] prev end %llu\n", m, k,
-                                    (unsigned long long)p.acc_lo, (unsigned long long)p.acc_hi,
-                                    (unsigned long long)pr.end_bit);
-                        p.d.start_bit = p.acc_lo = p.acc_hi = pr.end_bit;
-                        p.d.flags &= ~PF_MIDBLOCK;
-                        p.d.hdr_bit = 0;
-                        todo.push_back(pi);
-                        // a redo that starts at or past its own stop bit stops
-                        // at once (the header there is non-final: the
-                        // predecessor stopped before it), so the piece after
-                        // it starts at that same bit, in this launch too
-                        while (k + 1 < M.pieces.size() && P[M.pieces[k]].d.stop_bit <= pr.end_bit &&
-                               !P[M.pieces[k + 1]].dropped) {
-                            PieceHost &q = P[M.pieces[++k]];
-                            q.d.start_bit = q.acc_lo = q.acc_hi = pr.end_bit;
-                            q.d.flags &= ~PF_MIDBLOCK;
-                            q.d.hdr_bit = 0;
-                            linked[M.pieces[k]] = 0;
-                            todo.push_back(M.pieces[k]);
-                        }
-                    }
-                }
+                const PieceOut &pr = R[prev];
+                const bool now = pr.status == PS_STOPPED && pr.end_bit >= p.acc_lo && pr.end_bit <= p.acc_hi;
+                if (now && !linked[pi] && (p.d.flags & PF_MIDBLOCK)) c->st.midblock_linked++;
+                linked[pi] = now ? 1 : 0;
+                prev = pi;
             }
-            // errors and overflows: fatal / regrow inside the linked prefix;
-            // overflowed speculative pieces are regrown too (most are valid)
-            bool prefix = true;
-            for (size_t k = 0; k < M.pieces.size(); k++) {
-                uint32_t pi = M.pieces[k];
+            // pass 2: the end of the member, misses
+            prev = M.pieces[0];
+            bool chain = true;  // every piece so far linked
+            for (size_t k = 1; k < np; k++) {
+                const uint32_t pi = M.pieces[k];
                 PieceHost &p = P[pi];
                 if (p.dropped) break;
-                prefix = prefix && linked[pi];
-                if (std::find(todo.begin(), todo.end(), pi) != todo.end()) continue;
+                if (p.merged) continue;
+                const PieceOut &pr = R[prev];
+                if (chain && pr.status == PS_FINAL) {
+                    // the member ends in the piece before: nothing after it
+                    for (size_t j = k; j < np; j++) P[M.pieces[j]].dropped = true;
+                    break;
+                }
+                if (!linked[pi] && pr.status == PS_STOPPED && !in_todo[prev]) {
+                    const uint64_t e = pr.end_bit;
+                    // the group [k, j): re-decoded as one piece from e, up to
+                    // the planned start of the next piece whose start looks
+                    // true: its predecessor's decode ends there, or its own
+                    // decode ran cleanly to its stop (a false start almost
+                    // always meets an invalid code or distance first)
+                    auto plausible = [&](size_t q) {
+                        const PieceHost &h = P[M.pieces[q]];
+                        const PieceOut &o = R[M.pieces[q]];
+                        return !h.merged && h.acc_lo > e &&
+                               (linked[M.pieces[q]] || o.status == PS_STOPPED || o.status == PS_FINAL);
+                    };
+                    size_t j = k + 1;
+                    if (rounds < SPEC_FALLBACK_ROUND) {
+                        while (j < np && !P[M.pieces[j]].dropped && !plausible(j)) j++;
+                    } else {
+                        j = np;
+                    }
+                    const bool to_end = j >= np || P[M.pieces[j]].dropped;
+                    if (debug_enabled())
+                        fprintf(stderr, "[gzgpu] miss: member %u piece %zu start [%llu,%llu] prev end %llu, redo %zu pieces\n",
+                                m, k, (unsigned long long)p.acc_lo, (unsigned long long)p.acc_hi,
+                                (unsigned long long)e, j - k);
+                    const PieceHost &last = P[M.pieces[j - 1]];
+                    uint64_t cap = 0;
+                    for (size_t q = k; q < j; q++) cap += P[M.pieces[q]].d.cap;
+                    p.d.stop_bit = to_end ? NO_STOP : last.d.stop_bit;
+                    p.d.stop_hdr = last.d.stop_hdr;
+                    p.d.flags = (p.d.flags & ~(PF_MIDBLOCK | PF_STOPMID)) | (to_end ? 0u : (last.d.flags & PF_STOPMID));
+                    p.d.start_bit = p.acc_lo = p.acc_hi = e;
+                    p.d.hdr_bit = 0;
+                    for (size_t q = k + 1; q < j; q++) {
+                        P[M.pieces[q]].merged = true;
+                        linked[M.pieces[q]] = 1;
+                    }
+                    if (!to_end) linked[M.pieces[j]] = 0;  // re-checked against the new end
+                    if (j > k + 1 || p.d.stop_bit == NO_STOP) {
+                        // the group's range in one slab (the tail piece's slab
+                        // already covers the member's rest)
+                        const uint64_t want = to_end ? marker_slab_bytes((uint64_t)(1.3 * (double)(M.de - e) /
+                                                                                     (double)std::max<uint64_t>(1, M.de - M.ds) *
+                                                                                     (double)M.exp_isize))
+                                                      : cap;
+                        p.d.cap = std::max(want, p.d.cap);
+                        extra_need += p.d.cap;
+                        grow.push_back(pi);
+                    }
+                    linked[pi] = 1;  // starts at e by construction
+                    push_todo(pi);
+                    k = j - 1;
+                    prev = pi;
+                    chain = false;
+                    continue;
+                }
+                chain = chain && linked[pi];
+                prev = pi;
+            }
+            // errors and overflows: fatal inside the linked prefix; overflowed
+            // speculative pieces are regrown (most are valid)
+            bool prefix = true;
+            for (size_t k = 0; k < np; k++) {
+                const uint32_t pi = M.pieces[k];
+                PieceHost &p = P[pi];
+                if (p.dropped) break;
+                if (p.merged) continue;
+                prefix = prefix && linked[pi] && !in_todo[pi];
+                if (in_todo[pi]) continue;
                 const PieceOut &r = R[pi];
                 if (prefix && r.status == PS_ERROR) {
                     M.status = GZGPU_DECODE_FAILED;
@@ -1041,14 +1120,32 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                     p.d.cap = std::max<uint64_t>(marker_slab_bytes(want), 2 * p.d.cap);
                     extra_need += p.d.cap;
                     grow.push_back(pi);
-                    todo.push_back(pi);
+                    push_todo(pi);
                 }
             }
-            if (M.status != GZGPU_OK)
-                todo.erase(std::remove_if(todo.begin(), todo.end(), [&](uint32_t pi) { return P[pi].member == m; }),
+            if (M.status == GZGPU_OK && rounds >= SPEC_MAX_ROUNDS) {
+                bool pending = false;
+                for (size_t k = 0; k < np && !pending; k++) pending = in_todo[M.pieces[k]] != 0;
+                if (pending) {
+                    M.status = GZGPU_DECODE_FAILED;
+                    M.msg = "speculation did not converge in " + std::to_string(rounds) + " launches";
+                }
+            }
+            if (M.status != GZGPU_OK) {
+                todo.erase(std::remove_if(todo.begin(), todo.end(),
+                                          [&](uint32_t pi) {
+                                              if (P[pi].member != m) return false;
+                                              in_todo[pi] = 0;
+                                              return true;
+                                          }),
                            todo.end());
+                grow.erase(std::remove_if(grow.begin(), grow.end(), [&](uint32_t pi) { return P[pi].member == m; }),
+                           grow.end());
+            }
         }
         if (!grow.empty()) {
+            extra_need = 0;
+            for (uint32_t pi : grow) extra_need += P[pi].d.cap;
             DevBuf b;
             b.ensure(extra_need);
             uint64_t off = 0;
@@ -1086,6 +1183,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         for (size_t k = 0; k < M.pieces.size(); k++) {
             uint32_t pi = M.pieces[k];
             if (P[pi].dropped) break;
+            if (P[pi].merged) continue;
             if (!linked[pi]) {
                 M.status = GZGPU_DECODE_FAILED;
                 M.msg = "speculative piece " + std::to_string(k) + " never linked to its predecessor";
@@ -1764,14 +1862,23 @@ int gzgpu_decompress_device(gzgpu_ctx *c, const void *d_in, size_t in_len, void 
         if (in_len < 2 || h[0] != 0x1f || h[1] != 0x8b) return (int)GZGPU_OK;
         size_t hl;
         int st = parse_header(h.data(), head, &hl);
+        const uint32_t exp_crc = le32(h.data() + head), exp_isize = le32(h.data() + head + 4);
+        // a long FEXTRA / FNAME / FCOMMENT: read more of the header
+        for (size_t more = head; st == GZGPU_INVALID_HEADER && more < in_len;) {
+            more = std::min<size_t>(in_len, 4 * more);
+            std::vector<uint8_t> big(more);
+            HIPCHK(hipMemcpyAsync(big.data(), d_in, more, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            st = parse_header(big.data(), more, &hl);
+        }
         if (st) return finish(c, st, "invalid gzip header");
         if (in_len < hl + 8) return finish(c, GZGPU_INVALID_FORMAT, "input shorter than header + trailer");
         Member M;
         M.ds = hl * 8;
         M.de = (in_len - 8) * 8;
         M.limit = in_len * 8;
-        M.exp_crc = le32(h.data() + head);
-        M.exp_isize = le32(h.data() + head + 4);
+        M.exp_crc = exp_crc;
+        M.exp_isize = exp_isize;
         M.out = (uint8_t *)d_out;
         M.out_cap = out_cap;
         M.split = true;

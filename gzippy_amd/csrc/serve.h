@@ -80,13 +80,26 @@ struct gzgpu_server {
 
 namespace {
 
+// ISIZE of a member span, or ~0 when it cannot belong to that span: deflate
+// expands at most 1032:1 (258-byte copies from 2-bit codes, the reference's
+// MAX_DEFLATE_EXPANSION, chunk_data.rs:334), so a larger ISIZE means a false
+// magic inside the payload cut the span short.
+uint64_t span_isize(const uint8_t *p, size_t len) {
+    if (len < 18) return ~0ull;
+    const uint64_t isz = le32(p + len - 4);
+    return isz > 1032ull * len + 64 ? ~0ull : isz;
+}
+
 uint64_t request_reserve(const ServeRequest &r) {
-    // input (16-byte padded) + expected output from the last trailer's ISIZE
-    // (a multi-member stream is re-estimated per member when split)
+    // input (16-byte padded), expected output from the members' trailers, and
+    // the speculative pieces' marker slabs (u16 entries until the flip, sized
+    // 1.3 x the output share: up to 2.6 x the output)
     uint64_t out = 0;
-    for (auto &sp : member_spans(r.in.data(), r.in.size()))
-        if (sp.second >= 8) out += le32(r.in.data() + sp.first + sp.second - 4);
-    return (r.in.size() + 16) + out + 4096;
+    for (auto &sp : member_spans(r.in.data(), r.in.size())) {
+        const uint64_t isz = span_isize(r.in.data() + sp.first, sp.second);
+        out += isz == ~0ull ? 1032ull * sp.second : isz;
+    }
+    return (r.in.size() + 16) + out + (out * 26) / 10 + 64 * 1024;
 }
 
 // Decode one admitted batch on context c.
@@ -103,6 +116,7 @@ void serve_batch(gzgpu_server *S, gzgpu_ctx *c, std::vector<std::shared_ptr<Serv
     };
     std::vector<Span> spans;
     std::vector<uint64_t> in_base(batch.size()), out_base(batch.size()), out_size(batch.size());
+    std::vector<uint8_t> req_bad(batch.size(), 0);
     uint64_t in_total = 0, out_total = 0;
     for (uint32_t i = 0; i < batch.size(); i++) {
         ServeRequest &r = *batch[i];
@@ -112,6 +126,14 @@ void serve_batch(gzgpu_server *S, gzgpu_ctx *c, std::vector<std::shared_ptr<Serv
         uint64_t osz = 0;
         auto v = member_spans(r.in.data(), r.in.size());
         r.members = (uint32_t)v.size();
+        bool plausible = true;
+        for (auto &sp : v) plausible = plausible && span_isize(r.in.data() + sp.first, sp.second) != ~0ull;
+        if (!plausible && r.in.size() >= 2 && r.in[0] == 0x1f && r.in[1] == 0x8b) {
+            // a false member boundary: straight to the member-by-member walk
+            req_bad[i] = 1;
+            out_size[i] = 0;
+            continue;
+        }
         for (auto &sp : v) {
             spans.push_back({i, sp.first, sp.second});
             if (sp.second >= 8) osz += le32(r.in.data() + sp.first + sp.second - 4);
@@ -128,7 +150,6 @@ void serve_batch(gzgpu_server *S, gzgpu_ctx *c, std::vector<std::shared_ptr<Serv
     // one Member per span
     std::vector<Member> mem;
     std::vector<uint32_t> mem_req;
-    std::vector<uint8_t> req_bad(batch.size(), 0);
     {
         std::vector<uint64_t> out_next(out_base);
         for (auto &sp : spans) {

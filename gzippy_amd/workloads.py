@@ -113,6 +113,89 @@ def gzip_compress(data, level=6, encoder=None):
     return np.frombuffer(body, dtype=np.uint8).copy()
 
 
+def _crc_mulmod(a, b):
+    """a * b mod P in the reflected CRC-32 polynomial basis."""
+    r = 0
+    m = 1 << 31
+    while m:
+        if a & m:
+            r ^= b
+        b = (b >> 1) ^ 0xEDB88320 if b & 1 else b >> 1
+        m >>= 1
+    return r
+
+
+def crc32_combine(crc_a, crc_b, len_b):
+    """CRC32 of A || B from crc(A), crc(B) and |B|: crc_a * x^(8 |B|) mod P,
+    xor crc_b (the standard combine; x^k by square-and-multiply)."""
+    p = 1 << 31  # x^0
+    sq = 1 << 30  # x^1
+    k = 8 * len_b
+    while k:
+        if k & 1:
+            p = _crc_mulmod(p, sq)
+        sq = _crc_mulmod(sq, sq)
+        k >>= 1
+    return _crc_mulmod(p, crc_a) ^ crc_b
+
+
+def crc32_parallel(a, seg=SEG, threads=16):
+    """zlib CRC32 of a large uint8 array, segments in threads, then combined."""
+    a = np.ascontiguousarray(a)
+    offs = list(range(0, a.size, seg)) or [0]
+    with ThreadPoolExecutor(min(threads, len(offs))) as ex:
+        parts = list(ex.map(lambda o: (zlib.crc32(memoryview(a[o:o + seg])), min(seg, a.size - o)), offs))
+    crc = 0
+    for c, n in parts:
+        crc = crc32_combine(crc, c, n) if crc or n else c
+    return crc
+
+
+ENC_SEG = 8 << 20  # pigz-style encoder segment
+
+
+def gzip_compress_segments(data, level, seg=ENC_SEG, threads=16):
+    """One RFC 1952 member of `data` at `level`, encoded pigz-style in
+    parallel: zlib raw deflate per `seg`-byte segment, each primed with the
+    32 KiB before it (back-references cross the segment seams) and ended with
+    a sync flush (no BFINAL, byte-aligned), the last one finished.  The
+    concatenation is a single deflate stream, so the result is one member of
+    stored, fixed and dynamic blocks however large `data` is."""
+    a = np.ascontiguousarray(np.frombuffer(data, dtype=np.uint8) if not isinstance(data, np.ndarray) else data)
+    offs = list(range(0, a.size, seg)) or [0]
+
+    def one(i):
+        o = offs[i]
+        kw = {}
+        if o:
+            kw["zdict"] = bytes(a[max(0, o - 32768):o])
+        co = zlib.compressobj(int(level), zlib.DEFLATED, -15, 9, zlib.Z_DEFAULT_STRATEGY, **kw)
+        body = co.compress(memoryview(a[o:o + seg]))
+        body += co.flush(zlib.Z_FINISH if i == len(offs) - 1 else zlib.Z_SYNC_FLUSH)
+        return body
+
+    with ThreadPoolExecutor(min(threads, len(offs))) as ex:
+        bodies = list(ex.map(one, range(len(offs))))
+    crc = crc32_parallel(a, seg, threads)
+    head = b"\x1f\x8b\x08\x00\x00\x00\x00\x00\x00\xff"
+    out = np.empty(len(head) + sum(len(b) for b in bodies) + 8, dtype=np.uint8)
+    out[:len(head)] = np.frombuffer(head, np.uint8)
+    pos = len(head)
+    for b in bodies:
+        out[pos:pos + len(b)] = np.frombuffer(b, np.uint8)
+        pos += len(b)
+    out[pos:] = np.frombuffer(struct.pack("<II", crc, a.size & 0xFFFFFFFF), np.uint8)
+    return out
+
+
+def mixed_member(n, seed, level, seg=SEG, threads=16):
+    """Mixed-entropy single member: `seg`-byte segments cycling text / uniform
+    random / byte runs / u32 deltas, encoded pigz-style (zlib) at `level`.
+    Returns (raw, gz)."""
+    raw = generate_mixed(n, seed, seg, threads=threads)
+    return raw, gzip_compress_segments(raw, level, ENC_SEG, threads)
+
+
 def trailer(gz):
     """(crc32, isize) from a member's 8-byte trailer."""
     crc, isize = struct.unpack("<II", bytes(gz[-8:]))

@@ -58,6 +58,9 @@ typedef struct {
     uint32_t pieces_per_chunk;  /* wavefronts per chunk: sub-splits at found block boundaries */
     uint32_t waves_per_cu;      /* persistent-grid occupancy cap (0: as many as LDS allows) */
     uint64_t batch_split_bytes; /* batch: split only members above this compressed size (0: by expected output) */
+    uint32_t miss_every;        /* test hook: plan every n-th speculative start one bit late, so that its
+                                   predecessor's end misses it and the re-decode path runs (0: off) */
+    uint32_t reserved_;         /* 0 */
 } gzgpu_tuning;
 
 /* Per-call timing and shape, filled when a non-NULL pointer is passed. */
