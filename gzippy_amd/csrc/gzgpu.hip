@@ -65,14 +65,38 @@ struct StatusError {
     std::string msg;
 };
 
+// Device memory of a call without allocation (gzgpu_decompress_device_noalloc):
+// every buffer the call needs is carved from the device's `scratch` array,
+// bump-allocated, nothing freed.  Set for the duration of such a call on the
+// calling thread.
+struct ScratchArena {
+    uint8_t *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+thread_local ScratchArena *t_arena = nullptr;
+
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool borrowed = false;  // carved from a ScratchArena: never freed
     void ensure(size_t n) {
         if (n <= cap) return;
-        if (p) HIPCHK(hipFree(p));
+        if (t_arena) {
+            const size_t c = (std::max<size_t>(n, 4096) + 255) & ~(size_t)255;
+            if (t_arena->used + c > t_arena->cap)
+                throw StatusError{GZGPU_OUT_OF_DEVICE_MEMORY,
+                                  "no-allocation scratch exhausted: " + std::to_string(t_arena->used + c) + " of " +
+                                      std::to_string(t_arena->cap) + " bytes needed"};
+            p = t_arena->base + t_arena->used;
+            t_arena->used += c;
+            cap = c;
+            borrowed = true;
+            return;
+        }
+        if (p && !borrowed) HIPCHK(hipFree(p));
         p = nullptr;
         cap = 0;
+        borrowed = false;
         size_t c = std::max<size_t>(n, 4096);
         hipError_t e = hipMalloc(&p, c);
         if (e != hipSuccess) {
@@ -83,9 +107,10 @@ struct DevBuf {
         cap = c;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        borrowed = false;
     }
     template <typename T>
     T *as() const {
@@ -207,8 +232,19 @@ __global__ void k_gather_envelope(const uint8_t *__restrict__ in, const uint64_t
 struct PinnedArena {
     std::vector<std::pair<uint8_t *, size_t>> blocks;
     size_t used = 0;
+    // a fixed block instead (calls without allocation): bump only, no growth
+    uint8_t *fixed = nullptr;
+    size_t fixed_cap = 0, fixed_used = 0;
     void *alloc(size_t n) {
         n = (n + 63) & ~(size_t)63;
+        if (fixed) {
+            if (fixed_used + n > fixed_cap)
+                throw StatusError{GZGPU_OUT_OF_DEVICE_MEMORY, "no-allocation host staging exhausted (" +
+                                                                  std::to_string(fixed_cap) + " bytes)"};
+            void *r = fixed + fixed_used;
+            fixed_used += n;
+            return r;
+        }
         if (blocks.empty() || used + n > blocks.back().second) {
             size_t cap = std::max<size_t>(n, 1 << 20);
             if (!blocks.empty()) cap = std::max(cap, 2 * blocks.back().second);
@@ -226,6 +262,10 @@ struct PinnedArena {
     }
     // caller guarantees no copy from or to the arena is in flight
     void reset() {
+        if (fixed) {
+            fixed_used = 0;
+            return;
+        }
         if (blocks.size() > 1) {
             size_t want = 0;
             for (auto &b : blocks) want += b.second;
@@ -294,6 +334,11 @@ struct gzgpu_ctx {
         d_seg_len, d_seg_crc, d_mseg, d_mlen, d_mcrc, d_env, d_env_off, d_env_len, d_bf_ranges, d_seg_tail;
     std::vector<DevBuf> extra_slabs;
     PinnedArena pin;
+    // calls without allocation: the device's `scratch` array (its address on
+    // this context's device) and a pinned staging block made at creation
+    uint8_t *scratch = nullptr;
+    uint8_t *noalloc_stage = nullptr;
+    uint32_t max_pieces = 0;  // planner cap on pieces per call (0: none)
     hipEvent_t ev[10]{};
     // host-buffer output: two pinned staging chunks, D2H of one while the
     // sink takes the other
@@ -303,6 +348,15 @@ struct gzgpu_ctx {
 };
 
 namespace {
+
+// Every device buffer of a context (destroy, and the swap around a call
+// without allocation).
+std::vector<DevBuf *> ctx_bufs(gzgpu_ctx *c) {
+    return {&c->d_in,    &c->d_out,   &c->d_pieces,  &c->d_pout,      &c->d_order,   &c->d_counters, &c->d_surv,
+            &c->d_hits,  &c->d_slab,  &c->d_crc_tab, &c->d_seg_off,   &c->d_seg_len, &c->d_seg_crc,  &c->d_mseg,
+            &c->d_mlen,  &c->d_mcrc,  &c->d_env,     &c->d_env_off,   &c->d_env_len, &c->d_bf_ranges, &c->d_seg_tail,
+            &c->d_lists, &c->d_scan,  &c->d_tile_crc};
+}
 
 void set_err(gzgpu_ctx *c, const std::string &m) {
     if (c) c->err = m;
@@ -724,8 +778,9 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     uint64_t total_cost = 0;
     for (auto &M : mem)
         if (M.status == GZGPU_OK) total_cost += cost_of(M);
-    const uint64_t target_cost = std::max<uint64_t>(
+    uint64_t target_cost = std::max<uint64_t>(
         MIN_PIECE_OUT, (uint64_t)((double)total_cost / (PIECES_PER_SLOT * SLOT_FRACTION * (double)slots)));
+    if (c->max_pieces) target_cost = std::max<uint64_t>(target_cost, total_cost / c->max_pieces + 1);
     const uint64_t ppc = c->tune.pieces_per_chunk;
     auto spacing_of = [&](const Member &M) -> uint64_t {
         if (ppc) return std::max<uint64_t>(chunk_bits / ppc, min_spacing);
@@ -1486,12 +1541,12 @@ int finish(gzgpu_ctx *c, int st, const std::string &msg) {
 }
 
 template <typename F>
-int guarded(gzgpu_ctx *c, F &&f) {
+int guarded(gzgpu_ctx *c, F &&f, bool reset_staging = true) {
     try {
         // every call starts with an idle stream, so staging memory is free
         if (c && c->stream) {
             HIPCHK(hipStreamSynchronize(c->stream));
-            c->pin.reset();
+            if (reset_staging) c->pin.reset();
         }
         return f();
     } catch (const HipError &e) {
@@ -1710,6 +1765,35 @@ int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink 
 
 }  // namespace
 
+// ---- scratch of the calls without allocation ------------------------------
+// One array per device, held for the life of the process.  Calls that
+// overlap on one device share it and race: callers serialise them.
+namespace {
+constexpr size_t NOALLOC_SCRATCH_BYTES = 256ull << 20;  // device buffers of one call
+constexpr size_t NOALLOC_STAGE_BYTES = 8ull << 20;      // pinned host staging per context
+constexpr uint32_t NOALLOC_MAX_PIECES = 256;
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+// The array's address on `device` (the current device), looked up once per
+// device.  The name `scratch` in host code is a host-side stand-in, not a
+// device address: host code uses only this pointer.
+uint8_t *device_scratch(int device) {
+    static std::mutex mu;
+    static std::unordered_map<int, uint8_t *> ptrs;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = ptrs.find(device);
+    if (it != ptrs.end()) return it->second;
+    uint8_t *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw StatusError{GZGPU_GPU_UNAVAILABLE, "no address for the scratch array"};
+    ptrs[device] = ptr;
+    return ptr;
+}
+}  // namespace
+
 // =========================================================================
 // C ABI
 // =========================================================================
@@ -1735,6 +1819,8 @@ int gzgpu_ctx_create(int device, gzgpu_ctx **out) {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
         init_constants(c);
+        c->scratch = device_scratch(device);
+        HIPCHK(hipHostMalloc((void **)&c->noalloc_stage, NOALLOC_STAGE_BYTES, hipHostMallocDefault));
         return (int)GZGPU_OK;
     });
     if (st != GZGPU_OK) {
@@ -1750,11 +1836,8 @@ void gzgpu_ctx_destroy(gzgpu_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *b : {&c->d_in, &c->d_out, &c->d_pieces, &c->d_pout, &c->d_order, &c->d_counters, &c->d_surv, &c->d_hits,
-                      &c->d_slab, &c->d_crc_tab, &c->d_seg_off, &c->d_seg_len, &c->d_seg_crc, &c->d_mseg, &c->d_mlen,
-                      &c->d_mcrc, &c->d_env, &c->d_env_off, &c->d_env_len, &c->d_bf_ranges, &c->d_seg_tail, &c->d_lists, &c->d_scan,
-                      &c->d_tile_crc})
-        b->release();
+    for (DevBuf *b : ctx_bufs(c)) b->release();
+    if (c->noalloc_stage) (void)hipHostFree(c->noalloc_stage);
     for (auto &b : c->extra_slabs) b.release();
     c->pin.release();
     for (auto &e : c->ev)
@@ -1844,10 +1927,16 @@ int gzgpu_decompress_to_writer(gzgpu_ctx *c, const uint8_t *in, size_t in_len, g
     });
 }
 
-int gzgpu_decompress_device(gzgpu_ctx *c, const void *d_in, size_t in_len, void *d_out, size_t out_cap,
-                            size_t *out_len, uint32_t *crc_out, gzgpu_stats *stats) {
-    if (!c || !d_in || !d_out || !out_len) return GZGPU_INVALID_ARGUMENT;
-    return guarded(c, [&]() {
+}  // extern "C"
+
+namespace {
+
+// One member, device-resident in and out.  max_isize: members whose trailer
+// ISIZE exceeds it are refused before any device work (calls without
+// allocation).
+int decompress_device_member(gzgpu_ctx *c, const void *d_in, size_t in_len, void *d_out, size_t out_cap,
+                             size_t *out_len, uint32_t *crc_out, gzgpu_stats *stats, uint64_t max_isize) {
+    {
         HIPCHK(hipSetDevice(c->device));
         c->st = gzgpu_stats{};
         *out_len = 0;
@@ -1873,6 +1962,10 @@ int gzgpu_decompress_device(gzgpu_ctx *c, const void *d_in, size_t in_len, void 
         }
         if (st) return finish(c, st, "invalid gzip header");
         if (in_len < hl + 8) return finish(c, GZGPU_INVALID_FORMAT, "input shorter than header + trailer");
+        if (exp_isize > max_isize)
+            return finish(c, GZGPU_INVALID_ARGUMENT,
+                          "member of " + std::to_string(exp_isize) + " bytes (trailer ISIZE) exceeds the limit of " +
+                              std::to_string(max_isize));
         Member M;
         M.ds = hl * 8;
         M.de = (in_len - 8) * 8;
@@ -1891,7 +1984,73 @@ int gzgpu_decompress_device(gzgpu_ctx *c, const void *d_in, size_t in_len, void 
         c->st.decompressed_bytes = M.out_len;
         if (stats) *stats = c->st;
         return finish(c, M.status, M.msg);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gzgpu_decompress_device(gzgpu_ctx *c, const void *d_in, size_t in_len, void *d_out, size_t out_cap,
+                            size_t *out_len, uint32_t *crc_out, gzgpu_stats *stats) {
+    if (!c || !d_in || !d_out || !out_len) return GZGPU_INVALID_ARGUMENT;
+    return guarded(c, [&]() {
+        return decompress_device_member(c, d_in, in_len, d_out, out_cap, out_len, crc_out, stats, ~0ull);
     });
+}
+
+// The same decode with no allocation: every device buffer comes from this
+// device's `scratch` array, host staging from the pinned block made with the
+// context; the planner caps the pieces so that a member within the limits
+// fits.  The context's own buffers are set aside for the call and put back
+// after it, untouched.
+int gzgpu_decompress_device_noalloc(gzgpu_ctx *c, const void *d_in, size_t in_len, void *d_out, size_t out_cap,
+                                    size_t *out_len, uint32_t *crc_out, gzgpu_stats *stats) {
+    if (!c || !d_in || !d_out || !out_len) return GZGPU_INVALID_ARGUMENT;
+    *out_len = 0;
+    if (!c->scratch || !c->noalloc_stage) {
+        set_err(c, "no scratch memory on this device");
+        return GZGPU_GPU_UNAVAILABLE;
+    }
+    if (in_len > GZGPU_NOALLOC_MAX_IN_BYTES) {
+        set_err(c, "input of " + std::to_string(in_len) + " bytes exceeds the no-allocation limit of " +
+                       std::to_string((uint64_t)GZGPU_NOALLOC_MAX_IN_BYTES));
+        return GZGPU_INVALID_ARGUMENT;
+    }
+    std::vector<DevBuf> saved;
+    const std::vector<DevBuf *> bufs = ctx_bufs(c);
+    saved.reserve(bufs.size());
+    for (DevBuf *b : bufs) {
+        saved.push_back(*b);
+        *b = DevBuf{};
+    }
+    ScratchArena arena{c->scratch, NOALLOC_SCRATCH_BYTES, 0};
+    const gzgpu_tuning tune = c->tune;
+    c->tune = gzgpu_tuning{};
+    c->max_pieces = NOALLOC_MAX_PIECES;
+    c->pin.fixed = c->noalloc_stage;
+    c->pin.fixed_cap = NOALLOC_STAGE_BYTES;
+    c->pin.fixed_used = 0;
+    t_arena = &arena;
+    const int st = guarded(
+        c,
+        [&]() {
+            return decompress_device_member(c, d_in, in_len, d_out, out_cap, out_len, crc_out, stats,
+                                            GZGPU_NOALLOC_MAX_OUT_BYTES);
+        },
+        false);
+    // the stream is idle here (every path syncs before returning or
+    // throwing out of the pipeline, and guarded waits at entry); wait anyway
+    // before the scratch is handed to the next call
+    (void)hipStreamSynchronize(c->stream);
+    t_arena = nullptr;
+    c->pin.fixed = nullptr;
+    c->max_pieces = 0;
+    c->tune = tune;
+    for (size_t i = 0; i < bufs.size(); i++) *bufs[i] = saved[i];
+    for (auto &b : c->extra_slabs) b.release();
+    c->extra_slabs.clear();
+    return st;
 }
 
 int gzgpu_decompress_batch(gzgpu_ctx *c, const void *d_in, const uint64_t *in_off, const uint64_t *in_len, void *d_out,

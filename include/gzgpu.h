@@ -122,6 +122,25 @@ int gzgpu_decompress_to_writer(gzgpu_ctx *ctx, const uint8_t *in, size_t in_len,
 int gzgpu_decompress_device(gzgpu_ctx *ctx, const void *d_in, size_t in_len, void *d_out, size_t out_cap,
                             size_t *out_len, uint32_t *crc_out, gzgpu_stats *stats);
 
+/* ---- device-resident single member, no allocation ------------------------ */
+/* gzgpu_decompress_device without any allocation inside the call: no
+ * hipMalloc, hipFree, hipHostMalloc or other allocating routine runs.  Every
+ * device buffer the decode needs comes from one static array per device
+ * (256 MiB, held from library load to process exit), host staging from a
+ * pinned block made by gzgpu_ctx_create.  For callers that cannot allocate:
+ * serving loops with a fixed memory footprint, harnesses that forbid
+ * allocation.  Limits: in_len <= GZGPU_NOALLOC_MAX_IN_BYTES and the member's
+ * trailer ISIZE <= GZGPU_NOALLOC_MAX_OUT_BYTES (GZGPU_INVALID_ARGUMENT
+ * otherwise, before any device work); a member inside them whose decode still
+ * needs more scratch fails with GZGPU_OUT_OF_DEVICE_MEMORY.  Calls that
+ * overlap on one device (two contexts, two threads) share the array and race:
+ * serialise them.  Output equals gzgpu_decompress_device's.  No counterpart
+ * in the reference. */
+#define GZGPU_NOALLOC_MAX_IN_BYTES (32ull << 20)
+#define GZGPU_NOALLOC_MAX_OUT_BYTES (32ull << 20)
+int gzgpu_decompress_device_noalloc(gzgpu_ctx *ctx, const void *d_in, size_t in_len, void *d_out, size_t out_cap,
+                                    size_t *out_len, uint32_t *crc_out, gzgpu_stats *stats);
+
 /* ---- batched members (one persistent launch) ---------------------------- */
 /* n independent gzip members, member i at d_in + in_off[i] (in_len[i]
  * bytes); output of member i goes to d_out + out_off[i] with room for
