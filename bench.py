@@ -181,6 +181,65 @@ def batch_leg(args, dev, ctx):
             "check": "every member's bytes equal the generator's; CRC32 and ISIZE verified per member"}
 
 
+def mixed_leg(args, dev, ctx):
+    """The mixed-entropy member: 2 GiB in 64 MiB segments cycling text /
+    uniform random / byte runs / u32 deltas, one member at level 1 and one at
+    level 9 (zlib, segments encoded in parallel pigz-style).  Device-resident,
+    every step checked byte for byte on the device."""
+    import torch
+
+    import gzippy_amd
+    from gzippy_amd import workloads as W
+
+    n = args.mixed_mib << 20
+    t0 = time.time()
+    raw = W.generate_mixed(n, args.seed + 2, threads=16)
+    d_raw = torch.from_numpy(raw).to(dev)
+    out = {"decompressed_bytes": n, "segment_bytes": W.SEG, "seed": args.seed + 2,
+           "encoder": "zlib-" + __import__("zlib").ZLIB_RUNTIME_VERSION + " pigz-style (8 MiB segments, 32 KiB "
+                      "dictionary priming, sync flush), one member", "levels": []}
+    log(f"mixed workload: {n} B generated in {time.time() - t0:.1f}s")
+    d_out = torch.empty(n, dtype=torch.uint8, device=dev)
+    for level in (1, 9):
+        t0 = time.time()
+        gz = W.gzip_compress_segments(raw, level)
+        log(f"mixed L{level}: {n} B -> {gz.size} B in {time.time() - t0:.1f}s")
+        d_in = torch.from_numpy(gz).to(dev)
+        torch.cuda.synchronize()
+        got, crc, st = gzippy_amd.decompress_device(d_in, d_out, ctx=ctx)
+        torch.cuda.synchronize()
+        if not (got == n and torch.equal(d_out, d_raw)):
+            log(f"FAILED: mixed L{level} output differs")
+            sys.exit(1)
+        times, stats = [], []
+        for _ in range(args.mixed_steps):
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            got, crc, st = gzippy_amd.decompress_device(d_in, d_out, ctx=ctx)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - ts)
+            stats.append(st)
+        if not torch.equal(d_out, d_raw):
+            log(f"FAILED: mixed L{level} output differs after the timed steps")
+            sys.exit(1)
+        i = sorted(range(len(times)), key=lambda k: times[k])[len(times) // 2]
+        t, st = times[i], stats[i]
+        C = int(gz.size)
+        k_ms = st["ms_inflate_kernel"] / max(1, st["inflate_launches"])
+        rec = {"level": level, "compressed_bytes": C, "ms": round(t * 1e3, 3), "gbs": round(n / t / 1e9, 3),
+               "phase_ms": {k: round(st[k], 3) for k in ("ms_find", "ms_inflate", "ms_resolve", "ms_crc", "ms_total")},
+               "pieces": st["pieces"], "redecoded": st["redecoded"], "inflate_launches": st["inflate_launches"],
+               "inflate_kernel_ms_per_launch": round(k_ms, 3),
+               "roofline_c_plus_d": {"achieved_gbs": round((C + n) / (st["ms_inflate_kernel"] / 1e3) / 1e9, 2),
+                                     "frac": round((C + n) / (st["ms_inflate_kernel"] / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}}
+        log(f"mixed L{level}: {rec}")
+        out["levels"].append(rec)
+        del d_in
+    del d_raw, d_out
+    torch.cuda.empty_cache()
+    return out
+
+
 def graph_leg(args, dev, ctx, members, max_kib):
     """A batch of small members (lognormal around 16 KiB, at most `max_kib`;
     the four kinds, levels 1/6/9): the hipGraph form (one replay: header
@@ -325,6 +384,8 @@ def main():
     ap.add_argument("--autotune", type=int, default=1, help="pick the tuning by measurement before timing")
     ap.add_argument("--batch-members", type=int, default=10000, help="batched-members leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--mixed-mib", type=int, default=2048, help="mixed-entropy member size (0: skip the leg)")
+    ap.add_argument("--mixed-steps", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="serving leg, seconds per rate (0: skip)")
     ap.add_argument("--serve-pool", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -441,8 +502,11 @@ def main():
     }
     line["host_buffers"] = host_leg(gz, n, ctx)
     log(f"host buffers: {line['host_buffers']}")
+    del d_in, d_out
+    torch.cuda.empty_cache()
+    if args.mixed_mib:
+        line["mixed"] = mixed_leg(args, dev, ctx)
     if args.batch_members:
-        del d_in, d_out
         torch.cuda.empty_cache()
         line["batch"] = batch_leg(args, dev, ctx)
         torch.cuda.empty_cache()

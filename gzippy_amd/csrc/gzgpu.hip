@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -319,6 +320,7 @@ struct PieceHost {
     uint32_t member = 0;
     bool confirmed = false, dropped = false;
     bool merged = false;  // folded into an earlier piece's re-decode: no output of its own
+    uint64_t plan_cost = 0;  // planner's decode-cost estimate (0: from the member's average)
 };
 
 struct gzgpu_ctx {
@@ -440,7 +442,7 @@ std::vector<uint32_t> radix_order(const std::vector<uint64_t> &keys) {
 // One stage-A launch over every range; hits of all ranges, sorted, duplicates
 // (ranges that overlap) removed.  Every hit lies inside the range it came from.
 std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len,
-                                      const std::vector<std::pair<uint64_t, uint64_t>> &rng) {
+                                      const std::vector<std::pair<uint64_t, uint64_t>> &rng, bool estimate = false) {
     std::vector<gzgpu_boundary> out;
     std::vector<BfRange> table;
     uint64_t nchunk = 0, nbits = 0;
@@ -474,6 +476,13 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
                            cnt + 0, surv_cap, c->d_hits.as<gzgpu_boundary>(), hit_cap, cnt + 1);
         HIPCHK(hipGetLastError());
         c->st.launches++;
+        if (estimate) {
+            const uint32_t be = std::min<uint32_t>(hit_cap, (uint32_t)c->num_cus * 16);
+            hipLaunchKernelGGL(k_bf_estimate, dim3(be), dim3(64), 0, c->stream, d_in, in_len,
+                               c->d_hits.as<gzgpu_boundary>(), cnt + 1, hit_cap);
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+        }
         const uint32_t first = std::min<uint32_t>(hit_cap, 8192);
         const uint32_t *h_cnt = download<uint32_t>(c, cnt, 2);
         const gzgpu_boundary *h_hits = download<gzgpu_boundary>(c, c->d_hits.p, first);
@@ -554,6 +563,9 @@ constexpr double TILT = 0.25;
 // launches fails alone (cannot happen for a valid stream: the remainder
 // piece either reaches the final block or meets a real error)
 constexpr int SPEC_FALLBACK_ROUND = 4, SPEC_MAX_ROUNDS = 16;
+// planner: members cut into at least this many pieces get per-block output
+// estimates from the block finder
+constexpr uint64_t ESTIMATE_MIN_PIECES = 64;
 
 // Decode cost of a member in output-entry units.  A wavefront's decode time
 // is about max(40 ns per compressed bit, 11 ns per output entry) at full
@@ -570,6 +582,17 @@ constexpr int SPEC_FALLBACK_ROUND = 4, SPEC_MAX_ROUNDS = 16;
 uint64_t member_cost(uint64_t cbits, uint32_t exp_isize) {
     const bool stored_like = (uint64_t)exp_isize * 100 < (cbits / 8) * 115;
     return std::max<uint64_t>(1, stored_like ? cbits / 32 : std::max<uint64_t>(4 * cbits, exp_isize));
+}
+
+// Entries per compressed bit for each 12-bit output estimate of the block
+// finder (log2 in 1/256 steps, offset 8).
+const double *ratio_table() {
+    static const std::vector<double> t = [] {
+        std::vector<double> v(4096);
+        for (int q = 0; q < 4096; q++) v[q] = std::exp2((double)q / 256.0 - 8.0);
+        return v;
+    }();
+    return t.data();
 }
 
 uint64_t marker_slab_bytes(uint64_t expect_entries) { return round_up(2 * (expect_entries + 64 * KIB) + 512, 256); }
@@ -701,7 +724,7 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
         const uint64_t mbits = std::max<uint64_t>(1, M.de - M.ds);
         const double per_bit = (double)member_cost(mbits, M.exp_isize) / (double)mbits;
         const uint64_t bits = stop_of(P[pi].d) - P[pi].d.start_bit;
-        cost.push_back((uint64_t)(per_bit * 256.0 * (double)bits));
+        cost.push_back(P[pi].plan_cost ? P[pi].plan_cost : (uint64_t)(per_bit * 256.0 * (double)bits));
         any_marker |= (P[pi].d.flags & PF_MARKER) != 0;
     }
     float ms = 0;
@@ -795,16 +818,27 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         const uint64_t spacing = spacing_of(M);
         return M.status == GZGPU_OK && M.split && spacing != ~0ull && M.de > M.ds + spacing;
     };
+    // Output estimates per block (k_bf_estimate) pay off where a member is
+    // cut into many pieces: only there can content change along it enough to
+    // matter.  Members cut into fewer are planned by their average.
     std::vector<std::pair<uint64_t, uint64_t>> ranges;
+    bool estimate = false;
     for (auto &M : mem)
-        if (splits(M)) ranges.push_back({M.ds + 1, M.de});
+        if (splits(M)) {
+            ranges.push_back({M.ds, M.de});
+            estimate = estimate || (M.de - M.ds) / spacing_of(M) >= ESTIMATE_MIN_PIECES;
+        }
     const double th0 = debug_enabled() ? host_ms() : 0;
-    const std::vector<gzgpu_boundary> all_hits = find_hits(c, d_in, in_len, ranges);
+    const std::vector<gzgpu_boundary> all_hits = find_hits(c, d_in, in_len, ranges, estimate);
     const double th1 = debug_enabled() ? host_ms() : 0;
     c->st.candidates += (uint32_t)all_hits.size();
     auto hit_lt = [](const gzgpu_boundary &a, uint64_t bit) { return a.bit_offset < bit; };
     std::vector<PieceHost> P;
     P.reserve(mem.size() + (size_t)std::min<uint64_t>(total_cost / target_cost, 1u << 24) + 16);
+    // per-member planning scratch, reused across members
+    std::vector<uint64_t> sx, gpts;           // stretch starts (bits) then the member end; grid points
+    std::vector<double> sent, scost, cc, ce;  // entries / cost per bit of each stretch; their running sums
+    std::vector<uint8_t> sstored, shadow, chain_ok;
     for (uint32_t m = 0; m < mem.size(); m++) {
         Member &M = mem[m];
         if (M.status != GZGPU_OK) continue;
@@ -824,6 +858,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         M.pieces.push_back((uint32_t)P.size());
         P.push_back(p0);
         if (!splits(M)) continue;
+        const double t_mem0 = debug_level() >= 2 ? host_ms() : 0;
         const uint64_t spacing = spacing_of(M);
         struct {
             const gzgpu_boundary *b, *e;
@@ -831,7 +866,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             const gzgpu_boundary &operator[](size_t i) const { return b[i]; }
             const gzgpu_boundary *begin() const { return b; }
             const gzgpu_boundary *end() const { return e; }
-        } hits{all_hits.data() + (std::lower_bound(all_hits.begin(), all_hits.end(), M.ds + 1, hit_lt) - all_hits.begin()),
+        } hits{all_hits.data() + (std::lower_bound(all_hits.begin(), all_hits.end(), M.ds, hit_lt) - all_hits.begin()),
                all_hits.data() + (std::lower_bound(all_hits.begin(), all_hits.end(), M.de, hit_lt) - all_hits.begin())};
         // piece starts: per grid point a hit within an eighth of a spacing
         // of it, else a start inside the block that holds the grid point
@@ -857,20 +892,24 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             bool mid;
         };
         std::vector<Start> chosen;
-        auto chained = [&](size_t i) {
+        // chained[i]: stored hit i's payload ends where another hit's
+        // accepted range begins, or at the member's end
+        chain_ok.assign(hits.size(), 0);
+        for (size_t i = 0; i < hits.size(); i++) {
+            if (hits[i].kind != 0) continue;
             const uint64_t o = hits[i].seek_bit + 3;  // the LEN field
             const uint64_t next = (o / 8 + 4 + (hits[i].info >> 16)) * 8;
-            if (next == M.de) return true;
+            bool ok = next == M.de;
             auto k = std::lower_bound(hits.begin(), hits.end(), next > 16 ? next - 16 : 0, hit_lt);
-            for (; k != hits.end() && k->bit_offset <= next + 1; ++k)
-                if (k->seek_bit >= next) return true;
-            return false;
-        };
+            for (; !ok && k != hits.end() && k->bit_offset <= next + 1; ++k) ok = k->seek_bit >= next;
+            chain_ok[i] = ok;
+        }
+        auto chained = [&](size_t i) { return chain_ok[i] != 0; };
         // Dynamic-header hits inside the payload of a chained stored block
         // are data, not blocks of this stream (a gzip of deflate data: .gz,
         // .zip, .png): every piece started there would miss.  They are
         // shadowed for planning.
-        std::vector<uint8_t> shadow(hits.size(), 0);
+        shadow.assign(hits.size(), 0);
         for (size_t i = 0; i < hits.size(); i++) {
             if (hits[i].kind != 0 || !chained(i)) continue;
             const uint64_t o = hits[i].seek_bit + 3;
@@ -880,6 +919,90 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 if (k->kind == 2) shadow[(size_t)(k - hits.begin())] = 1;
         }
         auto dyn_hit = [&](size_t j) { return hits[j].kind == 2 && !shadow[j]; };
+        // Decode cost and output along the member, per stretch between
+        // consecutive real hits (dynamic headers, chained stored blocks):
+        // a stored block's output is its LEN; a dynamic block's is estimated
+        // from its code lengths (entries per bit, from the block finder) and
+        // the estimates are scaled so that the member's total matches its
+        // trailer ISIZE.  Cost per bit as in member_cost: max(4, entries per
+        // bit) for Huffman blocks, 1/32 for stored ones.  Mixed content
+        // (stored runs between text and u32 stretches) then gets pieces of
+        // equal decode time, not of equal compressed size.
+        // (the first stretch, from the member start to its first real hit
+        // or at it, takes that hit's kind: a member's first block is almost
+        // always of the kind of its second)
+        sx.clear();
+        sent.clear();
+        scost.clear();
+        sstored.clear();
+        {
+            const double avg = (double)M.exp_isize / (double)std::max<uint64_t>(1, M.de - M.ds);
+            auto ratio_of = [&](size_t j) { return estimate ? ratio_table()[hits[j].info >> 20] : avg; };
+            for (size_t j = 0; j < hits.size(); j++) {
+                const bool st_hit = hits[j].kind == 0 && chained(j);
+                if (!dyn_hit(j) && !st_hit) continue;
+                const double v = st_hit ? (double)(hits[j].info >> 16) : ratio_of(j);  // stored: LEN, per bit below
+                if (sx.empty()) {
+                    sx.push_back(M.ds);
+                    sstored.push_back(st_hit ? 1 : 0);
+                    // the first block's LEN is unknown: a stored first stretch is
+                    // costed as bytes (8 bits per entry) until the ISIZE fit
+                    sent.push_back(st_hit ? -1.0 : v);
+                    if (hits[j].bit_offset <= M.ds) {
+                        sent.back() = v;
+                        continue;
+                    }
+                }
+                if (hits[j].bit_offset <= sx.back()) continue;
+                sx.push_back(hits[j].bit_offset);
+                sstored.push_back(st_hit ? 1 : 0);
+                sent.push_back(v);
+            }
+            if (sx.empty()) {
+                sx.push_back(M.ds);
+                sstored.push_back(0);
+                sent.push_back(avg);
+            }
+            sx.push_back(M.de);
+            for (size_t j = 0; j + 1 < sx.size(); j++)
+                if (sent[j] < 0) sent[j] = (double)(sx[j + 1] - sx[j]) / 8.0;
+            double stored_ent = 0, dyn_est = 0;
+            for (size_t j = 0; j + 1 < sx.size(); j++) {
+                const double bits = (double)(sx[j + 1] - sx[j]);
+                if (sstored[j]) {
+                    stored_ent += sent[j];
+                    sent[j] = bits > 0 ? sent[j] / bits : 0;
+                } else {
+                    dyn_est += sent[j] * bits;
+                }
+            }
+            const double want = (double)M.exp_isize - stored_ent;
+            const double scale = dyn_est > 0 && want > 0 ? std::min(8.0, std::max(0.125, want / dyn_est)) : 1.0;
+            scost.resize(sent.size());
+            for (size_t j = 0; j + 1 < sx.size(); j++) {
+                if (!sstored[j]) sent[j] *= scale;
+                scost[j] = sstored[j] ? 1.0 / 32.0 : std::max(4.0, sent[j]);
+            }
+        }
+        const double t_str = debug_level() >= 2 ? host_ms() : 0;
+        const size_t nseg = sx.size() - 1;
+        cc.assign(nseg + 1, 0.0);  // cumulative cost / entries at stretch starts
+        ce.assign(nseg + 1, 0.0);
+        for (size_t j = 0; j < nseg; j++) {
+            cc[j + 1] = cc[j] + scost[j] * (double)(sx[j + 1] - sx[j]);
+            ce[j + 1] = ce[j] + sent[j] * (double)(sx[j + 1] - sx[j]);
+        }
+        // (cost, entries) at bit x; queries in increasing x per cursor
+        struct Cursor {
+            size_t j = 0;
+        };
+        auto cum_at = [&](Cursor &cur, uint64_t x, double &cost, double &ent) {
+            x = std::min(std::max(x, M.ds), M.de);
+            while (cur.j + 1 < nseg && sx[cur.j + 1] <= x) cur.j++;
+            const double dx = (double)(x - sx[cur.j]);
+            cost = cc[cur.j] + scost[cur.j] * dx;
+            ent = ce[cur.j] + sent[cur.j] * dx;
+        };
         // grid points: the starts of pieces 1 .. np-1.  Piece sizes fall
         // linearly along the member (density 1 - TILT x over x in [0, 1]):
         // pieces are pulled largest first, so the member's first pieces go
@@ -892,12 +1015,21 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         // piece count.)  Measured on that member: inflate kernel 4.33 ms
         // flat, 4.20 ms at a slope of 0.22-0.3.
         const double tilt = TILT;
-        auto grid_at = [&](uint64_t k) {
-            const double x = (double)k / (double)np;
-            return M.ds + (uint64_t)((double)span * (x - 0.5 * tilt * x * x) / (1.0 - 0.5 * tilt));
-        };
+        gpts.resize(np + 1);  // grid point k: where the cumulative cost reaches its share
+        {
+            size_t j = 0;
+            for (uint64_t k = 0; k <= np; k++) {
+                const double x = (double)k / (double)np;
+                const double C = cc[nseg] * (x - 0.5 * tilt * x * x) / (1.0 - 0.5 * tilt);
+                while (j + 1 < nseg && cc[j + 1] <= C) j++;
+                const double b = (double)sx[j] + (C - cc[j]) / std::max(scost[j], 1e-12);
+                gpts[k] = std::min<uint64_t>(M.de, std::max<uint64_t>(M.ds, (uint64_t)b));
+            }
+        }
+        auto grid_at = [&](uint64_t k) { return gpts[k]; };
         for (uint64_t gi = 1; gi < np; gi++) {
             const uint64_t g = grid_at(gi);
+            if (g <= last) continue;
             const uint64_t rest = std::min(g - grid_at(gi - 1), grid_at(gi + 1) - g);
             const uint64_t lo = std::max<uint64_t>(last + 1, g - rest / 2), hi = g + rest / 2;
             while (h < hits.size() && hits[h].bit_offset < lo) h++;
@@ -940,14 +1072,13 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             last = hits[best].bit_offset;
             h = best + 1;
         }
-        const uint64_t total_bits = M.de - M.ds;
         if (debug_level() >= 2)
             fprintf(stderr,
-                    "[gzgpu] plan member %u: bits %llu isize %u cost %llu target %llu spacing %llu grid %llu hits %zu "
-                    "chosen %zu\n",
-                    m, (unsigned long long)span, M.exp_isize, (unsigned long long)cost_of(M),
+                    "[gzgpu] plan member %u: bits %llu isize %u cost %llu (by stretch %.0f) target %llu spacing %llu "
+                    "grid %llu hits %zu stretches %zu chosen %zu; stretches %.3f ms, grid %.3f ms\n",
+                    m, (unsigned long long)span, M.exp_isize, (unsigned long long)cost_of(M), cc[nseg],
                     (unsigned long long)target_cost, (unsigned long long)spacing, (unsigned long long)np, hits.size(),
-                    chosen.size());
+                    nseg, chosen.size(), t_str - t_mem0, host_ms() - t_str);
         uint64_t tail_hdr = M.ds;  // header of the member's last real hit
         for (size_t j = hits.size(); j-- > 0;)
             if (dyn_hit(j) || (hits[j].kind == 0 && chained(j))) {
@@ -957,6 +1088,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         if (c->tune.miss_every)  // test hook: every n-th start one bit late, a certain miss
             for (size_t k = c->tune.miss_every - 1; k < chosen.size(); k += c->tune.miss_every)
                 chosen[k].start = chosen[k].acc_lo = chosen[k].acc_hi = chosen[k].start + 1;
+        Cursor cur_lo, cur_hi;
         for (size_t k = 0; k < chosen.size(); k++) {
             const Start &b = chosen[k];
             PieceDesc &prev = P[M.pieces.back()].d;
@@ -984,9 +1116,14 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             // sized for the rest of the member (no overflow and no second
             // launch for it)
             const bool tail_next = k + 1 < chosen.size() && chosen[k + 1].mid && chosen[k + 1].hdr == tail_hdr;
-            uint64_t end = k + 1 < chosen.size() && !tail_next ? chosen[k + 1].acc_lo : M.de;
-            double frac = (double)(end - b.acc_lo) / (double)total_bits;
-            q.d.cap = marker_slab_bytes((uint64_t)(1.3 * frac * (double)M.exp_isize));
+            const uint64_t next = k + 1 < chosen.size() ? std::max(chosen[k + 1].acc_lo, b.acc_lo) : M.de;
+            double c0, e0, c1, e1;
+            cum_at(cur_lo, b.acc_lo, c0, e0);
+            cum_at(cur_hi, next, c1, e1);
+            const double est = (tail_next ? ce[nseg] : e1) - e0;
+            q.d.cap = marker_slab_bytes((uint64_t)(1.4 * est) + 4096);
+            q.plan_cost = (uint64_t)(256.0 * (c1 - c0)) + 1;
+            if (k == 0) P[M.pieces.back()].plan_cost = (uint64_t)(256.0 * c0) + 1;
             M.pieces.push_back((uint32_t)P.size());
             P.push_back(q);
         }
@@ -1119,6 +1256,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                     p.d.flags = (p.d.flags & ~(PF_MIDBLOCK | PF_STOPMID)) | (to_end ? 0u : (last.d.flags & PF_STOPMID));
                     p.d.start_bit = p.acc_lo = p.acc_hi = e;
                     p.d.hdr_bit = 0;
+                    p.plan_cost = 0;
                     for (size_t q = k + 1; q < j; q++) {
                         P[M.pieces[q]].merged = true;
                         linked[M.pieces[q]] = 1;
@@ -2182,6 +2320,7 @@ int gzgpu_find_blocks(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t st
         for (auto &b : h) {
             b.bit_offset -= sb;
             b.seek_bit -= sb;
+            if (b.kind == 2) b.info &= 0xFFFFFu;  // the planner's output estimate stays internal
         }
         *count = h.size();
         if (out) std::memcpy(out, h.data(), std::min(cap, h.size()) * sizeof(gzgpu_boundary));

@@ -409,4 +409,144 @@ __global__ void __launch_bounds__(256) k_bf_stage_b(const uint8_t *__restrict__ 
     }
 }
 
+// Output estimate of every dynamic-header hit, in bits 20-31 of its info:
+// the block's expected output per compressed bit, as its code lengths imply
+// it (symbol s drawn with probability 2^-len(s)): entries and bits of an
+// average literal/length symbol, a length symbol's distance bits averaged
+// the same way.  A planning estimate (cutting a member into pieces of equal
+// decode time), not a parity value: log2(entries per bit) in 1/256 steps,
+// offset by 8, clamped to 12 bits.
+//
+// One wavefront per hit, so the serial walk over the code lengths (a
+// hundred or more precode symbols) becomes ~15 rounds: every lane decodes
+// the precode symbol that would start at bit P + lane from a 128-bit window,
+// a readlane walk from lane 0 picks the true starts, a prefix sum gives each
+// its code index, and each taken lane adds its run's weight.
+__global__ void __launch_bounds__(64) k_bf_estimate(const uint8_t *__restrict__ in, uint64_t n,
+                                                    gzgpu_boundary *__restrict__ hits, const uint32_t *hit_count,
+                                                    uint32_t hit_cap) {
+    __shared__ uint8_t lut[128];
+    const uint32_t lane = lane_id();
+    const uint32_t counted = uni(*hit_count);
+    const uint32_t nh = counted < hit_cap ? counted : hit_cap;
+    for (uint32_t hidx = blockIdx.x; hidx < nh; hidx += gridDim.x) {
+        if (uni(hits[hidx].kind) != 2) continue;
+        const uint64_t o = uni64(hits[hidx].bit_offset);
+        const uint64_t h = uni64(peek64(in, n, o));
+        const uint32_t hl = ((uint32_t)(h >> 3) & 31) + 257, hdist = (uint32_t)(h >> 8) & 31,
+                       pc = ((uint32_t)(h >> 13) & 15) + 4;
+        const uint64_t pre = uni64(peek64(in, n, o + 17));
+        // precode LUT: symbol s = lane, canonical codes in symbol order
+        uint32_t l = 0;
+        if (lane < 19) {
+            const uint32_t ps = c_pre_pos[lane];
+            l = ps < pc ? (uint32_t)(pre >> (3 * ps)) & 7 : 0;
+        }
+        uint32_t code = 0, cprev = 0, my_code = 0;
+#pragma unroll
+        for (uint32_t L = 1; L < 8; L++) {
+            const uint64_t mL = ballot(l == L);
+            code = (code + cprev) << 1;
+            if (l == L) my_code = code + mbcnt(mL);
+            cprev = (uint32_t)__popcll(mL);
+        }
+        for (uint32_t f = lane; f < 128; f += 64) lut[f] = 0;
+        __syncthreads();
+        if (l) {
+            const uint32_t rev = __brev(my_code) >> (32 - l);
+            for (uint32_t f = rev; f < 128; f += 1u << l) lut[f] = (uint8_t)(lane | l << 5);
+        }
+        __syncthreads();
+        const uint32_t total = hl + hdist + 1;
+        uint64_t P = o + 17 + 3 * pc;
+        uint32_t i = 0, prev = 0;
+        float e_ent = 0.f, e_bits = 0.f, m_len = 0.f, d_bits = 0.f, d_w = 0.f;
+        for (uint32_t round = 0; i < total && round < 256; round++) {
+            const uint64_t W0 = uni64(peek64(in, n, P)), W1 = uni64(peek64(in, n, P + 64));
+            const uint64_t ch = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
+            const uint32_t e = lut[ch & 127];
+            const uint32_t L = e >> 5, sym = e & 31;
+            const uint64_t x = ch >> L;
+            const uint32_t ext = sym < 16 ? 0u : sym == 16 ? 2u : sym == 17 ? 3u : 7u;
+            const uint32_t rep = sym < 16    ? 1u
+                                 : sym == 16 ? 3u + (uint32_t)(x & 3)
+                                 : sym == 17 ? 3u + (uint32_t)(x & 7)
+                                             : 11u + (uint32_t)(x & 127);
+            const uint32_t nxt = lane + L + ext;
+            const uint64_t OK = ballot(L != 0);
+            if (!(OK & 1)) break;  // (not for a validated header)
+            uint64_t M = 0;
+            uint32_t st = 0;
+            for (;;) {
+                M |= 1ull << st;
+                const uint32_t t = uni((uint32_t)__builtin_amdgcn_readlane((int)nxt, (int)st));
+                st = t;
+                if (t >= 64 || !((OK >> t) & 1)) break;
+            }
+            M = uni64(M);
+            const bool inM = (M >> lane) & 1;
+            const uint32_t v = inM ? rep : 0u;
+            const uint32_t incl = wave_scan_incl(v);
+            const uint32_t idx = i + incl - v;
+            const bool take = inM && idx < total;
+            const uint64_t T = uni64(ballot(take));
+            // a repeat (16) takes the value of the nearest earlier taken
+            // symbol that is not a repeat, or the carried one
+            const uint64_t NZ = T & ballot(sym != 16);
+            const uint64_t below = NZ & ((1ull << lane) - 1);
+            const int src = below ? 63 - __builtin_clzll(below) : 0;
+            const uint32_t def = sym < 16 ? sym : 0u;
+            const uint32_t from = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)def);
+            const uint32_t val = sym == 16 ? (below ? from : prev) : def;
+            if (take && val) {
+                const float w = (float)(1u << (15 - val));
+                const uint32_t hi = idx + rep < total ? idx + rep : total;
+                const uint32_t lo = idx;
+                const uint32_t ll_hi = hi < hl ? hi : hl;
+                if (ll_hi > lo) {
+                    const uint32_t lit_hi = ll_hi < 256 ? ll_hi : 256;
+                    if (lit_hi > lo) {
+                        e_ent += w * (float)(lit_hi - lo);
+                        e_bits += w * (float)(val * (lit_hi - lo));
+                    }
+                    if (lo <= 256 && 256 < ll_hi) e_bits += w * (float)val;  // end-of-block
+                    for (uint32_t q = lo > 257 ? lo : 257; q < ll_hi && q < 286; q++) {
+                        const uint32_t k = q - 257, E = c_lextra[k];
+                        e_ent += w * ((float)c_lbase[k] + 0.5f * (float)((1u << E) - 1));
+                        e_bits += w * (float)(val + E);
+                        m_len += w;
+                    }
+                }
+                for (uint32_t d = (lo > hl ? lo : hl); d < hi && d - hl < 30; d++) {
+                    d_w += w;
+                    d_bits += w * (float)(val + c_dextra[d - hl]);
+                }
+            }
+            if (!T) break;
+            const uint32_t last = uni((uint32_t)(63 - __builtin_clzll(T)));
+            prev = uni((uint32_t)__builtin_amdgcn_readlane((int)val, (int)last));
+            i += uni((uint32_t)__builtin_amdgcn_readlane((int)(incl - v + (take ? rep : 0u)), (int)last));
+            i = uni(i);
+            P += st;
+            P = uni64(P);
+            if (T != M) break;  // the header ended inside this round
+        }
+        for (uint32_t off = 32; off; off >>= 1) {
+            e_ent += __shfl_xor(e_ent, off);
+            e_bits += __shfl_xor(e_bits, off);
+            m_len += __shfl_xor(m_len, off);
+            d_bits += __shfl_xor(d_bits, off);
+            d_w += __shfl_xor(d_w, off);
+        }
+        if (lane == 0) {
+            const float bits = e_bits + m_len * (d_w > 0.f ? d_bits / d_w : 5.f);
+            const float ratio = bits > 0.f && e_ent > 0.f ? e_ent / bits : 1.f;
+            int r = (int)((__log2f(ratio) + 8.f) * 256.f + 0.5f);
+            r = r < 0 ? 0 : r > 4095 ? 4095 : r;
+            hits[hidx].info = (hits[hidx].info & 0xFFFFFu) | (uint32_t)r << 20;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace gz
