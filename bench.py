@@ -31,26 +31,124 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(gz, raw_size, budget_s=12.0, max_runs=3):
-    """Oracle serial decode (one core) of the same member; median GB/s."""
-    import numpy as np
-    from oracle import gzoracle
-    out = np.empty(raw_size + 65536, dtype=np.uint8)
+def host_cpu():
+    """(cores this process may use, CPU model) of the host running the bench."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
+def _median_time(fn, budget_s, max_runs):
     times = []
     t_start = time.time()
     while len(times) < max_runs and (not times or time.time() - t_start < budget_s):
         t0 = time.perf_counter()
-        import ctypes
+        fn()
+        times.append(time.perf_counter() - t0)
+    return sorted(times)[len(times) // 2], len(times)
+
+
+def cpu_baseline(gz, raw, budget_s=6.0, max_runs=3):
+    """gzippy's CPU path, restated in C (oracle/), timed on this host's cores
+    on the same member, every output checked against the generator bytes:
+    - value: the threaded speculative restatement (oracle/gzpar.c, the
+      ParallelSM pipeline of chunk_fetcher.rs:638-1600) at all host cores;
+    - t1: the serial restatement (drive_thin_t1_oracle, one core);
+    - zlib and libdeflate single-thread decodes as independent references."""
+    import ctypes
+    import zlib
+
+    import numpy as np
+    from oracle import gzoracle
+    cores, model = host_cpu()
+    n_raw = raw.size
+    out = np.empty(n_raw + 65536, dtype=np.uint8)
+    L = gzoracle.lib()
+    n = ctypes.c_size_t()
+    m = ctypes.c_size_t()
+
+    def par():
+        st = L.gzo_gunzip_parallel(gz.ctypes.data, gz.size, out.ctypes.data, out.size, ctypes.byref(n), cores,
+                                   512 << 10)
+        assert st == 0 and n.value == n_raw
+
+    def t1():
+        st = L.gzo_gunzip(gz.ctypes.data, gz.size, out.ctypes.data, out.size, ctypes.byref(n), ctypes.byref(m))
+        assert st == 0 and n.value == n_raw
+
+    rec = {"unit": "GB/s", "cores": cores, "kind": "port", "cpu_model": model}
+    t, runs = _median_time(par, budget_s, max_runs)
+    assert np.array_equal(out[:n_raw], raw)
+    rec["value"] = round(n_raw / t / 1e9, 4)
+    rec["sample"] = (f"the full {n_raw} B member, threaded speculative restatement (oracle/gzpar.c: 512 KiB "
+                     f"partitions, block finder, marker decode, window chain, parallel resolve + CRC combine) on "
+                     f"{cores} threads, median of {runs} runs")
+    t, runs = _median_time(t1, budget_s, max_runs)
+    rec["t1"] = {"value": round(n_raw / t / 1e9, 4), "cores": 1, "runs": runs,
+                 "what": "serial restatement (oracle/gzoracle.c gzo_gunzip)"}
+    mv = memoryview(gz)
+    t, runs = _median_time(lambda: zlib.decompress(mv, 31), budget_s, 1)
+    rec["zlib"] = {"value": round(n_raw / t / 1e9, 4), "cores": 1, "runs": runs,
+                   "version": zlib.ZLIB_RUNTIME_VERSION}
+    try:
+        ld = ctypes.CDLL("libdeflate.so.0")
+        ld.libdeflate_alloc_decompressor.restype = ctypes.c_void_p
+        ld.libdeflate_gzip_decompress.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                  ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        ld.libdeflate_free_decompressor.argtypes = [ctypes.c_void_p]
+        d = ld.libdeflate_alloc_decompressor()
+
+        def ldd():
+            st = ld.libdeflate_gzip_decompress(d, gz.ctypes.data, gz.size, out.ctypes.data, out.size,
+                                               ctypes.byref(n))
+            assert st == 0 and n.value == n_raw
+
+        t, runs = _median_time(ldd, budget_s, max_runs)
+        ld.libdeflate_free_decompressor(d)
+        rec["libdeflate"] = {"value": round(n_raw / t / 1e9, 4), "cores": 1, "runs": runs}
+    except OSError:
+        rec["libdeflate"] = None
+    return rec
+
+
+def cpu_members_baseline(ins, raws, cores):
+    """Member-per-core CPU baseline for a batch (MultiMemberPar, bgzf.rs:2370):
+    every member decoded by the serial restatement, members spread over the
+    host's cores; returns GB/s over all members."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    from oracle import gzoracle
+    L = gzoracle.lib()
+    outs = [np.empty(r.size + 64, dtype=np.uint8) for r in raws]
+
+    def one(i):
         n = ctypes.c_size_t()
         m = ctypes.c_size_t()
-        st = gzoracle.lib().gzo_gunzip(gz.ctypes.data, gz.size, out.ctypes.data, out.size, ctypes.byref(n),
-                                       ctypes.byref(m))
-        times.append(time.perf_counter() - t0)
-        assert st == 0 and n.value == raw_size
-    t = sorted(times)[len(times) // 2]
-    return {"value": round(raw_size / t / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"the full {raw_size} B member decoded serially by oracle/gzoracle.c "
-                      f"(restatement of drive_thin_t1_oracle), median of {len(times)} runs"}
+        st = L.gzo_gunzip(ins[i].ctypes.data, ins[i].size, outs[i].ctypes.data, outs[i].size, ctypes.byref(n),
+                          ctypes.byref(m))
+        return st == 0 and n.value == raws[i].size
+
+    with ThreadPoolExecutor(cores) as ex:
+        t0 = time.perf_counter()
+        ok = all(ex.map(one, range(len(ins))))
+        t = time.perf_counter() - t0
+    ok = ok and all(np.array_equal(outs[i][:raws[i].size], raws[i]) for i in range(0, len(raws), 97))
+    D = sum(r.size for r in raws)
+    return {"value": round(D / t / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port", "ok": bool(ok),
+            "what": "member per core: oracle/gzoracle.c gzo_gunzip on each member, a thread pool of `cores`"}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_inflate_hbm_traffic_1gib_text_l6.md")
@@ -123,6 +221,10 @@ def batch_leg(args, dev, ctx):
     d_in = torch.from_numpy(np.concatenate(ins)).to(dev)
     d_out = torch.empty(D + 1, dtype=torch.uint8, device=dev)
     ref = torch.from_numpy(np.concatenate(raws)).to(dev)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_members_baseline(ins, raws, host_cpu()[0])
+        log(f"batch cpu baseline: {cpu}")
     del raws, ins, mem
     torch.cuda.synchronize()
 
@@ -178,6 +280,7 @@ def batch_leg(args, dev, ctx):
             "batched_phase_ms": {k: round(st[k], 3) for k in ("ms_find", "ms_inflate", "ms_resolve", "ms_crc",
                                                               "ms_total", "ms_inflate_marker", "ms_inflate_clean")},
             "batched_launches": st["launches"],
+            "cpu_baseline": cpu,
             "check": "every member's bytes equal the generator's; CRC32 and ISIZE verified per member"}
 
 
@@ -528,7 +631,8 @@ def main():
         log(f"calibration copy: {nb} B")
     if not args.no_cpu:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(gz, n)
+        line["cpu_baseline"] = cpu_baseline(gz, raw)
+        log(f"cpu baseline: {line['cpu_baseline']}")
     print(json.dumps(line), flush=True)
 
 

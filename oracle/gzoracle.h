@@ -90,6 +90,11 @@ size_t gzo_find_blocks(const uint8_t *in, size_t in_len, uint64_t start_bit, uin
 int gzo_gunzip(const uint8_t *in, size_t in_len, uint8_t *out, size_t out_cap, size_t *out_len,
                size_t *members);
 
+/* ---- threaded speculative decode of one member (gzpar.c; chunk_fetcher.rs
+ *      :638-1600 ParallelSM with `threads` workers and `chunk_bytes` partitions) ---- */
+int gzo_gunzip_parallel(const uint8_t *in, size_t in_len, uint8_t *out, size_t out_cap, size_t *out_len,
+                        int threads, size_t chunk_bytes);
+
 #ifdef __cplusplus
 }
 #endif

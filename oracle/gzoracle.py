@@ -72,6 +72,9 @@ def lib():
         L.gzo_gunzip.restype = ctypes.c_int
         L.gzo_gunzip.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
                                  ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+        L.gzo_gunzip_parallel.restype = ctypes.c_int
+        L.gzo_gunzip_parallel.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t), ctypes.c_int, ctypes.c_size_t]
         del u8p
         _lib = L
     return _lib
@@ -157,3 +160,16 @@ def gunzip(data, out_cap=None):
     m = ctypes.c_size_t()
     st = lib().gzo_gunzip(a.ctypes.data, a.size, out.ctypes.data, out_cap, ctypes.byref(n), ctypes.byref(m))
     return st, out[: n.value], m.value
+
+
+def gunzip_parallel(data, threads=4, chunk_bytes=512 << 10, out_cap=None):
+    """Threaded speculative decode of one member (gzpar.c); returns
+    (status, bytes)."""
+    a = _buf(data)
+    if out_cap is None:
+        out_cap = max(1 << 16, int(a[-4:].view(np.uint32)[0]) + 65536 if a.size >= 18 else 1 << 16)
+    out = np.empty(out_cap, dtype=np.uint8)
+    n = ctypes.c_size_t()
+    st = lib().gzo_gunzip_parallel(a.ctypes.data, a.size, out.ctypes.data, out_cap, ctypes.byref(n), int(threads),
+                                   int(chunk_bytes))
+    return st, out[: n.value]

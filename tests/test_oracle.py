@@ -313,3 +313,20 @@ def test_block_finder_stored(oracle):
     stored = [h for h in hits if h[2] == 0]
     # LEN at byte 4: 8 zero bits before it (byte 3) -> earliest = 32 - 8 = 24, seek = 29
     assert (24, 29, 0) in stored
+
+
+@pytest.mark.parametrize("kind", ["text", "random", "runs", "u32"])
+@pytest.mark.parametrize("level", [1, 6, 9])
+def test_parallel_restatement_matches_generator(oracle, kind, level):
+    """The threaded speculative restatement (the bench's multi-core CPU
+    baseline) returns the generator's bytes, with small partitions so that
+    every member is split and some partitions start on false candidates."""
+    raw = W.generate(kind, (2 << 20) + 4321, seed=40 + level)
+    gz = W.gzip_compress(raw, level)
+    for threads in (1, 3):
+        st, out = oracle.gunzip_parallel(gz, threads=threads, chunk_bytes=32 << 10)
+        assert st == 0 and np.array_equal(out, raw), (kind, level, threads)
+    bad = gz.copy()
+    bad[-7] ^= 1  # CRC
+    st, _ = oracle.gunzip_parallel(bad, threads=2, chunk_bytes=32 << 10)
+    assert st == oracle.CRC_MISMATCH
