@@ -14,7 +14,7 @@
  *                               src/decompress/mod.rs:629-650), device-resident
  *   gzgpu_decompress_batch      src/decompress/bgzf.rs:2370
  *                               decompress_multi_member_parallel_to_vec (member-per-worker)
- *   gzgpu_find_blocks           src/decompress/parallel/blockfinder_validation.rs:520 find_blocks
+ *   gzgpu_find_blocks           src/decompress/parallel/blockfinder_validation.rs:483 find_blocks
  *   gzgpu_crc32                 src/decompress/parallel/crc32.rs:155 crc32_fold
  *   gzgpu_status_string         src/decompress/parallel/single_member.rs:1415-1437 Display
  *                               for ParallelError
