@@ -63,7 +63,8 @@ class Stats(ctypes.Structure):
                 ("ms_inflate_marker", ctypes.c_double), ("ms_inflate_clean", ctypes.c_double),
                 ("clean_in_bytes", ctypes.c_uint64), ("clean_out_bytes", ctypes.c_uint64),
                 ("marker_in_bytes", ctypes.c_uint64), ("marker_out_bytes", ctypes.c_uint64),
-                ("midblock_pieces", ctypes.c_uint32), ("midblock_linked", ctypes.c_uint32)]
+                ("midblock_pieces", ctypes.c_uint32), ("midblock_linked", ctypes.c_uint32),
+                ("stored_members", ctypes.c_uint32), ("stored_blocks", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -41,6 +41,7 @@
 #include "k_crc.h"
 #include "k_inflate.h"
 #include "k_resolve.h"
+#include "k_stored.h"
 
 using namespace gz;
 
@@ -291,6 +292,7 @@ struct Member {
     uint64_t out_cap = 0;
     uint32_t exp_crc = 0, exp_isize = 0;
     bool split = false;
+    bool stored = false;  // decoded by the stored-block split (no pieces)
     // results
     int status = GZGPU_OK;
     std::string msg;
@@ -781,7 +783,106 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
     }
 }
 
+// StoredParallel (decompress_stored_parallel, stored_split.rs:265; routed by
+// parallel_sm_unprofitable && first_block_is_stored, decompress/mod.rs:282-286):
+// a member whose output is no larger than 1.15x its compressed bytes and
+// which is one chain of stored blocks from its first byte to its trailer is
+// copied block by block, with no block finder and no speculation.  The chain
+// is walked on the host through the byte-aligned LEN/NLEN candidates of
+// k_stored_scan; a member whose chain breaks (a Huffman block, or a final
+// block that ends before the trailer) goes through the speculative path, as
+// the reference's NotStoredDominated does.  Only members of at least
+// STORED_MIN_BYTES: a smaller one is one piece anyway.
+constexpr uint64_t STORED_MIN_BYTES = 1 * MIB;
+
+void stored_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
+    std::vector<uint32_t> cand_m;
+    for (uint32_t m = 0; m < mem.size(); m++) {
+        const Member &M = mem[m];
+        const uint64_t cbytes = (M.de - M.ds) / 8;
+        if (M.status != GZGPU_OK || !M.split || cbytes < STORED_MIN_BYTES || M.ds % 8 || M.de % 8) continue;
+        if ((uint64_t)M.exp_isize * 100 >= cbytes * 115) continue;  // parallel_sm_unprofitable
+        cand_m.push_back(m);
+    }
+    if (cand_m.empty()) return;
+    // one scan per member into one candidate list (a few per MiB besides the
+    // true headers: LEN/NLEN coincidences)
+    uint64_t cap = 0;
+    for (uint32_t m : cand_m) cap += (mem[m].de - mem[m].ds) / 8 / 8192 + 4096;
+    cap = std::min<uint64_t>(cap, 1ull << 28);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        c->d_counters.ensure(64);
+        c->d_surv.ensure(cap * 8);
+        HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
+        for (uint32_t m : cand_m) {
+            const uint64_t first = mem[m].ds / 8, last = mem[m].de / 8;
+            const uint32_t blocks = (uint32_t)std::min<uint64_t>((last - first + 1023) / 1024 + 1, (uint64_t)c->num_cus * 8);
+            hipLaunchKernelGGL(k_stored_scan, dim3(blocks), dim3(256), 0, c->stream, d_in, in_len, first, last,
+                               c->d_surv.as<unsigned long long>(), (uint32_t)cap, c->d_counters.as<uint32_t>());
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+        }
+        const uint32_t *h_cnt = download<uint32_t>(c, c->d_counters.p, 1);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const uint32_t nc = h_cnt[0];
+        if (nc > cap) {
+            cap = nc + 1024;
+            continue;
+        }
+        const uint64_t *hc = download<uint64_t>(c, c->d_surv.p, nc);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<uint64_t> keys(nc);
+        for (uint32_t i = 0; i < nc; i++) keys[i] = hc[i] & ((1ull << 40) - 1);
+        const std::vector<uint32_t> ord = radix_order(keys);
+        std::vector<StoredRun> runs;
+        for (uint32_t m : cand_m) {
+            Member &M = mem[m];
+            const uint64_t first = M.ds / 8, last = M.de / 8;
+            size_t k = (size_t)(std::lower_bound(ord.begin(), ord.end(), first,
+                                                 [&](uint32_t i, uint64_t v) { return keys[i] < v; }) -
+                                ord.begin());
+            uint64_t p = first, out = 0;
+            const size_t run0 = runs.size();
+            bool ok = false;
+            for (;;) {
+                while (k < ord.size() && keys[ord[k]] < p) k++;
+                if (k == ord.size() || keys[ord[k]] != p) break;  // not a stored header: the chain ends
+                const uint64_t v = hc[ord[k]];
+                const uint32_t len = (uint32_t)(v >> 40) & 0xFFFF;
+                if (len && out + len <= M.out_cap) runs.push_back({p + 5, (uint64_t)(uintptr_t)(M.out + out), len, 0});
+                out += len;
+                p += 5 + len;
+                if (v >> 56) {
+                    ok = p == last;  // the trailer right after the final block
+                    break;
+                }
+                if (p >= last) break;
+            }
+            if (!ok || out > M.out_cap || out != M.exp_isize) {
+                // not pure-stored (or a size the trailer contradicts): the
+                // speculative path decodes it and reports what it finds
+                runs.resize(run0);
+                continue;
+            }
+            M.stored = true;
+            M.out_len = out;
+            M.end_bit = last * 8;
+            c->st.stored_members++;
+            c->st.stored_blocks += (uint32_t)(runs.size() - run0);
+        }
+        if (runs.empty()) return;
+        upload(c, c->d_order, runs.data(), runs.size() * sizeof(StoredRun));
+        hipLaunchKernelGGL(k_stored_copy, dim3((uint32_t)std::min<size_t>(runs.size(), (size_t)c->num_cus * 8)),
+                           dim3(256), 0, c->stream, d_in, in_len, c->d_order.as<StoredRun>(), (uint32_t)runs.size());
+        HIPCHK(hipGetLastError());
+        c->st.launches++;
+        return;
+    }
+}
+
 void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    stored_members(c, d_in, in_len, mem);
     const uint64_t chunk_bits = (c->tune.chunk_bytes ? c->tune.chunk_bytes : 512 * KIB) * 8;
     const uint64_t min_spacing = 8 * MIN_SPACING_KIB * KIB;
     auto min_spacing_of = [&](const Member &M) -> uint64_t {
@@ -800,7 +901,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     auto cost_of = [](const Member &M) -> uint64_t { return member_cost(M.de - M.ds, M.exp_isize); };
     uint64_t total_cost = 0;
     for (auto &M : mem)
-        if (M.status == GZGPU_OK) total_cost += cost_of(M);
+        if (M.status == GZGPU_OK && !M.stored) total_cost += cost_of(M);
     uint64_t target_cost = std::max<uint64_t>(
         MIN_PIECE_OUT, (uint64_t)((double)total_cost / (PIECES_PER_SLOT * SLOT_FRACTION * (double)slots)));
     if (c->max_pieces) target_cost = std::max<uint64_t>(target_cost, total_cost / c->max_pieces + 1);
@@ -812,11 +913,10 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         return std::max<uint64_t>((M.de - M.ds) / npieces, min_spacing_of(M));
     };
 
-    HIPCHK(hipEventRecord(c->ev[0], c->stream));
     // ---- 2/3: find (one pass over every member to be split) + plan
     auto splits = [&](const Member &M) {
         const uint64_t spacing = spacing_of(M);
-        return M.status == GZGPU_OK && M.split && spacing != ~0ull && M.de > M.ds + spacing;
+        return M.status == GZGPU_OK && !M.stored && M.split && spacing != ~0ull && M.de > M.ds + spacing;
     };
     // Output estimates per block (k_bf_estimate) pay off where a member is
     // cut into many pieces: only there can content change along it enough to
@@ -841,7 +941,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     std::vector<uint8_t> sstored, shadow, chain_ok;
     for (uint32_t m = 0; m < mem.size(); m++) {
         Member &M = mem[m];
-        if (M.status != GZGPU_OK) continue;
+        if (M.status != GZGPU_OK || M.stored) continue;
         PieceHost p0;
         p0.d.start_bit = M.ds;
         p0.d.stop_bit = NO_STOP;
@@ -1369,6 +1469,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         if (M.status != GZGPU_OK) continue;
         tile_lo[m] = tile_hi[m] = ntiles;
         rp_lo[m] = rp_hi[m] = (uint32_t)rps.size();
+        if (M.stored) continue;  // copied whole by k_stored_copy; out_len is known
         // pass 1: sizes, offsets, status
         outs.clear();
         uint64_t pos = 0;
@@ -1777,13 +1878,53 @@ std::vector<uint64_t> sized_member_walk(const uint8_t *in, size_t n) {
     return offs;
 }
 
-// Decode the members of a host buffer one after another (multi-member walk,
-// src/decompress/mod.rs:725-741): each member runs the parallel pipeline.
-// Members that all carry their size (gzippy's parallel "GZ" format, BGZF)
-// decode together instead, in one pipeline, like a batch
-// (decompress_bgzf_parallel, src/decompress/bgzf.rs:1874); if any of them
-// fails there, the member-by-member walk decodes the input again and reports
-// the failure where it finds it.
+// Member spans of a gzip stream: candidate starts at 1f 8b 08 with clear
+// reserved flag bits, each span running to the next candidate (the
+// reference's member-boundary scan, scan_member_boundaries_fast,
+// src/decompress/bgzf.rs:2090).  A candidate inside a member's payload cuts
+// it short; decoding finds that out (the member does not end at its span's
+// trailer).
+std::vector<std::pair<size_t, size_t>> member_spans(const uint8_t *p, size_t n) {
+    std::vector<std::pair<size_t, size_t>> v;
+    if (n < 18 || p[0] != 0x1f || p[1] != 0x8b) return v;
+    size_t start = 0;
+    for (size_t q = 18; q + 18 <= n; q++) {
+        const uint8_t *m = (const uint8_t *)memchr(p + q, 0x1f, n - 18 - q + 1);
+        if (!m) break;
+        q = (size_t)(m - p);
+        if (q + 18 > n) break;
+        if (p[q + 1] == 0x8b && p[q + 2] == 8 && (p[q + 3] & 0xE0) == 0 && q - start >= 18) {
+            v.push_back({start, q - start});
+            start = q;
+        }
+    }
+    v.push_back({start, n - start});
+    return v;
+}
+
+// ISIZE of a member span, or ~0 when it cannot belong to that span: deflate
+// expands at most 1032:1 (258-byte copies from 2-bit codes, the reference's
+// MAX_DEFLATE_EXPANSION, chunk_data.rs:334), so a larger ISIZE means a false
+// magic inside the payload cut the span short.
+uint64_t span_isize(const uint8_t *p, size_t len) {
+    if (len < 18) return ~0ull;
+    const uint64_t isz = le32(p + len - 4);
+    return isz > 1032ull * len + 64 ? ~0ull : isz;
+}
+
+// Decode the members of a host buffer (multi-member walk,
+// src/decompress/mod.rs:725-741):
+//   1. members that all carry their size (gzippy's parallel "GZ" format,
+//      BGZF) decode together in one pipeline, like a batch
+//      (decompress_bgzf_parallel, src/decompress/bgzf.rs:1874);
+//   2. else, with more than one candidate member start, the candidate spans
+//      decode together in one pipeline (MultiMemberPar, bgzf.rs:2370, on the
+//      spans of scan_member_boundaries_fast, :2090), accepted when every
+//      member ends exactly at its span's trailer;
+//   3. else, or when any member of those failed, member by member: each
+//      member first bounded by the next candidate start, widened to the rest
+//      of the input only when it does not end there; this walk reports a
+//      failure where it finds it.
 template <typename Sink>
 int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink &&sink, uint64_t *total,
                            gzgpu_stats *stats) {
@@ -1792,108 +1933,141 @@ int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink 
     if (in_len < 2 || in[0] != 0x1f || in[1] != 0x8b) return GZGPU_OK;  // non-gzip: Ok(0)
     c->d_in.ensure(in_len + 64);
     HIPCHK(hipMemcpyAsync(c->d_in.p, in, in_len, hipMemcpyHostToDevice, c->stream));
-    const std::vector<uint64_t> moff = sized_member_walk(in, in_len);
-    if (moff.size() > 2) {
-        gzgpu_tuning tn;
-        gzgpu_get_tuning(c, &tn);
-        std::vector<Member> mem(moff.size() - 1);
+    gzgpu_tuning tn;
+    gzgpu_get_tuning(c, &tn);
+    // members [start, end) decoded together; true (and the output drained)
+    // when every one decoded, verified and, if `exact`, ended at its trailer
+    auto together = [&](const std::vector<std::pair<size_t, size_t>> &sp, bool exact, const char *what,
+                        int &sink_rc) -> bool {
+        std::vector<Member> mem(sp.size());
         uint64_t out_total = 0;
-        bool ok = true;
-        for (size_t i = 0; i < mem.size() && ok; i++) {
+        for (size_t i = 0; i < mem.size(); i++) {
             std::string msg;
-            ok = member_from_host(in + moff[i], moff[i + 1] - moff[i], moff[i], mem[i], msg) == GZGPU_OK;
+            if (member_from_host(in + sp[i].first, sp[i].second, sp[i].first, mem[i], msg) != GZGPU_OK) return false;
             out_total += mem[i].exp_isize;
         }
-        if (ok) {
-            c->d_out.ensure(out_total + 64);
-            uint64_t o = 0;
-            for (size_t i = 0; i < mem.size(); i++) {
-                Member &M = mem[i];
-                M.out = c->d_out.as<uint8_t>() + o;
-                M.out_cap = M.exp_isize;
-                M.split = tn.batch_split_bytes == 0 || moff[i + 1] - moff[i] > tn.batch_split_bytes;
-                o += M.exp_isize;
-            }
-            run_members(c, c->d_in.as<uint8_t>(), in_len, mem);
-            bool all = true;
-            for (auto &M : mem) all = all && M.status == GZGPU_OK;
-            if (debug_enabled()) {
-                size_t bad = 0;
-                while (bad < mem.size() && mem[bad].status == GZGPU_OK) bad++;
-                fprintf(stderr, "[gzgpu] sized members: %zu in one pipeline, %s%s\n", mem.size(),
-                        all ? "all ok" : "first failure: ", all ? "" : mem[bad].msg.c_str());
-            }
-            if (all) {
-                int sr = drain_to_sink(c, c->d_out.as<uint8_t>(), out_total, sink);
-                if (sr) return sr;
-                *total = out_total;
-                c->st.compressed_bytes = in_len;
-                c->st.decompressed_bytes = out_total;
-                if (stats) *stats = c->st;
-                return GZGPU_OK;
-            }
-            c->st = gzgpu_stats{};
+        c->d_out.ensure(out_total + 64);
+        uint64_t o = 0;
+        for (size_t i = 0; i < mem.size(); i++) {
+            Member &M = mem[i];
+            M.out = c->d_out.as<uint8_t>() + o;
+            M.out_cap = M.exp_isize;
+            M.split = tn.batch_split_bytes == 0 || sp[i].second > tn.batch_split_bytes;
+            o += M.exp_isize;
         }
+        run_members(c, c->d_in.as<uint8_t>(), in_len, mem);
+        size_t bad = 0;
+        while (bad < mem.size() && mem[bad].status == GZGPU_OK &&
+               (!exact || (mem[bad].end_bit + 7) / 8 == mem[bad].de / 8))
+            bad++;
+        if (debug_enabled())
+            fprintf(stderr, "[gzgpu] %s: %zu members in one pipeline, %s%s\n", what, mem.size(),
+                    bad == mem.size() ? "all ok" : "first failure: ",
+                    bad == mem.size() ? "" : (mem[bad].msg.empty() ? "not at its span's end" : mem[bad].msg.c_str()));
+        if (bad != mem.size()) {
+            c->st = gzgpu_stats{};
+            return false;
+        }
+        sink_rc = drain_to_sink(c, c->d_out.as<uint8_t>(), out_total, sink);
+        *total = out_total;
+        c->st.compressed_bytes = in_len;
+        c->st.decompressed_bytes = out_total;
+        if (stats) *stats = c->st;
+        return true;
+    };
+    int sink_rc = 0;
+    const std::vector<uint64_t> moff = sized_member_walk(in, in_len);
+    if (moff.size() > 2) {
+        std::vector<std::pair<size_t, size_t>> sp;
+        for (size_t i = 0; i + 1 < moff.size(); i++) sp.push_back({moff[i], moff[i + 1] - moff[i]});
+        if (together(sp, false, "sized members", sink_rc)) return sink_rc;
     }
+    const std::vector<std::pair<size_t, size_t>> spans = member_spans(in, in_len);
+    if (moff.size() <= 2 && spans.size() > 1) {
+        bool plausible = true;
+        for (auto &sp : spans) plausible = plausible && span_isize(in + sp.first, sp.second) != ~0ull;
+        if (plausible && together(spans, true, "candidate members", sink_rc)) return sink_rc;
+    }
+    auto next_candidate = [&](size_t off) -> size_t {
+        auto it = std::upper_bound(spans.begin(), spans.end(), off,
+                                   [](size_t v, const std::pair<size_t, size_t> &sp) { return v < sp.first; });
+        return it == spans.end() ? in_len : it->first;
+    };
     size_t off = 0;
     int members = 0;
     while (off < in_len) {
         if (members > 0 && (in_len - off < 2 || in[off] != 0x1f || in[off + 1] != 0x8b)) break;  // trailing bytes
+        // first the span to the next candidate start, then (if the member
+        // does not end there) the rest of the input
+        const size_t bound = next_candidate(off);
         Member M;
-        std::string msg;
-        int st = member_from_host(in + off, in_len - off, off, M, msg);
-        if (st) {
-            if (members > 0) break;
-            return finish(c, st, msg);
-        }
-        M.split = true;
-        // the member may be followed by others: size the output from the
-        // trailer at the end of the input, growing on overflow
-        uint64_t cap = std::max<uint64_t>((uint64_t)M.exp_isize + 64 * KIB, 1 * MIB);
-        for (int tries = 0;; tries++) {
-            c->d_out.ensure(cap);
-            M.out = c->d_out.as<uint8_t>();
-            M.out_cap = c->d_out.cap;
-            M.status = GZGPU_OK;
-            std::vector<Member> v{M};
-            run_members(c, c->d_in.as<uint8_t>(), in_len, v);
-            M = v[0];
-            if (M.status == GZGPU_OUTPUT_TOO_SMALL && tries < 8) {
-                cap = c->d_out.cap * 4;
+        bool done = false;
+        for (int attempt = bound < in_len ? 0 : 1; attempt < 2 && !done; attempt++) {
+            const size_t avail = (attempt == 0 ? bound : in_len) - off;
+            std::string msg;
+            int st = member_from_host(in + off, avail, off, M, msg);
+            if (st) {
+                if (attempt == 0) continue;
+                if (members > 0) {
+                    off = in_len;  // trailing bytes that are not a member
+                    break;
+                }
+                return finish(c, st, msg);
+            }
+            M.split = true;
+            // the member may be followed by others: size the output from
+            // the trailer at the end of the span, growing on overflow
+            uint64_t cap = std::max<uint64_t>((uint64_t)M.exp_isize + 64 * KIB, 1 * MIB);
+            for (int tries = 0;; tries++) {
+                c->d_out.ensure(cap);
+                M.out = c->d_out.as<uint8_t>();
+                M.out_cap = c->d_out.cap;
+                M.status = GZGPU_OK;
+                std::vector<Member> v{M};
+                run_members(c, c->d_in.as<uint8_t>(), in_len, v);
+                M = v[0];
+                if (M.status == GZGPU_OUTPUT_TOO_SMALL && tries < 8) {
+                    cap = c->d_out.cap * 4;
+                    continue;
+                }
+                break;
+            }
+            if (attempt == 0) {
+                // accepted only when it ends exactly at this span's trailer
+                done = M.status == GZGPU_OK && (M.end_bit + 7) / 8 == M.de / 8;
                 continue;
             }
-            break;
-        }
-        // the member's real end: trailer right after the final block
-        if (M.status == GZGPU_OK || M.status == GZGPU_SIZE_MISMATCH || M.status == GZGPU_CRC_MISMATCH) {
-            uint64_t tr = (M.end_bit + 7) / 8;
-            if (tr + 8 <= in_len && tr + 8 != in_len) {
-                // more bytes follow: re-check against this member's own trailer
-                uint32_t crc = le32(in + tr), isz = le32(in + tr + 4);
-                M.exp_crc = crc;
-                M.exp_isize = isz;
-                if ((uint32_t)M.out_len != isz) {
-                    M.status = GZGPU_SIZE_MISMATCH;
-                    M.msg = "ISIZE mismatch";
-                } else if (M.crc != crc) {
-                    M.status = GZGPU_CRC_MISMATCH;
-                    M.msg = "CRC32 mismatch";
-                } else {
-                    M.status = GZGPU_OK;
+            // the member's real end: trailer right after the final block
+            if (M.status == GZGPU_OK || M.status == GZGPU_SIZE_MISMATCH || M.status == GZGPU_CRC_MISMATCH) {
+                uint64_t tr = (M.end_bit + 7) / 8;
+                if (tr + 8 <= in_len && tr + 8 != in_len) {
+                    // more bytes follow: re-check against this member's own trailer
+                    uint32_t crc = le32(in + tr), isz = le32(in + tr + 4);
+                    M.exp_crc = crc;
+                    M.exp_isize = isz;
+                    if ((uint32_t)M.out_len != isz) {
+                        M.status = GZGPU_SIZE_MISMATCH;
+                        M.msg = "ISIZE mismatch";
+                    } else if (M.crc != crc) {
+                        M.status = GZGPU_CRC_MISMATCH;
+                        M.msg = "CRC32 mismatch";
+                    } else {
+                        M.status = GZGPU_OK;
+                    }
+                } else if (tr + 8 > in_len) {
+                    M.status = GZGPU_DECODE_FAILED;
+                    M.msg = "truncated gzip trailer";
                 }
-            } else if (tr + 8 > in_len) {
-                M.status = GZGPU_DECODE_FAILED;
-                M.msg = "truncated gzip trailer";
             }
             if (M.status != GZGPU_OK) return finish(c, M.status, M.msg);
-            int sr = drain_to_sink(c, M.out, M.out_len, sink);
-            if (sr) return sr;
-            *total += M.out_len;
-            off = tr + 8;
-            members++;
-            continue;
+            done = true;
         }
-        return finish(c, M.status, M.msg);
+        if (!done) break;
+        int sr = drain_to_sink(c, M.out, M.out_len, sink);
+        if (sr) return sr;
+        *total += M.out_len;
+        off = (M.end_bit + 7) / 8 + 8;
+        members++;
     }
     c->st.compressed_bytes = in_len;
     c->st.decompressed_bytes = *total;

@@ -41,26 +41,6 @@ struct ServeRequest {
     uint64_t reserve = 0;  // HBM bytes reserved while admitted
 };
 
-// Member spans of a gzip stream: candidate starts at 1f 8b 08 with clear
-// reserved flag bits, each span running to the next candidate.
-std::vector<std::pair<size_t, size_t>> member_spans(const uint8_t *p, size_t n) {
-    std::vector<std::pair<size_t, size_t>> v;
-    if (n < 18 || p[0] != 0x1f || p[1] != 0x8b) return v;
-    size_t start = 0;
-    for (size_t q = 18; q + 18 <= n; q++) {
-        const uint8_t *m = (const uint8_t *)memchr(p + q, 0x1f, n - 18 - q + 1);
-        if (!m) break;
-        q = (size_t)(m - p);
-        if (q + 18 > n) break;
-        if (p[q + 1] == 0x8b && p[q + 2] == 8 && (p[q + 3] & 0xE0) == 0 && q - start >= 18) {
-            v.push_back({start, q - start});
-            start = q;
-        }
-    }
-    v.push_back({start, n - start});
-    return v;
-}
-
 }  // namespace
 
 struct gzgpu_server {
@@ -79,16 +59,6 @@ struct gzgpu_server {
 };
 
 namespace {
-
-// ISIZE of a member span, or ~0 when it cannot belong to that span: deflate
-// expands at most 1032:1 (258-byte copies from 2-bit codes, the reference's
-// MAX_DEFLATE_EXPANSION, chunk_data.rs:334), so a larger ISIZE means a false
-// magic inside the payload cut the span short.
-uint64_t span_isize(const uint8_t *p, size_t len) {
-    if (len < 18) return ~0ull;
-    const uint64_t isz = le32(p + len - 4);
-    return isz > 1032ull * len + 64 ? ~0ull : isz;
-}
 
 uint64_t request_reserve(const ServeRequest &r) {
     // input (16-byte padded), expected output from the members' trailers, and

@@ -87,6 +87,8 @@ typedef struct {
     uint64_t marker_out_bytes;  /* bytes written by the u16-ring kernel (u16 entries count 2) */
     uint32_t midblock_pieces;   /* pieces planned to start inside a block (at a sync point) */
     uint32_t midblock_linked;   /* of those, linked to their predecessor's exact stop */
+    uint32_t stored_members;    /* members copied by the stored-block split (no speculation) */
+    uint32_t stored_blocks;     /* their stored blocks */
 } gzgpu_stats;
 
 /* ---- context ---------------------------------------------------------- */

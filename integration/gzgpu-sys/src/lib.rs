@@ -70,6 +70,8 @@ pub struct gzgpu_stats {
     pub marker_out_bytes: u64,
     pub midblock_pieces: u32,
     pub midblock_linked: u32,
+    pub stored_members: u32,
+    pub stored_blocks: u32,
 }
 
 #[repr(C)]

@@ -412,3 +412,80 @@ def test_batch_graph_replays_match_batch(gpu):
         status, _, _, _ = g.launch(raise_on_error=False)
         assert np.array_equal(status, ref[0])
     assert ref[0][5] != 0 and ref[0][7] != 0 and ref[0][9] != 0
+
+
+def test_unsized_multi_member_stream_one_pipeline(gpu):
+    """A pigz-style stream of 64 members without size fields: the members are
+    found at their gzip magics and decode together in one pipeline (not one
+    pipeline per member, each over the rest of the input)."""
+    raws = [W.generate(("text", "u32", "runs", "random")[i % 4], 200_000 + 3331 * i, 500 + i) for i in range(64)]
+    gz = b"".join(W.gzip_compress(r, (1, 6, 9)[i % 3]).tobytes() for i, r in enumerate(raws))
+    st = {}
+    out = gpu.decompress(gz, stats=st)
+    say("unsized members", len(gz), {k: st[k] for k in ("pieces", "inflate_launches", "launches")})
+    assert out == b"".join(r.tobytes() for r in raws)
+    assert st["inflate_launches"] <= 4
+    sink = io.BytesIO()
+    assert gpu.decompress_to_writer(gz + b"trailing", sink) == len(out) and sink.getvalue() == out
+
+
+def test_false_member_magic_inside_a_member(gpu):
+    """Gzip magic bytes inside a member's payload (stored data that holds a
+    gzip member) make a false candidate boundary: the span decode does not
+    end at its trailer, and the member walk decodes the stream exactly."""
+    inner = W.gzip_compress(W.generate("text", 300_000, 9), 6).tobytes()
+    payload = bytes(W.generate("random", 100_000, 8)) + inner + bytes(W.generate("random", 100_000, 7))
+    a = W.gzip_compress(np.frombuffer(payload, np.uint8), 0, encoder="zlib").tobytes()
+    b = W.gzip_compress(W.generate("u32", 400_000, 6), 6).tobytes()
+    for gz, want in ((a + b, payload + W.generate("u32", 400_000, 6).tobytes()), (a, payload)):
+        st = {}
+        assert gpu.decompress(gz, stats=st) == want
+        say("false magic", len(gz), st["inflate_launches"])
+
+
+@pytest.mark.parametrize("kind,level", [("random", 6), ("text", 0), ("random", 1)])
+def test_stored_members_split_without_speculation(gpu, kind, level):
+    """A member of stored blocks only (incompressible data, or level 0) is
+    copied block by block from its LEN chain: no block finder, no
+    speculation (StoredParallel, stored_split.rs:265)."""
+    import torch
+    raw = W.generate(kind, (40 << 20) + 4097, seed=60 + level)
+    gz = W.gzip_compress(raw, level, encoder="zlib" if level == 0 else None)
+    d_in = dev(gz)
+    out = torch.zeros(raw.size, dtype=torch.uint8, device="cuda")
+    n, crc, st = gpu.decompress_device(d_in, out)
+    say("stored member", kind, level, gz.size, {k: st[k] for k in ("stored_members", "stored_blocks", "pieces",
+                                                                    "candidates", "ms_total")})
+    assert n == raw.size and crc == zlib.crc32(raw.tobytes())
+    assert np.array_equal(out.cpu().numpy(), raw)
+    assert st["stored_members"] == 1 and st["pieces"] == 0 and st["candidates"] == 0
+    # at an odd output offset and through the host path too
+    out2 = torch.zeros(raw.size + 3, dtype=torch.uint8, device="cuda")
+    n, crc, st = gpu.decompress_device(d_in, out2[3:])
+    assert n == raw.size and np.array_equal(out2[3:].cpu().numpy(), raw) and not out2[:3].any()
+    assert gpu.decompress(gz) == raw.tobytes()
+    # a corrupt payload byte: the CRC says so
+    bad = gz.copy()
+    bad[gz.size // 2] ^= 0x40
+    with pytest.raises(gpu.GzippyError) as e:
+        gpu.decompress(bad)
+    assert e.value.kind == "CrcMismatch"
+
+
+def test_stored_prefix_with_huffman_tail(gpu):
+    """Stored blocks, then Huffman blocks in the same member (ratio still
+    under 1.15): the LEN chain breaks at the first Huffman block, so the
+    member takes the speculative path, byte-exact."""
+    import struct
+    a = W.generate("random", 24 << 20, 71)
+    b = W.generate("text", 1 << 20, 72)
+    c0 = zlib.compressobj(0, zlib.DEFLATED, -15)
+    body = c0.compress(a.tobytes()) + c0.flush(zlib.Z_SYNC_FLUSH)
+    c1 = zlib.compressobj(6, zlib.DEFLATED, -15, 9, zlib.Z_DEFAULT_STRATEGY, a[-32768:].tobytes())
+    body += c1.compress(b.tobytes()) + c1.flush()
+    raw = np.concatenate([a, b]).tobytes()
+    gz = b"\x1f\x8b\x08\x00\x00\x00\x00\x00\x00\xff" + body + struct.pack("<II", zlib.crc32(raw), len(raw))
+    st = {}
+    assert gpu.decompress(gz, stats=st) == raw
+    say("stored prefix", len(gz), {k: st[k] for k in ("stored_members", "pieces", "redecoded")})
+    assert st["stored_members"] == 0
