@@ -30,7 +30,7 @@ from ._native import GzgpuError, Stats, Tuning  # noqa: F401
 __all__ = [
     "GzippyError", "decompress", "decompress_with_threads", "decompress_to_writer",
     "decompress_to_writer_with_threads", "decompress_device", "decompress_batch", "find_blocks", "crc32",
-    "inflate_piece", "context", "Server", "autotune", "BatchGraph",
+    "inflate_piece", "context", "Server", "autotune", "BatchGraph", "build_index", "seek_decompress",
 ]
 
 
@@ -129,6 +129,42 @@ def decompress_to_writer_with_threads(data, writer, threads, stats=None):
     if stats is not None:
         stats.update(st_obj.as_dict())
     return total.value
+
+
+def build_index(data, interval_bytes=1 << 20):
+    """Seek index of a single-member gzip stream (index.rs:54 build_index,
+    :145 serialize_index): bytes in index.rs's version-1 layout."""
+    ctx = context()
+    a = _as_bytes_view(data)
+    need = ctypes.c_size_t()
+    st = ctx.lib.gzgpu_index_build(ctx.h, a.ctypes.data, a.size, int(interval_bytes), None, 0, ctypes.byref(need))
+    if st not in (_native.OK, _native.OUTPUT_TOO_SMALL):
+        _raise(ctx, st)
+    buf = np.empty(need.value, dtype=np.uint8)
+    st = ctx.lib.gzgpu_index_build(ctx.h, a.ctypes.data, a.size, int(interval_bytes), buf.ctypes.data, buf.size,
+                                   ctypes.byref(need))
+    if st != _native.OK:
+        _raise(ctx, st)
+    return buf[: need.value].tobytes()
+
+
+def seek_decompress(data, index, offset, max_bytes=None):
+    """Decompressed bytes [offset, offset + max_bytes) of `data` through its
+    index (index.rs:344 seek_decompress); max_bytes None: to the end."""
+    ctx = context()
+    a = _as_bytes_view(data)
+    ix = _as_bytes_view(index)
+    total = int(ix[10:18].view(np.uint64)[0]) if ix.size >= 18 else 0
+    if max_bytes is None:
+        max_bytes = max(0, total - offset)
+    cap = max(1, min(int(max_bytes), max(0, total - int(offset))))
+    out = np.empty(cap, dtype=np.uint8)
+    n = ctypes.c_size_t()
+    st = ctx.lib.gzgpu_index_seek(ctx.h, a.ctypes.data, a.size, ix.ctypes.data, ix.size, int(offset), int(max_bytes),
+                                  out.ctypes.data, out.size, ctypes.byref(n))
+    if st != _native.OK:
+        _raise(ctx, st)
+    return out[: n.value].tobytes()
 
 
 # ---------------------------------------------------------------- device-resident

@@ -24,7 +24,7 @@ EXPORTS = [
     "gzgpu_decompress_batch", "gzgpu_find_blocks", "gzgpu_crc32", "gzgpu_inflate_piece",
     "gzgpu_server_create", "gzgpu_server_destroy", "gzgpu_submit", "gzgpu_poll", "gzgpu_fetch",
     "gzgpu_server_get_stats", "gzgpu_autotune", "gzgpu_batch_graph_create", "gzgpu_batch_graph_launch",
-    "gzgpu_batch_graph_destroy", "gzgpu_decompress_device_noalloc",
+    "gzgpu_batch_graph_destroy", "gzgpu_decompress_device_noalloc", "gzgpu_index_build", "gzgpu_index_seek",
 ]
 PENDING = 11
 
@@ -121,6 +121,8 @@ def load_library():
         L.gzgpu_decompress_to_writer.argtypes = [vp, vp, sz, WRITE_FN, vp, P(u64), P(Stats)]
         L.gzgpu_decompress_device.argtypes = [vp, vp, sz, vp, sz, P(sz), P(u32), P(Stats)]
         L.gzgpu_decompress_device_noalloc.argtypes = [vp, vp, sz, vp, sz, P(sz), P(u32), P(Stats)]
+        L.gzgpu_index_build.argtypes = [vp, vp, sz, u64, vp, sz, P(sz)]
+        L.gzgpu_index_seek.argtypes = [vp, vp, sz, vp, sz, u64, u64, vp, sz, P(sz)]
         L.gzgpu_decompress_batch.argtypes = [vp, vp, P(u64), P(u64), vp, P(u64), P(u64), sz, P(i32), P(u64),
                                              P(u32), P(Stats)]
         L.gzgpu_find_blocks.argtypes = [vp, vp, sz, u64, u64, P(Boundary), sz, P(sz)]

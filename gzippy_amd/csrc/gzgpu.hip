@@ -343,6 +343,9 @@ struct gzgpu_ctx {
     uint8_t *scratch = nullptr;
     uint8_t *noalloc_stage = nullptr;
     uint32_t max_pieces = 0;  // planner cap on pieces per call (0: none)
+    // index builds: (bit after the member's deflate start, output offset) of
+    // every block start the decode can resume at, collected by the pipeline
+    std::vector<std::pair<uint64_t, uint64_t>> *cp_collect = nullptr;
     hipEvent_t ev[10]{};
     // host-buffer output: two pinned staging chunks, D2H of one while the
     // sink takes the other
@@ -849,6 +852,7 @@ void stored_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
                 if (k == ord.size() || keys[ord[k]] != p) break;  // not a stored header: the chain ends
                 const uint64_t v = hc[ord[k]];
                 const uint32_t len = (uint32_t)(v >> 40) & 0xFFFF;
+                if (c->cp_collect) c->cp_collect->push_back({p * 8 - M.ds, out});
                 if (len && out + len <= M.out_cap) runs.push_back({p + 5, (uint64_t)(uintptr_t)(M.out + out), len, 0});
                 out += len;
                 p += 5 + len;
@@ -1491,6 +1495,8 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 break;
             }
             outs.push_back({pi, pos, L});
+            if (c->cp_collect && r.cp_bit != NO_CP && r.cp_bit >= M.ds)
+                c->cp_collect->push_back({r.cp_bit - M.ds, pos + r.cp_out});
             pos += L;
             if (r.status == PS_FINAL) {
                 final_seen = true;
@@ -2609,3 +2615,4 @@ int gzgpu_inflate_piece(gzgpu_ctx *c, const void *d_in, size_t in_len, uint64_t 
 
 #include "serve.h"
 #include "graph.h"
+#include "index.h"

@@ -68,7 +68,13 @@ struct PieceOut {
     uint32_t err;
     uint32_t nblocks;
     uint32_t mode;
+    // the first block header the piece decodes at or after its start (NO_CP:
+    // none), and the piece's entries before it: a point where a decode can
+    // resume with the 32 KiB of output before it as the window
+    uint64_t cp_bit;
+    uint64_t cp_out;
 };
+constexpr uint64_t NO_CP = ~0ull;
 
 // Ring of the most recent entries in LDS: 2048 u16 or 4096 u8 entries, 4 KiB
 // either way, so that with the LUTs a wavefront needs 10 KB and 16 share a
@@ -1039,6 +1045,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
     uint32_t status = PS_NONE, err = E_NONE, nblocks = 0;
     uint64_t end_bit = 0;
     uint64_t t_begin = wall_clock64();
+    uint64_t cp_bit = NO_CP, cp_out = 0;
     uint32_t fixed_built = 0;  // the fixed tables are in the LUTs
 
     // ring = the last RING entries of the window before the piece (piece
@@ -1088,6 +1095,10 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
             status = PS_STOPPED;
             end_bit = hbit;
             break;
+        }
+        if (cp_bit == NO_CP && hbit >= pd.start_bit) {
+            cp_bit = hbit;
+            cp_out = o.seg0 + o.pos;
         }
         nblocks++;
         uint32_t r = R_EOB;
@@ -1239,6 +1250,8 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
         po->err = err;
         po->nblocks = nblocks;
         po->mode = marker ? (flipped ? 2u : 1u) : 0u;
+        po->cp_bit = cp_bit;
+        po->cp_out = cp_out;
     }
 }
 

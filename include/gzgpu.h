@@ -219,6 +219,26 @@ typedef struct {
 int gzgpu_inflate_piece(gzgpu_ctx *ctx, const void *d_in, size_t in_len, uint64_t start_bit, uint64_t stop_bit,
                         const uint8_t *window, void *d_out, size_t out_cap_elems, gzgpu_piece_result *res);
 
+/* ---- seekable index ------------------------------------------------------ */
+/* Random access into a single-member gzip stream (src/decompress/index.rs).
+ * gzgpu_index_build (index.rs:54 build_index + :145 serialize_index) decodes
+ * the member once and writes an index in index.rs's version-1 layout
+ * ("GZIDX\x01", deflate offset u32, total size u64, point count u32, 2
+ * reserved bytes; per point: bit offset from the deflate start u64, output
+ * offset u64, 32 KiB window) with checkpoints at block starts at least
+ * `interval` output bytes apart.  *index_len receives the size needed; with
+ * cap below it the call returns GZGPU_OUTPUT_TOO_SMALL.  Multi-member input
+ * and members that carry their size (BGZF) are refused (GZGPU_INVALID_ARGUMENT),
+ * as index.rs:61-72 does.
+ * gzgpu_index_seek (index.rs:184 load_index + :344 seek_decompress) writes the
+ * output bytes [offset, offset + max_bytes) (clamped to the stream's end) to
+ * out: the checkpoints around the range decode in parallel from their
+ * windows; an index that does not belong to the input is an error. */
+int gzgpu_index_build(gzgpu_ctx *ctx, const uint8_t *in, size_t in_len, uint64_t interval, uint8_t *index,
+                      size_t cap, size_t *index_len);
+int gzgpu_index_seek(gzgpu_ctx *ctx, const uint8_t *in, size_t in_len, const uint8_t *index, size_t index_len,
+                     uint64_t offset, uint64_t max_bytes, uint8_t *out, size_t out_cap, size_t *out_len);
+
 /* ---- serving: continuous batching --------------------------------------- */
 /* A request is one gzip stream (one member or several concatenated, like
  * gzippy::decompress on that buffer: src/lib.rs:176, src/decompress/mod.rs:725-741).
