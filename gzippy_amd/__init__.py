@@ -31,6 +31,7 @@ __all__ = [
     "GzippyError", "decompress", "decompress_with_threads", "decompress_to_writer",
     "decompress_to_writer_with_threads", "decompress_device", "decompress_batch", "find_blocks", "crc32",
     "inflate_piece", "context", "Server", "autotune", "BatchGraph", "build_index", "seek_decompress",
+    "decompress_members_plain",
 ]
 
 
@@ -241,6 +242,33 @@ def decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None, ra
     st_obj = Stats()
     P64 = ctypes.POINTER(ctypes.c_uint64)
     st = ctx.lib.gzgpu_decompress_batch(
+        ctx.h, _ptr(d_in), a[0].ctypes.data_as(P64), a[1].ctypes.data_as(P64), _ptr(d_out),
+        a[2].ctypes.data_as(P64), a[3].ctypes.data_as(P64), n,
+        status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), olen.ctypes.data_as(P64),
+        crcs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(st_obj))
+    if st != _native.OK and raise_on_error:
+        _raise(ctx, st)
+    return status, olen, crcs, st_obj.as_dict()
+
+
+def decompress_members_plain(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None, raise_on_error=True):
+    """The plain per-member form: one inflate launch per member, back to back,
+    no host synchronisation in between.  Returns (status[], out_len[], crc[],
+    stats dict)."""
+    _check(d_in, "d_in")
+    _check(d_out, "d_out")
+    _check_ranges("input", in_off, in_len, d_in)
+    _check_ranges("output", out_off, out_cap, d_out)
+    ctx = ctx or context(d_in.device.index or 0)
+    _sync_torch(d_in)
+    n = len(in_off)
+    a = [np.ascontiguousarray(np.asarray(x, dtype=np.uint64)) for x in (in_off, in_len, out_off, out_cap)]
+    status = np.zeros(n, dtype=np.int32)
+    olen = np.zeros(n, dtype=np.uint64)
+    crcs = np.zeros(n, dtype=np.uint32)
+    st_obj = Stats()
+    P64 = ctypes.POINTER(ctypes.c_uint64)
+    st = ctx.lib.gzgpu_decompress_members_plain(
         ctx.h, _ptr(d_in), a[0].ctypes.data_as(P64), a[1].ctypes.data_as(P64), _ptr(d_out),
         a[2].ctypes.data_as(P64), a[3].ctypes.data_as(P64), n,
         status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), olen.ctypes.data_as(P64),

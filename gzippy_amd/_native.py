@@ -25,6 +25,7 @@ EXPORTS = [
     "gzgpu_server_create", "gzgpu_server_destroy", "gzgpu_submit", "gzgpu_poll", "gzgpu_fetch",
     "gzgpu_server_get_stats", "gzgpu_autotune", "gzgpu_batch_graph_create", "gzgpu_batch_graph_launch",
     "gzgpu_batch_graph_destroy", "gzgpu_decompress_device_noalloc", "gzgpu_index_build", "gzgpu_index_seek",
+    "gzgpu_decompress_members_plain",
 ]
 PENDING = 11
 
@@ -122,6 +123,8 @@ def load_library():
         L.gzgpu_decompress_device.argtypes = [vp, vp, sz, vp, sz, P(sz), P(u32), P(Stats)]
         L.gzgpu_decompress_device_noalloc.argtypes = [vp, vp, sz, vp, sz, P(sz), P(u32), P(Stats)]
         L.gzgpu_index_build.argtypes = [vp, vp, sz, u64, vp, sz, P(sz)]
+        L.gzgpu_decompress_members_plain.argtypes = [vp, vp, P(u64), P(u64), vp, P(u64), P(u64), sz, P(i32), P(u64),
+                                                     P(u32), P(Stats)]
         L.gzgpu_index_seek.argtypes = [vp, vp, sz, vp, sz, u64, u64, vp, sz, P(sz)]
         L.gzgpu_decompress_batch.argtypes = [vp, vp, P(u64), P(u64), vp, P(u64), P(u64), sz, P(i32), P(u64),
                                              P(u32), P(Stats)]

@@ -134,8 +134,11 @@ __global__ void k_graph_verify(const PieceOut *__restrict__ pout, const GraphHea
 struct gzgpu_batch_graph {
     gzgpu_ctx *c = nullptr;
     uint32_t n = 0, nseg = 0;
+    const uint8_t *in_base = nullptr;
+    uint8_t *d_out = nullptr;
+    uint64_t in_total = 0;
     DevBuf members, pieces, order, pout, queue, head, seg_member, seg_start, seg_off, seg_len, seg_tail, mseg, mlen,
-        seg_crc, mcrc, status, olen, ocrc;
+        seg_crc, mcrc, status, olen, ocrc, zero;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void release() {
@@ -144,10 +147,137 @@ struct gzgpu_batch_graph {
         exec = nullptr;
         graph = nullptr;
         for (DevBuf *b : {&members, &pieces, &order, &pout, &queue, &head, &seg_member, &seg_start, &seg_off, &seg_len,
-                          &seg_tail, &mseg, &mlen, &seg_crc, &mcrc, &status, &olen, &ocrc})
+                          &seg_tail, &mseg, &mlen, &seg_crc, &mcrc, &status, &olen, &ocrc, &zero})
             b->release();
     }
 };
+
+namespace {
+
+// Buffers of a fixed batch layout (members, their CRC segments, longest-first
+// order), uploaded once.
+void graph_setup(gzgpu_batch_graph *g, const void *d_in, const uint64_t *in_off, const uint64_t *in_len,
+                 void *d_out, const uint64_t *out_off, const uint64_t *out_cap, size_t n) {
+    gzgpu_ctx *c = g->c;
+    const AlignedIn ai = align_input(d_in);
+    g->in_base = ai.base;
+    g->d_out = (uint8_t *)d_out;
+    uint64_t in_total = 0;
+    std::vector<GraphMember> gm(n);
+    std::vector<uint32_t> seg_member, mseg;
+    std::vector<uint64_t> seg_start, seg_off;
+    for (size_t i = 0; i < n; i++) {
+        gm[i] = {ai.shift + in_off[i], in_len[i], out_off[i], out_cap[i]};
+        in_total = std::max<uint64_t>(in_total, ai.shift + in_off[i] + in_len[i]);
+        mseg.push_back((uint32_t)seg_off.size());
+        // segment boundaries on 16-byte addresses (full aligned frames
+        // but for a member's first and last segment)
+        const uint64_t lead = ((uintptr_t)d_out + out_off[i]) & 15;
+        for (uint64_t o = 0; o < out_cap[i]; o += o ? CRC_SEG : CRC_SEG - lead) {
+            seg_member.push_back((uint32_t)i);
+            seg_start.push_back(o);
+            seg_off.push_back(out_off[i] + o);
+        }
+    }
+    g->in_total = in_total;
+    mseg.push_back((uint32_t)seg_off.size());
+    if (seg_off.size() > 0xFFFFFFFFull) throw StatusError{GZGPU_INVALID_ARGUMENT, "too many CRC segments"};
+    g->nseg = (uint32_t)seg_off.size();
+    // longest compressed member first
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });
+    auto put = [&](DevBuf &b, const void *src, size_t bytes) {
+        b.ensure(std::max<size_t>(bytes, 16));
+        if (bytes) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    put(g->members, gm.data(), n * sizeof(GraphMember));
+    put(g->order, order.data(), n * 4);
+    put(g->seg_member, seg_member.data(), seg_member.size() * 4);
+    put(g->seg_start, seg_start.data(), seg_start.size() * 8);
+    put(g->seg_off, seg_off.data(), seg_off.size() * 8);
+    put(g->mseg, mseg.data(), mseg.size() * 4);
+    g->pieces.ensure(n * sizeof(PieceDesc));
+    g->pout.ensure(n * sizeof(PieceOut));
+    g->queue.ensure(std::max<size_t>(64, n * 4));
+    g->head.ensure(n * sizeof(GraphHead));
+    g->seg_len.ensure(std::max<size_t>(g->nseg, 1) * 4);
+    g->seg_tail.ensure(std::max<size_t>(g->nseg, 1) * 8);
+    g->seg_crc.ensure(std::max<size_t>(g->nseg, 1) * 4);
+    g->mlen.ensure(n * 8);
+    g->mcrc.ensure(n * 4);
+    g->status.ensure(n * 4);
+    g->olen.ensure(n * 8);
+    g->ocrc.ensure(n * 4);
+    ensure_crc_tables(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// The decode of the layout on the context's stream.  plain == false: one
+// persistent inflate launch over every member (the graph's body);
+// plain == true: one inflate launch per member, one wavefront each, queued
+// back to back with no host synchronisation in between (the plain
+// one-launch-per-member form the batched forms are measured against).
+void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
+    gzgpu_ctx *c = g->c;
+    const uint32_t nb = (g->n + 255) / 256, sb = (std::max(g->nseg, g->n) + 255) / 256;
+    hipLaunchKernelGGL(k_graph_prep, dim3(nb), dim3(256), 0, c->stream, g->in_base, g->d_out,
+                       g->members.as<GraphMember>(), g->n, g->pieces.as<PieceDesc>(), g->head.as<GraphHead>(),
+                       g->queue.as<uint32_t>());
+    if (!plain) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(g->n, (uint64_t)c->num_cus * U8_WAVES_PER_CU);
+        hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(grid), dim3(64), 0, c->stream, g->in_base, g->in_total,
+                           g->pieces.as<PieceDesc>(), g->order.as<uint32_t>(), g->n, g->queue.as<uint32_t>(),
+                           g->pout.as<PieceOut>());
+    } else {
+        // member i: piece i, a queue counter of its own (cleared first), order
+        // entry 0 of the identity (the kernel decodes pieces[order[0]])
+        HIPCHK(hipMemsetAsync(g->queue.p, 0, (size_t)g->n * 4, c->stream));
+        for (uint32_t i = 0; i < g->n; i++)
+            hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(1), dim3(64), 0, c->stream, g->in_base, g->in_total,
+                               g->pieces.as<PieceDesc>() + i, g->zero.as<uint32_t>(), 1u, g->queue.as<uint32_t>() + i,
+                               g->pout.as<PieceOut>() + i);
+    }
+    hipLaunchKernelGGL(k_graph_seglen, dim3(sb), dim3(256), 0, c->stream, g->pout.as<PieceOut>(),
+                       g->seg_member.as<uint32_t>(), g->seg_start.as<uint64_t>(), g->nseg, g->n,
+                       g->seg_len.as<uint32_t>(), g->seg_tail.as<uint64_t>(), g->mlen.as<uint64_t>());
+    if (g->nseg)
+        hipLaunchKernelGGL(k_crc_segments, dim3(std::min<uint32_t>(g->nseg, (uint32_t)c->num_cus * 8)), dim3(256), 0,
+                           c->stream, (const uint8_t *)g->d_out, g->seg_off.as<uint64_t>(), g->seg_len.as<uint32_t>(),
+                           g->seg_tail.as<uint64_t>(), g->nseg, c->d_crc_tab.as<uint32_t>(), g->seg_crc.as<uint32_t>());
+    hipLaunchKernelGGL(k_crc_members, dim3(std::min<uint32_t>(g->n, (uint32_t)c->num_cus * 8)), dim3(256), 0,
+                       c->stream, g->seg_crc.as<uint32_t>(), g->mseg.as<uint32_t>(), g->mlen.as<uint64_t>(), g->n,
+                       g->mcrc.as<uint32_t>());
+    hipLaunchKernelGGL(k_graph_verify, dim3(nb), dim3(256), 0, c->stream, g->pout.as<PieceOut>(),
+                       g->head.as<GraphHead>(), g->mcrc.as<uint32_t>(), g->n, g->status.as<int32_t>(),
+                       g->olen.as<uint64_t>(), g->ocrc.as<uint32_t>());
+}
+
+// per-member results back to the caller; the first failing member's status
+int graph_results(gzgpu_batch_graph *g, int32_t *status, uint64_t *out_len, uint32_t *crc) {
+    gzgpu_ctx *c = g->c;
+    const int32_t *hs = download<int32_t>(c, g->status.p, g->n);
+    const uint64_t *hl = download<uint64_t>(c, g->olen.p, g->n);
+    const uint32_t *hc = download<uint32_t>(c, g->ocrc.p, g->n);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int first_bad = GZGPU_OK;
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < g->n; i++) {
+        if (status) status[i] = hs[i];
+        if (out_len) out_len[i] = hl[i];
+        if (crc) crc[i] = hc[i];
+        total += hl[i];
+        if (hs[i] != GZGPU_OK && first_bad == GZGPU_OK) {
+            first_bad = hs[i];
+            set_err(c, "member " + std::to_string(i) + ": " + gzgpu_status_string(hs[i]));
+        }
+    }
+    c->st.decompressed_bytes = total;
+    c->st.pieces = g->n;
+    return first_bad;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -162,79 +292,9 @@ int gzgpu_batch_graph_create(gzgpu_ctx *c, const void *d_in, const uint64_t *in_
     g->n = (uint32_t)n;
     const int st = guarded(c, [&]() {
         HIPCHK(hipSetDevice(c->device));
-        const AlignedIn ai = align_input(d_in);
-        uint64_t in_total = 0;
-        std::vector<GraphMember> gm(n);
-        std::vector<uint32_t> seg_member, mseg;
-        std::vector<uint64_t> seg_start, seg_off;
-        for (size_t i = 0; i < n; i++) {
-            gm[i] = {ai.shift + in_off[i], in_len[i], out_off[i], out_cap[i]};
-            in_total = std::max<uint64_t>(in_total, ai.shift + in_off[i] + in_len[i]);
-            mseg.push_back((uint32_t)seg_off.size());
-            // segment boundaries on 16-byte addresses (full aligned frames
-            // but for a member's first and last segment)
-            const uint64_t lead = ((uintptr_t)d_out + out_off[i]) & 15;
-            for (uint64_t o = 0; o < out_cap[i]; o += o ? CRC_SEG : CRC_SEG - lead) {
-                seg_member.push_back((uint32_t)i);
-                seg_start.push_back(o);
-                seg_off.push_back(out_off[i] + o);
-            }
-        }
-        mseg.push_back((uint32_t)seg_off.size());
-        if (seg_off.size() > 0xFFFFFFFFull) throw StatusError{GZGPU_INVALID_ARGUMENT, "too many CRC segments"};
-        g->nseg = (uint32_t)seg_off.size();
-        // longest compressed member first
-        std::vector<uint32_t> order(n);
-        for (uint32_t i = 0; i < n; i++) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });
-        auto put = [&](DevBuf &b, const void *src, size_t bytes) {
-            b.ensure(std::max<size_t>(bytes, 16));
-            if (bytes) HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
-        };
-        put(g->members, gm.data(), n * sizeof(GraphMember));
-        put(g->order, order.data(), n * 4);
-        put(g->seg_member, seg_member.data(), seg_member.size() * 4);
-        put(g->seg_start, seg_start.data(), seg_start.size() * 8);
-        put(g->seg_off, seg_off.data(), seg_off.size() * 8);
-        put(g->mseg, mseg.data(), mseg.size() * 4);
-        g->pieces.ensure(n * sizeof(PieceDesc));
-        g->pout.ensure(n * sizeof(PieceOut));
-        g->queue.ensure(64);
-        g->head.ensure(n * sizeof(GraphHead));
-        g->seg_len.ensure(std::max<size_t>(g->nseg, 1) * 4);
-        g->seg_tail.ensure(std::max<size_t>(g->nseg, 1) * 8);
-        g->seg_crc.ensure(std::max<size_t>(g->nseg, 1) * 4);
-        g->mlen.ensure(n * 8);
-        g->mcrc.ensure(n * 4);
-        g->status.ensure(n * 4);
-        g->olen.ensure(n * 8);
-        g->ocrc.ensure(n * 4);
-        ensure_crc_tables(c);
-        HIPCHK(hipStreamSynchronize(c->stream));
-
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(n, (uint64_t)c->num_cus * U8_WAVES_PER_CU);
-        const uint32_t nb = (g->n + 255) / 256, sb = (std::max(g->nseg, g->n) + 255) / 256;
+        graph_setup(g.get(), d_in, in_off, in_len, d_out, out_off, out_cap, n);
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipLaunchKernelGGL(k_graph_prep, dim3(nb), dim3(256), 0, c->stream, ai.base, (uint8_t *)d_out,
-                           g->members.as<GraphMember>(), g->n, g->pieces.as<PieceDesc>(), g->head.as<GraphHead>(),
-                           g->queue.as<uint32_t>());
-        hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(grid), dim3(64), 0, c->stream, ai.base, in_total,
-                           g->pieces.as<PieceDesc>(), g->order.as<uint32_t>(), g->n, g->queue.as<uint32_t>(),
-                           g->pout.as<PieceOut>());
-        hipLaunchKernelGGL(k_graph_seglen, dim3(sb), dim3(256), 0, c->stream, g->pout.as<PieceOut>(),
-                           g->seg_member.as<uint32_t>(), g->seg_start.as<uint64_t>(), g->nseg, g->n,
-                           g->seg_len.as<uint32_t>(), g->seg_tail.as<uint64_t>(), g->mlen.as<uint64_t>());
-        if (g->nseg)
-            hipLaunchKernelGGL(k_crc_segments, dim3(std::min<uint32_t>(g->nseg, (uint32_t)c->num_cus * 8)), dim3(256), 0,
-                               c->stream, (const uint8_t *)d_out, g->seg_off.as<uint64_t>(), g->seg_len.as<uint32_t>(),
-                               g->seg_tail.as<uint64_t>(), g->nseg, c->d_crc_tab.as<uint32_t>(),
-                               g->seg_crc.as<uint32_t>());
-        hipLaunchKernelGGL(k_crc_members, dim3(std::min<uint32_t>(g->n, (uint32_t)c->num_cus * 8)), dim3(256), 0,
-                           c->stream, g->seg_crc.as<uint32_t>(), g->mseg.as<uint32_t>(), g->mlen.as<uint64_t>(), g->n,
-                           g->mcrc.as<uint32_t>());
-        hipLaunchKernelGGL(k_graph_verify, dim3(nb), dim3(256), 0, c->stream, g->pout.as<PieceOut>(),
-                           g->head.as<GraphHead>(), g->mcrc.as<uint32_t>(), g->n, g->status.as<int32_t>(),
-                           g->olen.as<uint64_t>(), g->ocrc.as<uint32_t>());
+        graph_enqueue(g.get(), false);
         const hipError_t le = hipGetLastError();
         const hipError_t ce = hipStreamEndCapture(c->stream, &g->graph);
         HIPCHK(le);
@@ -250,6 +310,36 @@ int gzgpu_batch_graph_create(gzgpu_ctx *c, const void *d_in, const uint64_t *in_
     return GZGPU_OK;
 }
 
+int gzgpu_decompress_members_plain(gzgpu_ctx *c, const void *d_in, const uint64_t *in_off, const uint64_t *in_len,
+                                   void *d_out, const uint64_t *out_off, const uint64_t *out_cap, size_t n,
+                                   int32_t *status, uint64_t *out_len, uint32_t *crc, gzgpu_stats *stats) {
+    if (!c || !n || !d_in || !d_out || !in_off || !in_len || !out_off || !out_cap || n > 0xFFFFFFFFu)
+        return GZGPU_INVALID_ARGUMENT;
+    gzgpu_batch_graph g;
+    g.c = c;
+    g.n = (uint32_t)n;
+    const int st = guarded(c, [&]() {
+        HIPCHK(hipSetDevice(c->device));
+        c->st = gzgpu_stats{};
+        graph_setup(&g, d_in, in_off, in_len, d_out, out_off, out_cap, n);
+        const uint32_t zero = 0;
+        g.zero.ensure(16);
+        HIPCHK(hipMemcpy(g.zero.p, &zero, 4, hipMemcpyHostToDevice));
+        HIPCHK(hipEventRecord(c->ev[0], c->stream));
+        graph_enqueue(&g, true);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        const int r = graph_results(&g, status, out_len, crc);
+        c->st.ms_total = ev_ms(c->ev[0], c->ev[4]);
+        c->st.launches = g.n + 5;
+        c->st.inflate_launches = g.n;
+        if (stats) *stats = c->st;
+        return r;
+    });
+    g.release();
+    return st;
+}
+
 int gzgpu_batch_graph_launch(gzgpu_batch_graph *g, int32_t *status, uint64_t *out_len, uint32_t *crc,
                              gzgpu_stats *stats) {
     if (!g || !g->c) return GZGPU_INVALID_ARGUMENT;
@@ -260,25 +350,8 @@ int gzgpu_batch_graph_launch(gzgpu_batch_graph *g, int32_t *status, uint64_t *ou
         HIPCHK(hipEventRecord(c->ev[0], c->stream));
         HIPCHK(hipGraphLaunch(g->exec, c->stream));
         HIPCHK(hipEventRecord(c->ev[4], c->stream));
-        const int32_t *hs = download<int32_t>(c, g->status.p, g->n);
-        const uint64_t *hl = download<uint64_t>(c, g->olen.p, g->n);
-        const uint32_t *hc = download<uint32_t>(c, g->ocrc.p, g->n);
-        HIPCHK(hipStreamSynchronize(c->stream));
-        int first_bad = GZGPU_OK;
-        uint64_t total = 0;
-        for (uint32_t i = 0; i < g->n; i++) {
-            if (status) status[i] = hs[i];
-            if (out_len) out_len[i] = hl[i];
-            if (crc) crc[i] = hc[i];
-            total += hl[i];
-            if (hs[i] != GZGPU_OK && first_bad == GZGPU_OK) {
-                first_bad = hs[i];
-                set_err(c, "member " + std::to_string(i) + ": " + gzgpu_status_string(hs[i]));
-            }
-        }
+        const int first_bad = graph_results(g, status, out_len, crc);
         c->st.ms_total = ev_ms(c->ev[0], c->ev[4]);
-        c->st.decompressed_bytes = total;
-        c->st.pieces = g->n;
         c->st.launches = 1;
         if (stats) *stats = c->st;
         return first_bad;

@@ -186,6 +186,15 @@ int gzgpu_batch_graph_launch(gzgpu_batch_graph *g, int32_t *status, uint64_t *ou
                              gzgpu_stats *stats);
 void gzgpu_batch_graph_destroy(gzgpu_batch_graph *g);
 
+/* The plain per-member form: the same members decoded with one inflate launch
+ * per member (one wavefront each, whole member), queued back to back on the
+ * context's stream with no host synchronisation in between; header parse,
+ * CRC32 and trailer checks on the device as in the graph.  The baseline the
+ * batched forms are measured against.  No counterpart in the reference. */
+int gzgpu_decompress_members_plain(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_off, const uint64_t *in_len,
+                                   void *d_out, const uint64_t *out_off, const uint64_t *out_cap, size_t n,
+                                   int32_t *status, uint64_t *out_len, uint32_t *crc, gzgpu_stats *stats);
+
 /* ---- exposed kernels (parity tests, tools) ------------------------------- */
 typedef struct {
     uint64_t bit_offset; /* earliest valid start */
