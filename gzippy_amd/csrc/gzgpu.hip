@@ -28,6 +28,7 @@
 #include <cstring>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -351,6 +352,14 @@ struct gzgpu_ctx {
     // sink takes the other
     uint8_t *drain[2]{};
     hipEvent_t drain_ev[2]{};
+    // streaming output (a single member to a sink): resolve in ranges, each
+    // range's end event, D2H on a copy stream of their own while later ranges
+    // resolve and the CRC runs
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t res_ev[8]{};
+    std::function<int(const uint8_t *, size_t)> *stream_sink = nullptr;
+    bool streamed = false;
+    int stream_rc = 0;
     gzgpu_stats st{};
 };
 
@@ -785,6 +794,13 @@ void run_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector
         if (M.end_bit >= sb) M.end_bit -= sb;
     }
 }
+
+// Streaming output: a single member of at least STREAM_MIN bytes to a sink
+// resolves in STREAM_RANGES launches, and the D2H of each finished range
+// overlaps the later ones.
+constexpr uint32_t STREAM_RANGES = 8;
+constexpr uint64_t STREAM_MIN = 16 * MIB;
+int drain_ranges(gzgpu_ctx *c, const uint8_t *d, uint64_t n, const std::vector<std::pair<uint64_t, int>> &ready);
 
 // StoredParallel (decompress_stored_parallel, stored_split.rs:265; routed by
 // parallel_sm_unprofitable && first_block_is_stored, decompress/mod.rs:282-286):
@@ -1665,6 +1681,9 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     if (debug_enabled())
         fprintf(stderr, "[gzgpu] host: link %.3f ms, assemble %.3f ms, crc lists + stage %.3f ms (%zu KB)\n",
                 tlink - tpost, tasm - tlink, host_ms() - tasm, lists_bytes >> 10);
+    const bool streaming = c->stream_sink && mem.size() == 1 && mem[0].status == GZGPU_OK && !mem[0].stored &&
+                           ntiles > 0 && mem[0].out_len >= STREAM_MIN;
+    std::vector<std::pair<uint64_t, int>> ready;  // (final output prefix, event) per resolve range
     if (!rps.empty()) {
         const uint32_t nch = (uint32_t)(chains.size() / 2);  // (first, end) pairs
         HIPCHK(hipEventRecord(c->ev[7], c->stream));
@@ -1699,11 +1718,29 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         if (ntiles) {
             ensure_crc_tables(c);
             c->d_tile_crc.ensure((size_t)ntiles * 4);
-            hipLaunchKernelGGL(k_resolve_body, dim3((uint32_t)std::min<size_t>(ntiles, (size_t)c->num_cus * 32)), dim3(256),
-                               0, c->stream, (const ResolvePiece *)part(0), (const uint32_t *)part(2),
-                               (uint32_t)rps.size(), c->d_crc_tab.as<uint32_t>(), c->d_tile_crc.as<uint32_t>());
-            HIPCHK(hipGetLastError());
-            c->st.launches++;
+            // streaming: the tiles in ranges, in output order; after range r
+            // every output byte before the first tile of range r + 1 is final
+            // (the rest of the member is written by the inflate and k_chain)
+            const uint32_t nr = streaming ? std::min<uint32_t>(STREAM_RANGES, ntiles) : 1;
+            for (uint32_t r = 0; r < nr; r++) {
+                const uint32_t t0 = (uint32_t)((uint64_t)ntiles * r / nr), t1 = (uint32_t)((uint64_t)ntiles * (r + 1) / nr);
+                hipLaunchKernelGGL(k_resolve_body, dim3((uint32_t)std::min<size_t>(t1 - t0, (size_t)c->num_cus * 32)),
+                                   dim3(256), 0, c->stream, (const ResolvePiece *)part(0), (const uint32_t *)part(2),
+                                   (uint32_t)rps.size(), c->d_crc_tab.as<uint32_t>(), c->d_tile_crc.as<uint32_t>(),
+                                   t0, t1);
+                HIPCHK(hipGetLastError());
+                c->st.launches++;
+                if (streaming) {
+                    HIPCHK(hipEventRecord(c->res_ev[r], c->stream));
+                    uint64_t prefix = mem[0].out_len;
+                    if (r + 1 < nr) {
+                        const size_t k = (size_t)(std::upper_bound(tile_first.begin(), tile_first.end() - 1, t1) -
+                                                  tile_first.begin()) - 1;
+                        prefix = rps[k].out_pos + (uint64_t)(t1 - tile_first[k]) * RESOLVE_TILE;
+                    }
+                    ready.push_back({prefix, (int)r});
+                }
+            }
         }
     }
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
@@ -1729,6 +1766,13 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         c->st.launches++;
         const uint32_t *crcs = download<uint32_t>(c, c->d_mcrc.p, nm);
         HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        if (streaming) {
+            // the output reaches the sink range by range while later ranges
+            // resolve and the CRC runs; a CRC or size error is reported after
+            // (partial output before a terminal error, single_member.rs:17-22)
+            c->stream_rc = drain_ranges(c, mem[0].out, mem[0].out_len, ready);
+            c->streamed = true;
+        }
         HIPCHK(hipStreamSynchronize(c->stream));
         for (uint32_t i = 0; i < nm; i++) {
             Member &M = mem[crc_members[i]];
@@ -1845,6 +1889,46 @@ int drain_to_sink(gzgpu_ctx *c, const uint8_t *d, uint64_t n, Sink &sink) {
         if (i + 1 < chunks) enqueue(i + 1);
         HIPCHK(hipEventSynchronize(c->drain_ev[i & 1]));
         if (!rc) rc = sink(c->drain[i & 1], (size_t)std::min(DRAIN_CHUNK, n - i * DRAIN_CHUNK));
+    }
+    return rc;
+}
+
+// Hand the member's output to the context's streaming sink in order, in
+// DRAIN_CHUNK pieces: the D2H of a piece waits (on the copy stream) for the
+// resolve range that finishes it, so it overlaps the later ranges and the
+// CRC; two pinned buffers let the sink take one piece while the next copies.
+int drain_ranges(gzgpu_ctx *c, const uint8_t *d, uint64_t n, const std::vector<std::pair<uint64_t, int>> &ready) {
+    if (!n) return 0;
+    for (int i = 0; i < 2; i++) {
+        if (!c->drain[i]) {
+            if (hipHostMalloc((void **)&c->drain[i], DRAIN_CHUNK, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                c->drain[i] = nullptr;
+                throw StatusError{GZGPU_GPU_UNAVAILABLE, "hipHostMalloc(output staging) failed"};
+            }
+        }
+        if (!c->drain_ev[i]) HIPCHK(hipEventCreateWithFlags(&c->drain_ev[i], hipEventDisableTiming));
+    }
+    const uint64_t chunks = (n + DRAIN_CHUNK - 1) / DRAIN_CHUNK;
+    size_t r = 0;  // the first range not yet waited for
+    int waited = -1;
+    auto enqueue = [&](uint64_t i) {
+        const uint64_t off = i * DRAIN_CHUNK, len = std::min(DRAIN_CHUNK, n - off);
+        while (r < ready.size() && ready[r].first < off + len) r++;
+        const int ev = ready[std::min(r, ready.size() - 1)].second;
+        if (ev > waited) {
+            HIPCHK(hipStreamWaitEvent(c->copy_stream, c->res_ev[ev], 0));
+            waited = ev;
+        }
+        HIPCHK(hipMemcpyAsync(c->drain[i & 1], d + off, len, hipMemcpyDeviceToHost, c->copy_stream));
+        HIPCHK(hipEventRecord(c->drain_ev[i & 1], c->copy_stream));
+    };
+    enqueue(0);
+    int rc = 0;
+    for (uint64_t i = 0; i < chunks; i++) {
+        if (i + 1 < chunks) enqueue(i + 1);
+        HIPCHK(hipEventSynchronize(c->drain_ev[i & 1]));
+        if (!rc) rc = (*c->stream_sink)(c->drain[i & 1], (size_t)std::min(DRAIN_CHUNK, n - i * DRAIN_CHUNK));
     }
     return rc;
 }
@@ -1982,6 +2066,7 @@ int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink 
         return true;
     };
     int sink_rc = 0;
+    std::function<int(const uint8_t *, size_t)> sink_fn = [&](const uint8_t *d, size_t n) { return sink(d, n); };
     const std::vector<uint64_t> moff = sized_member_walk(in, in_len);
     if (moff.size() > 2) {
         std::vector<std::pair<size_t, size_t>> sp;
@@ -2024,20 +2109,33 @@ int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink 
             // the member may be followed by others: size the output from
             // the trailer at the end of the span, growing on overflow
             uint64_t cap = std::max<uint64_t>((uint64_t)M.exp_isize + 64 * KIB, 1 * MIB);
+            // the input's only member: its output streams to the sink while
+            // the decode finishes (resolve ranges, D2H on a copy stream)
+            const bool stream = attempt == 1 && members == 0 && spans.size() <= 1 && moff.size() <= 2;
             for (int tries = 0;; tries++) {
                 c->d_out.ensure(cap);
                 M.out = c->d_out.as<uint8_t>();
                 M.out_cap = c->d_out.cap;
                 M.status = GZGPU_OK;
                 std::vector<Member> v{M};
-                run_members(c, c->d_in.as<uint8_t>(), in_len, v);
+                c->streamed = false;
+                c->stream_rc = 0;
+                c->stream_sink = stream ? &sink_fn : nullptr;
+                try {
+                    run_members(c, c->d_in.as<uint8_t>(), in_len, v);
+                } catch (...) {
+                    c->stream_sink = nullptr;
+                    throw;
+                }
+                c->stream_sink = nullptr;
                 M = v[0];
-                if (M.status == GZGPU_OUTPUT_TOO_SMALL && tries < 8) {
+                if (M.status == GZGPU_OUTPUT_TOO_SMALL && tries < 8 && !c->streamed) {
                     cap = c->d_out.cap * 4;
                     continue;
                 }
                 break;
             }
+            if (c->streamed && c->stream_rc) return c->stream_rc;
             if (attempt == 0) {
                 // accepted only when it ends exactly at this span's trailer
                 done = M.status == GZGPU_OK && (M.end_bit + 7) / 8 == M.de / 8;
@@ -2069,8 +2167,11 @@ int decompress_host_stream(gzgpu_ctx *c, const uint8_t *in, size_t in_len, Sink 
             done = true;
         }
         if (!done) break;
-        int sr = drain_to_sink(c, M.out, M.out_len, sink);
-        if (sr) return sr;
+        if (!c->streamed) {
+            int sr = drain_to_sink(c, M.out, M.out_len, sink);
+            if (sr) return sr;
+        }
+        c->streamed = false;
         *total += M.out_len;
         off = (M.end_bit + 7) / 8 + 8;
         members++;
@@ -2135,7 +2236,9 @@ int gzgpu_ctx_create(int device, gzgpu_ctx **out) {
             throw StatusError{GZGPU_GPU_UNAVAILABLE, std::string("device is ") + prop.gcnArchName + ", need gfx950"};
         c->num_cus = prop.multiProcessorCount;
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+        for (auto &e : c->res_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         init_constants(c);
         c->scratch = device_scratch(device);
         HIPCHK(hipHostMalloc((void **)&c->noalloc_stage, NOALLOC_STAGE_BYTES, hipHostMallocDefault));
@@ -2160,6 +2263,10 @@ void gzgpu_ctx_destroy(gzgpu_ctx *c) {
     c->pin.release();
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->res_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     for (int i = 0; i < 2; i++) {
         if (c->drain_ev[i]) (void)hipEventDestroy(c->drain_ev[i]);
         if (c->drain[i]) (void)hipHostFree(c->drain[i]);

@@ -217,15 +217,16 @@ __device__ __forceinline__ uint32_t resolve16(uint32_t e, gptr<const uint8_t> wi
 __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__restrict__ rp,
                                                       const uint32_t *__restrict__ tile_first, uint32_t npieces,
                                                       const uint32_t *__restrict__ crc_tables,
-                                                      uint32_t *__restrict__ tile_crc) {
-    const uint32_t ntiles = tile_first[npieces];
+                                                      uint32_t *__restrict__ tile_crc, uint32_t t_begin,
+                                                      uint32_t t_end) {
+    const uint32_t ntiles = tile_first[npieces] < t_end ? tile_first[npieces] : t_end;
     __shared__ uint32_t t8[8][256];
     __shared__ uint32_t mx[4][256];
     __shared__ uint32_t red;
     for (uint32_t i = threadIdx.x; i < 8 * 256; i += blockDim.x) t8[i >> 8][i & 255] = crc_tables[i];
     for (uint32_t i = threadIdx.x; i < 4 * 256; i += blockDim.x) mx[i >> 8][i & 255] = crc_tables[8 * 256 + i];
     __syncthreads();
-    for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    for (uint32_t b = t_begin + blockIdx.x; b < ntiles; b += gridDim.x) {
         if (threadIdx.x == 0) red = 0;
         __syncthreads();
         uint32_t acc = 0;

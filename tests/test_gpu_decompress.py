@@ -489,3 +489,32 @@ def test_stored_prefix_with_huffman_tail(gpu):
     assert gpu.decompress(gz, stats=st) == raw
     say("stored prefix", len(gz), {k: st[k] for k in ("stored_members", "pieces", "redecoded")})
     assert st["stored_members"] == 0
+
+
+def test_streamed_output_and_late_crc_error(gpu):
+    """A single large member reaches the writer range by range while the rest
+    of it resolves; a CRC error is still reported, after output that equals
+    the true bytes' prefix (partial output before a terminal error,
+    single_member.rs:17-22)."""
+    raw, gz = W.text_member(40 << 20, seed=17)
+
+    class Rec:
+        def __init__(self):
+            self.parts = []
+
+        def write(self, b):
+            self.parts.append(bytes(b))
+            return len(b)
+
+    rec = Rec()
+    assert gpu.decompress_to_writer(gz, rec) == raw.size
+    assert b"".join(rec.parts) == raw.tobytes() and len(rec.parts) >= 5
+    bad = gz.copy()
+    bad[-6] ^= 0x01  # CRC
+    rec = Rec()
+    with pytest.raises(gpu.GzippyError) as e:
+        gpu.decompress_to_writer(bad, rec)
+    assert e.value.kind == "CrcMismatch"
+    got = b"".join(rec.parts)
+    assert raw.tobytes().startswith(got)
+    say("streamed before the CRC error", len(got), "of", raw.size)
