@@ -57,7 +57,15 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v) {
     return (uint64_t)hi << 32 | lo;
 }
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// The i1 form: the wave mask comes straight from the compare that computes
+// p (HIP's int-typed __ballot widens p to a VGPR and compares it again).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+// This lane's bit of the wave-uniform mask m (the inverse of ballot): used
+// as a select condition it is the SGPR mask itself, with no per-lane shift.
+__device__ __forceinline__ bool lane_in(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+// d < 0 as a signed 64-bit value, tested on the high word (an SALU compare;
+// LLVM lowers a 64-bit signed compare to the VALU).
+__device__ __forceinline__ bool neg64(uint64_t d) { return (int32_t)(uint32_t)(d >> 32) < 0; }
 // number of active lanes below this one in mask
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

@@ -353,28 +353,95 @@ __device__ __forceinline__ void put(const OutSeg &o, uint64_t i, uint32_t v) {
     if (is_wide<RT>(o)) ((gptr<uint16_t>)o.p)[i] = (uint16_t)v;
     else o.p[i] = (uint8_t)v;
 }
+// Entry soff + voff of the segment (soff wave-uniform, voff per lane) as a
+// 32-bit byte offset from the segment base: the store takes the base from
+// SGPRs and the offset from one VGPR, with no 64-bit address arithmetic.
+// Segment positions stay below 2^30 + a round, so offsets fit 32 bits.
+template <typename RT>
+__device__ __forceinline__ void put_e(const OutSeg &o, uint32_t soff, uint32_t voff, uint32_t v) {
+    if (is_wide<RT>(o)) *(gptr<uint16_t>)(o.p + (uint64_t)((soff + voff) * 2u)) = (uint16_t)v;
+    else o.p[(uint64_t)(soff + voff)] = (uint8_t)v;
+}
+// The ring slot at byte offset b (taken modulo the ring's size in bytes).
+// Callers pass a wave-uniform part plus a per-lane part, both in bytes, so
+// a slot address is one add and one AND.
+template <typename RT>
+__device__ __forceinline__ RT &ring_at(RT *ring, uint32_t b) {
+    return *reinterpret_cast<RT *>(reinterpret_cast<uint8_t *>(ring) + (b & (RING<RT> * (uint32_t)sizeof(RT) - 1u)));
+}
+// A use of v on every path (an empty asm statement taking it as an
+// operand).  Values loaded from HBM are otherwise used only inside
+// lane-masked branches, which the compiler's wait-count analysis must assume
+// can be skipped; a load it believes may still be in flight at the top of
+// the decode loop costs a full vmcnt wait (on this ISA, for every store
+// before it too) each time its register is reused.
+__device__ __forceinline__ void consumed(uint32_t v) { __asm__ __volatile__("" ::"v"(v)); }
+// Lanes of a chunk inside the copy: the first `rem` lanes (rem >= 1).
+__device__ __forceinline__ uint64_t first_lanes(uint32_t rem) { return rem >= WAVE ? ~0ull : (1ull << rem) - 1; }
+// Lanes whose entry is a marker (>= 256), as a wave mask straight from the
+// compare (no intermediate boolean).
+__device__ __forceinline__ uint64_t marker_lanes(uint32_t v) { return __builtin_amdgcn_uicmp(v, 255u, 34 /* unsigned > */); }
 
 // ---- LZ77 copy by the whole wave -----------------------------------------
 // Entry pos+i takes the value at pos - dist + (i mod dist): every source lies
 // before pos, so no entry of a copy depends on another entry of the same
 // copy.  All reads (at most five chunks of 64 for a 258-byte copy) are issued
 // before any write, with one wait in between; the only ring slot both read
-// and written is dist == 32768 for the same entry, read first.  LDS
+// and written is dist == ring size for the same entry, read first.  LDS
 // operations of one wave execute in order, which orders a copy after the
 // symbols before it.  All index math is 32-bit (segment positions stay
 // below 2^31).
+//
+// Chunk kb (entries kb .. kb + 63 of the copy, lane j holding entry kb + j)
+// goes to the ring and the output.
 template <typename RT>
-__device__ __forceinline__ void copy_chunk_out(RT *ring, const OutSeg &o, uint32_t gpos, uint32_t pos, uint32_t i,
-                                               uint32_t len, uint32_t v, uint64_t &last_marker_end) {
-    const bool act = i < len;
-    if (act) {
-        ring[(gpos + i) & RMASK<RT>] = (RT)v;
-        put<RT>(o, pos + i, v);
+__device__ __forceinline__ void copy_chunk_out(RT *ring, const OutSeg &o, uint32_t gpos, uint32_t pos, uint32_t kb,
+                                               uint32_t len, uint32_t v) {
+    const uint32_t lane = lane_id();
+    if (lane + kb < len) {
+        ring_at<RT>(ring, (gpos + kb) * (uint32_t)sizeof(RT) + lane * (uint32_t)sizeof(RT)) = (RT)v;
+        put_e<RT>(o, pos + kb, lane, v);
     }
-    if (is_wide<RT>(o)) {
-        const uint64_t m = ballot(act && v >= 256);
-        if (m) last_marker_end = o.seg0 + pos + (i - lane_id()) + 64 - (uint64_t)__clzll(m);
+    consumed(v);
+}
+
+// Marker bookkeeping of a copy while the output is u16.  last_marker_end is
+// a bound: every entry from it to the current position is a literal (the
+// flip waits until it is a window behind).  A copy whose source starts at or
+// after it copies literals only (an overlapping copy repeats its own source)
+// and needs no check; otherwise a marker among the copied entries moves the
+// bound to the copy's end (at most 257 entries past the exact last marker,
+// which only delays the flip by as much).  v holds the entries of chunk k.
+template <typename RT>
+__device__ __forceinline__ void copy_markers(const OutSeg &o, uint32_t pos, uint32_t len, uint32_t dist, uint32_t v0,
+                                             uint32_t v1, uint32_t v2, uint32_t v3, uint32_t v4,
+                                             uint64_t &last_marker_end) {
+    if (!is_wide<RT>(o) || (int64_t)(o.seg0 + pos) - (int64_t)dist >= (int64_t)last_marker_end) return;
+    // full chunks, then the last one without the lanes past the copy's end
+    const uint32_t kl = (len - 1) & ~(WAVE - 1);  // first entry of the last chunk
+    uint64_t m = 0, ml;
+    if (kl == 0) {
+        ml = marker_lanes(v0);
+    } else {
+        m = marker_lanes(v0);
+        if (kl == 64) {
+            ml = marker_lanes(v1);
+        } else {
+            m |= marker_lanes(v1);
+            if (kl == 128) {
+                ml = marker_lanes(v2);
+            } else {
+                m |= marker_lanes(v2);
+                if (kl == 192) {
+                    ml = marker_lanes(v3);
+                } else {
+                    m |= marker_lanes(v3);
+                    ml = marker_lanes(v4);
+                }
+            }
+        }
     }
+    if (m | (ml & first_lanes(len - kl))) last_marker_end = o.seg0 + pos + len;
 }
 
 // Where entries more than a ring back are read from (see InflateShared).
@@ -400,79 +467,114 @@ __device__ __forceinline__ uint32_t far_fetch(const FarSrc &f, int64_t r) {
     return 0u;
 }
 
+// A copy whose source is more than a ring back, in NC chunks: the piece's
+// output in HBM, written by this wavefront at least a ring earlier, or the
+// window before the piece (never overlapping the copy: dist > ring > len).
+// A wavefront's vector memory operations reach the CU's L1 in order and the
+// loads' wait covers its earlier stores.  Each chunk count has its own
+// straight-line body, so every load issued is consumed on the same path:
+// no load is left in flight past the copy, where the decode loop would have
+// to wait for it (and, on this ISA, for every store before it).
+template <typename RT, uint32_t NC>
+__device__ __forceinline__ void far_copy(RT *ring, const OutSeg &o, const FarSrc &f, uint32_t pos, uint32_t gpos,
+                                         uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
+    const uint32_t lane = lane_id();
+    const int64_t rel = (int64_t)o.seg0 + (int64_t)pos - (int64_t)dist;  // piece position of entry 0
+    uint32_t v[5] = {0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+    for (uint32_t k = 0; k < NC; k++) {
+        const uint32_t i = lane + WAVE * k;
+        v[k] = i < len ? far_fetch<RT>(f, rel + (int64_t)i) : 0u;
+    }
+    compiler_fence();
+#pragma unroll
+    for (uint32_t k = 0; k < NC; k++) copy_chunk_out<RT>(ring, o, gpos, pos, WAVE * k, len, v[k]);
+    copy_markers<RT>(o, pos, len, dist, v[0], v[1], v[2], v[3], v[4], last_marker_end);
+    compiler_fence();
+}
+
 template <typename RT, typename SH>
 __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o, uint32_t pos, uint32_t g0,
                                                          uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
     RT *ring = sh.ring;
     const uint32_t lane = lane_id();
     const uint32_t gpos = g0 + pos;
+    constexpr uint32_t SZ = (uint32_t)sizeof(RT);
     if (dist > RING<RT>) {
-        // Source more than a ring back (never overlapping the copy: dist >
-        // 8192 > len): the piece's output in HBM, written by this wavefront
-        // at least a ring earlier, or the window before the piece.  A
-        // wavefront's vector memory operations reach the CU's L1 in order
-        // and the loads' wait covers its earlier stores.
         const FarSrc f = far_src(sh);
-        const int64_t rel = (int64_t)o.seg0 + (int64_t)pos - (int64_t)dist;  // piece position of entry 0
-        uint32_t v[5];
-#pragma unroll
-        for (uint32_t k = 0; k < 5; k++) {
-            const uint32_t i = lane + WAVE * k;
-            v[k] = i < len ? far_fetch<RT>(f, rel + (int64_t)i) : 0u;
-        }
-        compiler_fence();
-#pragma unroll
-        for (uint32_t k = 0; k < 5; k++)
-            if (len > WAVE * k) copy_chunk_out<RT>(ring, o, gpos, pos, lane + WAVE * k, len, v[k], last_marker_end);
-        compiler_fence();
+        const uint32_t nc = (len + WAVE - 1) / WAVE;
+        if (nc <= 1) far_copy<RT, 1>(ring, o, f, pos, gpos, len, dist, last_marker_end);
+        else if (nc == 2) far_copy<RT, 2>(ring, o, f, pos, gpos, len, dist, last_marker_end);
+        else if (nc == 3) far_copy<RT, 3>(ring, o, f, pos, gpos, len, dist, last_marker_end);
+        else if (nc == 4) far_copy<RT, 4>(ring, o, f, pos, gpos, len, dist, last_marker_end);
+        else far_copy<RT, 5>(ring, o, f, pos, gpos, len, dist, last_marker_end);
         return;
     }
-    const uint32_t src0 = gpos - dist;  // ring position of the source's first entry
-    // source offsets of the (up to) five chunks: i, or i mod dist when the
-    // copy overlaps itself (quotient from the hardware reciprocal, corrected
-    // by one step either way; i < 320 and dist <= 258)
-    uint32_t s0 = lane, s1 = lane + 64, s2 = lane + 128, s3 = lane + 192, s4 = lane + 256;
+    const uint32_t sb = (gpos - dist) * SZ;  // ring byte of the source's first entry (before wrapping)
+    // every read before any write (ring offsets wrap, so lanes past the
+    // copy's end read harmlessly)
+    uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
     if (dist < len) {
-        const float rd = __builtin_amdgcn_rcpf((float)dist);
-        auto mod = [&](uint32_t i) {
-            const uint32_t q = (uint32_t)((float)i * rd);
-            int32_t r = (int32_t)(i - q * dist);
-            r += r < 0 ? (int32_t)dist : 0;
-            r -= r >= (int32_t)dist ? (int32_t)dist : 0;
-            return (uint32_t)r;
+        // The copy overlaps itself: entry i reads source entry i mod dist.
+        // Lane j's remainder comes from the hardware reciprocal (exact except
+        // at multiples of dist, where the quotient can come out one short and
+        // the remainder equal to dist, folded to 0 by the unsigned min);
+        // each next chunk adds 64 mod dist (lane 63's remainder + 1) and
+        // folds once, since both terms are below dist.
+        const float fd = (float)dist;
+        const float q = __builtin_floorf((float)lane * __builtin_amdgcn_rcpf(fd));
+        const uint32_t r = (uint32_t)__builtin_fmaf(-q, fd, (float)lane);
+        uint32_t s = min(r, r - dist);
+        const uint32_t r63 = uni((uint32_t)__builtin_amdgcn_readlane((int)s, 63));
+        const uint32_t step = r63 + 1 == dist ? 0u : r63 + 1;
+        auto adv = [&](uint32_t x) {
+            const uint32_t t = x + step;
+            return min(t, t - dist);
         };
-        s0 = mod(s0);
-        s1 = mod(s1);
-        s2 = mod(s2);
-        s3 = mod(s3);
-        s4 = mod(s4);
-    }
-    // every read before any write (ring indices are masked, so lanes past
-    // the copy's end read harmlessly)
-    uint32_t v0, v1, v2, v3, v4;
-    v0 = ring[(src0 + s0) & RMASK<RT>];
-    if (len > 64) {
-        v1 = ring[(src0 + s1) & RMASK<RT>];
-        if (len > 128) {
-            v2 = ring[(src0 + s2) & RMASK<RT>];
-            if (len > 192) {
-                v3 = ring[(src0 + s3) & RMASK<RT>];
-                if (len > 256) v4 = ring[(src0 + s4) & RMASK<RT>];
+        v0 = ring_at<RT>(ring, sb + s * SZ);
+        if (len > 64) {
+            s = adv(s);
+            v1 = ring_at<RT>(ring, sb + s * SZ);
+            if (len > 128) {
+                s = adv(s);
+                v2 = ring_at<RT>(ring, sb + s * SZ);
+                if (len > 192) {
+                    s = adv(s);
+                    v3 = ring_at<RT>(ring, sb + s * SZ);
+                    if (len > 256) {
+                        s = adv(s);
+                        v4 = ring_at<RT>(ring, sb + s * SZ);
+                    }
+                }
+            }
+        }
+    } else {
+        const uint32_t lb = lane * SZ;
+        v0 = ring_at<RT>(ring, sb + lb);
+        if (len > 64) {
+            v1 = ring_at<RT>(ring, sb + 64 * SZ + lb);
+            if (len > 128) {
+                v2 = ring_at<RT>(ring, sb + 128 * SZ + lb);
+                if (len > 192) {
+                    v3 = ring_at<RT>(ring, sb + 192 * SZ + lb);
+                    if (len > 256) v4 = ring_at<RT>(ring, sb + 256 * SZ + lb);
+                }
             }
         }
     }
     compiler_fence();
-    copy_chunk_out<RT>(ring, o, gpos, pos, lane, len, v0, last_marker_end);
+    copy_chunk_out<RT>(ring, o, gpos, pos, 0, len, v0);
     if (len > 64) {
-        copy_chunk_out<RT>(ring, o, gpos, pos, lane + 64, len, v1, last_marker_end);
+        copy_chunk_out<RT>(ring, o, gpos, pos, 64, len, v1);
         if (len > 128) {
-            copy_chunk_out<RT>(ring, o, gpos, pos, lane + 128, len, v2, last_marker_end);
+            copy_chunk_out<RT>(ring, o, gpos, pos, 128, len, v2);
             if (len > 192) {
-                copy_chunk_out<RT>(ring, o, gpos, pos, lane + 192, len, v3, last_marker_end);
-                if (len > 256) copy_chunk_out<RT>(ring, o, gpos, pos, lane + 256, len, v4, last_marker_end);
+                copy_chunk_out<RT>(ring, o, gpos, pos, 192, len, v3);
+                if (len > 256) copy_chunk_out<RT>(ring, o, gpos, pos, 256, len, v4);
             }
         }
     }
+    copy_markers<RT>(o, pos, len, dist, v0, v1, v2, v3, v4, last_marker_end);
     compiler_fence();
 }
 
@@ -518,7 +620,7 @@ __device__ __attribute__((always_inline)) void fused_round(SH &sh, const OutSeg 
             res = false;
             ptr = (uint32_t)src;
         } else if ((uint32_t)(-src) <= RING<RT>) {
-            val = sh.ring[(g0 + pos + (uint32_t)src) & RMASK<RT>];
+            val = ring_at<RT>(sh.ring, (g0 + pos) * (uint32_t)sizeof(RT) + (uint32_t)src * (uint32_t)sizeof(RT));
         } else {
             far = live;
         }
@@ -543,13 +645,11 @@ __device__ __attribute__((always_inline)) void fused_round(SH &sh, const OutSeg 
     }
     compiler_fence();
     if (live) {
-        sh.ring[(g0 + pos + lane) & RMASK<RT>] = (RT)val;
-        put<RT>(o, pos + lane, val);
+        ring_at<RT>(sh.ring, (g0 + pos) * (uint32_t)sizeof(RT) + lane * (uint32_t)sizeof(RT)) = (RT)val;
+        put_e<RT>(o, pos, lane, val);
     }
-    if (is_wide<RT>(o)) {
-        const uint64_t m = ballot(live && val >= 256);
-        if (m) last_marker_end = o.seg0 + pos + 64 - (uint64_t)__clzll(m);
-    }
+    if (is_wide<RT>(o) && (marker_lanes(val) & first_lanes(run))) last_marker_end = o.seg0 + pos + run;
+    consumed(val);
     compiler_fence();
 }
 
@@ -624,8 +724,8 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
     if (kind == K_LIT) {
         if (pos >= cap) return R_OVERFLOW;
         if (lane == 0) {
-            sh.ring[(g0 + pos) & RMASK<RT>] = (RT)(e >> 16);
-            put<RT>(o, pos, e >> 16);
+            ring_at<RT>(sh.ring, (g0 + pos) * (uint32_t)sizeof(RT)) = (RT)(e >> 16);
+            put_e<RT>(o, pos, 0, e >> 16);
         }
         compiler_fence();
         pos++;
@@ -793,7 +893,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
     // through memory, and a "divergent" loop control turns the whole decode
     // loop into exec-masked code with VGPR copies of scalar state
     o.p = (gptr<uint8_t>)uni64((uint64_t)o.p);
-    o.pos = uni64(o.pos);
+        o.pos = uni64(o.pos);
     o.cap = uni64(o.cap);
     o.seg0 = uni64(o.seg0);
     o.wide = uni(o.wide);
@@ -805,6 +905,9 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
     const uint32_t hist0 = uni((uint32_t)(o.seg0 < WIN ? o.seg0 : WIN));  // history before the segment, capped
     uint32_t r = R_EOB;
     P = uni64(P);
+    // nothing loaded before (stored blocks, the header reader) is still in
+    // flight when the loop starts; see consumed()
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unchanged
     Lookahead la;
     lookahead(sh, in, n, P, la);
     for (;;) {
@@ -812,14 +915,16 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
         pos = uni(pos);
         last_marker_end = uni64(last_marker_end);
         stop_at = uni64(stop_at);
-        if (P >= stop_at) {
+        // 64-bit orderings as the sign of a difference (bit positions are
+        // below 2^63): SALU subtractions instead of VALU 64-bit compares
+        if (stop_at != NO_SYNC && !neg64(P - stop_at)) {
             if (P == stop_at) {
                 r = R_STOPMID;
                 break;
             }
             stop_at = NO_SYNC;  // the decode went past it (single steps)
         }
-        if (P > limit_bit) {
+        if (neg64(limit_bit - P)) {
             err = E_TRUNC;
             r = R_ERROR;
             break;
@@ -846,7 +951,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
         // ---- a mid-block stop inside this round: keep the symbols before it
         // when one of them ends exactly there, else the stream does not pass
         // through it and it is dropped
-        if (stop_at - P < (uint64_t)s) {
+        if (stop_at != NO_SYNC && stop_at - P < (uint64_t)s) {
             const uint32_t rel = uni((uint32_t)(stop_at - P));
             if (rel < 64 && ((M >> rel) & 1)) {
                 M = uni64(M & ((1ull << rel) - 1));
@@ -857,7 +962,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             }
         }
         // ---- output offsets of the taken symbols
-        const uint32_t v = ((M >> lane) & 1) ? la.olen : 0u;
+        const uint32_t v = lane_in(M) ? la.olen : 0u;
         const uint32_t incl = wave_scan_incl(v);
         const uint32_t opos = incl - v;
         uint32_t run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, 63));
@@ -886,14 +991,23 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             //    it.
             // None of these can cut before the round's first symbol (lengths
             // are at most 258, the ring holds 4096 or more entries).
-            const bool inM = (M >> lane) & 1;
-            const bool is_copy = ((M & ~LITM & ~EOBM) >> lane) & 1;
-            const bool far = la.dist > RING<RT>;
-            const bool bad = is_copy && !far && la.dist + run - opos > RING<RT>;
-            const bool badf = lane != 0 && is_copy && far && la.dist < opos + la.olen;
-            const bool over = lane != 0 && inM && incl > RING<RT>;
-            if (ballot(bad || badf || over)) {  // rare: one ballot in the common case
-                const uint64_t BAD = ballot(bad), BADF = ballot(badf), OVER = ballot(over);
+            // A round no longer than the ring can only meet the first: the
+            // second needs a symbol ending past the ring, the third a far
+            // copy (distance > ring) whose source lies inside the round.
+            const uint64_t COPY = M & ~LITM & ~EOBM;
+            uint64_t BAD, BADF = 0, OVER = 0;
+            if (run <= RING<RT>) {
+                BAD = COPY & __builtin_amdgcn_uicmp(la.dist, RING<RT>, 37 /* unsigned <= */) &
+                      __builtin_amdgcn_uicmp(la.dist + run - opos, RING<RT>, 34 /* unsigned > */);
+            } else {
+                const bool inM = lane_in(M);
+                const bool is_copy = lane_in(COPY);
+                const bool far = la.dist > RING<RT>;
+                BAD = ballot(is_copy && !far && la.dist + run - opos > RING<RT>);
+                BADF = ballot(lane != 0 && is_copy && far && la.dist < opos + la.olen);
+                OVER = ballot(lane != 0 && inM && incl > RING<RT>);
+            }
+            if (BAD | BADF | OVER) {  // rare
                 uint32_t cut = 64;  // keep the symbols at lanes below
                 if (BAD) cut = uni((uint32_t)__builtin_ctzll(BAD)) + 1;
                 if (BADF) {
@@ -919,14 +1033,14 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             P += s;
             if (!eob) lookahead(sh, in, n, P, la);
             if (run <= WAVE && matches) {
-                fused_round<RT>(sh, o, pos, g0, run, (M >> lane) & 1, (lits >> lane) & 1, opos, c_e, c_dist, c_olen,
+                fused_round<RT>(sh, o, pos, g0, run, lane_in(M), lane_in(lits), opos, c_e, c_dist, c_olen,
                                 last_marker_end);
             } else {
                 // ---- literals of the round: one LDS write and one HBM store
-                if ((lits >> lane) & 1) {
-                    const uint32_t q = pos + opos;
-                    sh.ring[(g0 + q) & RMASK<RT>] = (RT)(c_e >> 16);
-                    put<RT>(o, q, c_e >> 16);
+                if (lane_in(lits)) {
+                    ring_at<RT>(sh.ring, (g0 + pos) * (uint32_t)sizeof(RT) + opos * (uint32_t)sizeof(RT)) =
+                        (RT)(c_e >> 16);
+                    put_e<RT>(o, pos, opos, c_e >> 16);
                 }
                 compiler_fence();
                 // ---- copies, in stream order
@@ -951,7 +1065,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
         if (pos >= 0x40000000u) {
             // rebase the segment so positions stay 32-bit
             o.p = o.p + (is_wide<RT>(o) ? 2ull : 1ull) * pos;
-            o.seg0 += pos;
+                        o.seg0 += pos;
             o.cap -= pos;
             g0 += pos;
             cap = (uint32_t)(o.cap < 0x7FFFFFFFull ? o.cap : 0x7FFFFFFFull);
