@@ -411,37 +411,23 @@ __device__ __forceinline__ void copy_chunk_out(RT *ring, const OutSeg &o, uint32
 // after it copies literals only (an overlapping copy repeats its own source)
 // and needs no check; otherwise a marker among the copied entries moves the
 // bound to the copy's end (at most 257 entries past the exact last marker,
-// which only delays the flip by as much).  v holds the entries of chunk k.
+// which only delays the flip by as much).
+struct MarkerTrack {
+    bool on;     // the copy may copy markers
+    uint64_t m;  // lanes of the chunks so far that copied one
+};
 template <typename RT>
-__device__ __forceinline__ void copy_markers(const OutSeg &o, uint32_t pos, uint32_t len, uint32_t dist, uint32_t v0,
-                                             uint32_t v1, uint32_t v2, uint32_t v3, uint32_t v4,
-                                             uint64_t &last_marker_end) {
-    if (!is_wide<RT>(o) || (int64_t)(o.seg0 + pos) - (int64_t)dist >= (int64_t)last_marker_end) return;
-    // full chunks, then the last one without the lanes past the copy's end
-    const uint32_t kl = (len - 1) & ~(WAVE - 1);  // first entry of the last chunk
-    uint64_t m = 0, ml;
-    if (kl == 0) {
-        ml = marker_lanes(v0);
-    } else {
-        m = marker_lanes(v0);
-        if (kl == 64) {
-            ml = marker_lanes(v1);
-        } else {
-            m |= marker_lanes(v1);
-            if (kl == 128) {
-                ml = marker_lanes(v2);
-            } else {
-                m |= marker_lanes(v2);
-                if (kl == 192) {
-                    ml = marker_lanes(v3);
-                } else {
-                    m |= marker_lanes(v3);
-                    ml = marker_lanes(v4);
-                }
-            }
-        }
-    }
-    if (m | (ml & first_lanes(len - kl))) last_marker_end = o.seg0 + pos + len;
+__device__ __forceinline__ MarkerTrack marker_track(const OutSeg &o, uint32_t pos, uint32_t dist,
+                                                    uint64_t last_marker_end) {
+    return {is_wide<RT>(o) && (int64_t)(o.seg0 + pos) - (int64_t)dist < (int64_t)last_marker_end, 0ull};
+}
+// chunk kb of the copy holds v (lanes past the copy's end do not count)
+__device__ __forceinline__ void marker_chunk(MarkerTrack &t, uint32_t kb, uint32_t len, uint32_t v) {
+    if (t.on) t.m |= marker_lanes(v) & first_lanes(len - kb);
+}
+__device__ __forceinline__ void marker_done(const MarkerTrack &t, const OutSeg &o, uint32_t pos, uint32_t len,
+                                            uint64_t &last_marker_end) {
+    if (t.m) last_marker_end = o.seg0 + pos + len;
 }
 
 // Where entries more than a ring back are read from (see InflateShared).
@@ -480,16 +466,20 @@ __device__ __forceinline__ void far_copy(RT *ring, const OutSeg &o, const FarSrc
                                          uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
     const uint32_t lane = lane_id();
     const int64_t rel = (int64_t)o.seg0 + (int64_t)pos - (int64_t)dist;  // piece position of entry 0
-    uint32_t v[5] = {0u, 0u, 0u, 0u, 0u};
+    uint32_t v[NC];
 #pragma unroll
     for (uint32_t k = 0; k < NC; k++) {
         const uint32_t i = lane + WAVE * k;
         v[k] = i < len ? far_fetch<RT>(f, rel + (int64_t)i) : 0u;
     }
     compiler_fence();
+    MarkerTrack t = marker_track<RT>(o, pos, dist, last_marker_end);
 #pragma unroll
-    for (uint32_t k = 0; k < NC; k++) copy_chunk_out<RT>(ring, o, gpos, pos, WAVE * k, len, v[k]);
-    copy_markers<RT>(o, pos, len, dist, v[0], v[1], v[2], v[3], v[4], last_marker_end);
+    for (uint32_t k = 0; k < NC; k++) {
+        copy_chunk_out<RT>(ring, o, gpos, pos, WAVE * k, len, v[k]);
+        marker_chunk(t, WAVE * k, len, v[k]);
+    }
+    marker_done(t, o, pos, len, last_marker_end);
     compiler_fence();
 }
 
@@ -512,8 +502,9 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
     }
     const uint32_t sb = (gpos - dist) * SZ;  // ring byte of the source's first entry (before wrapping)
     // every read before any write (ring offsets wrap, so lanes past the
-    // copy's end read harmlessly)
-    uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
+    // copy's end read harmlessly); chunk k's value is read only when the copy
+    // reaches it
+    uint32_t v0, v1, v2, v3, v4;
     if (dist < len) {
         // The copy overlaps itself: entry i reads source entry i mod dist.
         // Lane j's remainder comes from the hardware reciprocal (exact except
@@ -563,18 +554,26 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
         }
     }
     compiler_fence();
+    MarkerTrack t = marker_track<RT>(o, pos, dist, last_marker_end);
     copy_chunk_out<RT>(ring, o, gpos, pos, 0, len, v0);
+    marker_chunk(t, 0, len, v0);
     if (len > 64) {
         copy_chunk_out<RT>(ring, o, gpos, pos, 64, len, v1);
+        marker_chunk(t, 64, len, v1);
         if (len > 128) {
             copy_chunk_out<RT>(ring, o, gpos, pos, 128, len, v2);
+            marker_chunk(t, 128, len, v2);
             if (len > 192) {
                 copy_chunk_out<RT>(ring, o, gpos, pos, 192, len, v3);
-                if (len > 256) copy_chunk_out<RT>(ring, o, gpos, pos, 256, len, v4);
+                marker_chunk(t, 192, len, v3);
+                if (len > 256) {
+                    copy_chunk_out<RT>(ring, o, gpos, pos, 256, len, v4);
+                    marker_chunk(t, 256, len, v4);
+                }
             }
         }
     }
-    copy_markers<RT>(o, pos, len, dist, v0, v1, v2, v3, v4, last_marker_end);
+    marker_done(t, o, pos, len, last_marker_end);
     compiler_fence();
 }
 
@@ -785,34 +784,35 @@ __device__ __attribute__((always_inline)) void lookahead(SH &sh, const uint8_t *
     uint64_t W0, W1;
     window128(in, n, P, W0, W1);
     const uint64_t ch = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
+    // Straight-line per lane: the distance code is looked up on every lane
+    // (its result kept only where the code is a length), and the lane
+    // classes are wave masks from the compares themselves.
+    constexpr int EQ = 32, NE = 33, UGT = 34;
     uint32_t e = sh.ll[(uint32_t)ch & ((1u << SH::LLR) - 1)];
     // codes longer than the root: one more LDS round trip, only when a lane needs it
-    if (ballot((e & 15) == 0 && (e >> 16) != 0)) e = sub_lookup(sh.ll, e, ch >> SH::LLR);
+    const uint64_t SUB = __builtin_amdgcn_uicmp(e & 15, 0u, EQ) & __builtin_amdgcn_uicmp(e, 0xFFFFu, UGT);
+    if (SUB && lane_in(SUB)) e = sh.ll[(e >> 16) + ((uint32_t)(ch >> SH::LLR) & ((1u << ((e >> 4) & 15)) - 1))];
     const uint32_t L = e & 15, kind = (e >> 8) & 3, E = (e >> 4) & 15;
-    bool ok = (L != 0) & (kind != K_BAD);
-    uint32_t tot = L, olen = kind == K_LIT ? 1u : 0u, dist = 0, dcode = 0, doff = 0;
-    if (kind == K_LEN) {
-        const uint32_t off = L + E;
-        olen = (e >> 16) + ((uint32_t)(ch >> L) & ((1u << E) - 1));
-        dcode = sh.dl[(uint32_t)(ch >> off) & ((1u << SH::DR) - 1)];
-        doff = off;
-    }
-    if (ballot(kind == K_LEN && (dcode & 15) == 0 && (dcode >> 16) != 0))
-        if (kind == K_LEN) dcode = sub_lookup(sh.dl, dcode, ch >> (doff + SH::DR));
-    if (kind == K_LEN) {
-        const uint32_t d = dcode;
-        const uint32_t DL = d & 15, DE = (d >> 4) & 15;
-        ok &= (DL != 0) & (((d >> 8) & 3) == K_LEN);
-        dist = (d >> 16) + ((uint32_t)(ch >> (doff + DL)) & ((1u << DE) - 1));
-        tot = doff + DL + DE;
-    }
+    const uint32_t off = L + E;
+    const uint32_t len = (e >> 16) + ((uint32_t)(ch >> L) & ((1u << E) - 1));
+    uint32_t d = sh.dl[(uint32_t)(ch >> off) & ((1u << SH::DR) - 1)];
+    const uint64_t LENK = __builtin_amdgcn_uicmp(kind, K_LEN, EQ);
+    const uint64_t DSUB = LENK & __builtin_amdgcn_uicmp(d & 15, 0u, EQ) & __builtin_amdgcn_uicmp(d, 0xFFFFu, UGT);
+    if (DSUB && lane_in(DSUB))
+        d = sh.dl[(d >> 16) + ((uint32_t)(ch >> (off + SH::DR)) & ((1u << ((d >> 4) & 15)) - 1))];
+    const uint32_t DL = d & 15, DE = (d >> 4) & 15;
+    const uint32_t dist = (d >> 16) + ((uint32_t)(ch >> (off + DL)) & ((1u << DE) - 1));
+    const uint64_t NZ = __builtin_amdgcn_uicmp(L, 0u, NE);
+    const uint64_t LITK = __builtin_amdgcn_uicmp(kind, K_LIT, EQ), EOBK = __builtin_amdgcn_uicmp(kind, K_EOB, EQ);
+    const uint64_t DOK = __builtin_amdgcn_uicmp(DL, 0u, NE) & __builtin_amdgcn_uicmp((d >> 8) & 3, K_LEN, EQ);
+    const bool is_len = lane_in(LENK);
     la.e = e;
-    la.olen = olen;
-    la.dist = dist;
-    la.nxt = lane + tot;  // <= 63 + 48
-    la.CONT = ballot(ok && kind != K_EOB);
-    la.EOBM = ballot(ok && kind == K_EOB);
-    la.LITM = ballot(ok && kind == K_LIT);
+    la.olen = is_len ? len : lane_in(LITK) ? 1u : 0u;
+    la.dist = is_len ? dist : 0u;
+    la.nxt = lane + (is_len ? off + DL + DE : L);  // <= 63 + 48
+    la.CONT = NZ & (LITK | (LENK & DOK));
+    la.EOBM = NZ & EOBK;
+    la.LITM = NZ & LITK;
     const uint32_t nx = la.nxt;
     const bool go_on = nx < 64 && ((la.CONT >> (nx & 63)) & 1);
     la.step = nx | (go_on ? 0u : 0x100u);
@@ -913,6 +913,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
     for (;;) {
         P = uni64(P);
         pos = uni(pos);
+        o.p = (gptr<uint8_t>)uni64((uint64_t)o.p);  // stores then take the base from SGPRs
         last_marker_end = uni64(last_marker_end);
         stop_at = uni64(stop_at);
         // 64-bit orderings as the sign of a difference (bit positions are
