@@ -455,14 +455,18 @@ std::vector<uint32_t> radix_order(const std::vector<uint64_t> &keys) {
 // ---- block finder over ranges [start_bit, end_bit) of d_in ----------------
 // One stage-A launch over every range; hits of all ranges, sorted, duplicates
 // (ranges that overlap) removed.  Every hit lies inside the range it came from.
+// final_ok[i] (when given): range i also reports BFINAL = 1 dynamic headers
+// (planning only; such hits carry bit HIT_FINAL_SHIFT of their info)
 std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len,
-                                      const std::vector<std::pair<uint64_t, uint64_t>> &rng, bool estimate = false) {
+                                      const std::vector<std::pair<uint64_t, uint64_t>> &rng, bool estimate = false,
+                                      const std::vector<uint8_t> *final_ok = nullptr) {
     std::vector<gzgpu_boundary> out;
     std::vector<BfRange> table;
     uint64_t nchunk = 0, nbits = 0;
-    for (auto &r : rng) {
+    for (size_t i = 0; i < rng.size(); i++) {
+        const auto &r = rng[i];
         if (r.second <= r.first) continue;
-        table.push_back({r.first, r.second, nchunk});
+        table.push_back({r.first, r.second, nchunk, final_ok && (*final_ok)[i] ? 1u : 0u, 0u});
         nchunk += (r.second - (r.first & ~31ull) + 31) / 32;
         nbits += r.second - r.first;
     }
@@ -570,6 +574,13 @@ constexpr double PIECES_PER_SLOT = 1.0, SLOT_FRACTION = 31.0 / 32.0;
 // hit (a dynamic header is rarely longer; a start inside it fails its sync
 // and costs a re-decode) and MID_TAIL bits before the next hit
 constexpr uint64_t MID_HEAD = 2048, MID_TAIL = 2048;
+// The last FINAL_SCAN_BITS of every member to be split are scanned for
+// BFINAL = 1 dynamic headers too: the final block then takes starts inside
+// it like any other (the reference finder, and the public find_blocks, report
+// BFINAL = 0 only, so without this the piece reaching the final block decodes
+// all of it).  512 KiB covers the final block of the encoders measured; a
+// longer one keeps the old behaviour (its piece runs to the end).
+constexpr uint64_t FINAL_SCAN_BITS = 8ull * 512 * 1024;
 // planner: piece-size slope along a member (see the grid in run_members_aligned)
 constexpr double TILT = 0.25;
 // speculation: from this launch on a miss re-decodes the member's whole
@@ -942,14 +953,19 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     // cut into many pieces: only there can content change along it enough to
     // matter.  Members cut into fewer are planned by their average.
     std::vector<std::pair<uint64_t, uint64_t>> ranges;
+    std::vector<uint8_t> final_ok;
     bool estimate = false;
     for (auto &M : mem)
         if (splits(M)) {
-            ranges.push_back({M.ds, M.de});
+            const uint64_t tail0 = M.de - std::min<uint64_t>(M.de - M.ds, FINAL_SCAN_BITS);
+            ranges.push_back({M.ds, tail0});
+            final_ok.push_back(0);
+            ranges.push_back({tail0, M.de});
+            final_ok.push_back(1);
             estimate = estimate || (M.de - M.ds) / spacing_of(M) >= ESTIMATE_MIN_PIECES;
         }
     const double th0 = debug_enabled() ? host_ms() : 0;
-    const std::vector<gzgpu_boundary> all_hits = find_hits(c, d_in, in_len, ranges, estimate);
+    const std::vector<gzgpu_boundary> all_hits = find_hits(c, d_in, in_len, ranges, estimate, &final_ok);
     const double th1 = debug_enabled() ? host_ms() : 0;
     c->st.candidates += (uint32_t)all_hits.size();
     auto hit_lt = [](const gzgpu_boundary &a, uint64_t bit) { return a.bit_offset < bit; };
@@ -1039,6 +1055,9 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 if (k->kind == 2) shadow[(size_t)(k - hits.begin())] = 1;
         }
         auto dyn_hit = [&](size_t j) { return hits[j].kind == 2 && !shadow[j]; };
+        // a BFINAL = 1 header: a block to start inside of, never a piece start
+        // (the stop rule stops only at non-final headers)
+        auto final_hit = [&](size_t j) { return hits[j].kind == 2 && ((hits[j].info >> HIT_FINAL_SHIFT) & 1u); };
         // Decode cost and output along the member, per stretch between
         // consecutive real hits (dynamic headers, chained stored blocks):
         // a stored block's output is its LEN; a dynamic block's is estimated
@@ -1157,7 +1176,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             uint64_t bestd = ~0ull;
             for (int pass = 0; pass < 2 && best == hits.size(); pass++)
                 for (size_t j = h; j < hits.size() && hits[j].bit_offset < hi; j++) {
-                    const bool ok = pass == 0 ? dyn_hit(j) : hits[j].kind == 0 && chained(j);
+                    const bool ok = pass == 0 ? dyn_hit(j) && !final_hit(j) : hits[j].kind == 0 && chained(j);
                     if (!ok) continue;
                     const uint64_t d = hits[j].bit_offset > g ? hits[j].bit_offset - g : g - hits[j].bit_offset;
                     if (d < bestd) {
@@ -1181,6 +1200,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                 const bool dyn = prev == 0 || hits[prev - 1].kind == 2;
                 const uint64_t next = k < hits.size() ? hits[k].bit_offset : M.de;
                 if (dyn && g >= hdr + MID_HEAD && g >= last + MID_HEAD && next >= g + MID_TAIL) {
+                    if (prev && final_hit(prev - 1)) c->st.final_block_pieces++;
                     chosen.push_back({g, g, g, hdr, true});
                     last = g;
                     continue;

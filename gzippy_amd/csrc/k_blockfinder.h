@@ -124,8 +124,12 @@ __device__ __forceinline__ bool stored_hit(const uint8_t *__restrict__ in, uint6
 // One scan range of stage A: candidate offsets in [start_bit, end_bit);
 // chunk0 is the global index of its first 32-offset chunk (ranges are laid
 // end to end in chunk space, so one launch covers every member of a batch).
+// final_ok: dynamic headers with BFINAL = 1 are reported too (the planner's
+// scan of each member's last stretch, where the final block starts; the
+// reference predicate, and every other scan, takes BFINAL = 0 only).
 struct BfRange {
     uint64_t start_bit, end_bit, chunk0;
+    uint32_t final_ok, pad;
 };
 
 // Stage A.  Each lane owns 32 consecutive bit offsets [o0, o0 + 32) of one
@@ -174,6 +178,7 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
             else hi_r = mid;
         }
         const uint64_t start_bit = ranges[lo_r].start_bit, end_bit = ranges[lo_r].end_bit;
+        const bool final_ok = ranges[lo_r].final_ok != 0;
         const uint64_t o0 = (start_bit & ~31ull) + 32 * (c - ranges[lo_r].chunk0);
         const uint64_t sb = ((start_bit + 3 + 7) / 8) > 1 ? (start_bit + 3 + 7) / 8 : 1;
         uint64_t until = (end_bit + 7) / 8;
@@ -198,10 +203,11 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
             }
         }
         // ---- header test on all 32 offsets: bit j of the mask is the test
-        // at o0 + j.  BFINAL = 0 and BTYPE = 10 (bits 0, 1, 2 = 0, 0, 1);
+        // at o0 + j.  BFINAL = 0 (either, in a final_ok range) and BTYPE = 10
+        // (bits 0, 1, 2 = 0, 0, 1);
         // HLIT <= 29 and HDIST <= 29 (not all of bits 4-7, not all of 9-12)
         const uint64_t X = (uint64_t)w0 | (uint64_t)w1 << 32;
-        uint32_t hm = (uint32_t)(~X & ~(X >> 1) & (X >> 2) & ~((X >> 4) & (X >> 5) & (X >> 6) & (X >> 7)) &
+        uint32_t hm = (uint32_t)((final_ok ? ~0ull : ~X) & ~(X >> 1) & (X >> 2) & ~((X >> 4) & (X >> 5) & (X >> 6) & (X >> 7)) &
                                  ~((X >> 9) & (X >> 10) & (X >> 11) & (X >> 12)));
         // offsets inside [start_bit, dyn_end)
         {
@@ -269,12 +275,15 @@ __global__ void __launch_bounds__(256) k_bf_stage_a(const uint8_t *__restrict__ 
 }
 
 // Full dynamic-header validation of one survivor (one lane).
+// info of a dynamic hit: HLIT bits 0-4, HDIST 8-12, HCLEN 16-19, BFINAL bit 7
+// (set only by final_ok scans), the planner's estimate in bits 20-31
+constexpr uint32_t HIT_FINAL_SHIFT = 7;
 __device__ bool validate_dynamic(const uint8_t *__restrict__ in, uint64_t n, uint64_t o, uint8_t *lut,
                                  uint32_t &info) {
     const uint64_t total = n * 8;
     uint64_t h = peek64(in, n, o);
     uint32_t hlit = (uint32_t)(h >> 3) & 31, hdist = (uint32_t)(h >> 8) & 31, hclen = (uint32_t)(h >> 13) & 15;
-    info = hlit | hdist << 8 | hclen << 16;
+    info = hlit | hdist << 8 | hclen << 16 | ((uint32_t)h & 1u) << HIT_FINAL_SHIFT;
     uint64_t p = peek64(in, n, o + 17);
     uint32_t pc = hclen + 4;
     // canonical precode: counts, then codes assigned by symbol order

@@ -89,6 +89,8 @@ typedef struct {
     uint32_t midblock_linked;   /* of those, linked to their predecessor's exact stop */
     uint32_t stored_members;    /* members copied by the stored-block split (no speculation) */
     uint32_t stored_blocks;     /* their stored blocks */
+    uint32_t final_block_pieces; /* pieces planned to start inside a member's final block */
+    uint32_t pad2_;
 } gzgpu_stats;
 
 /* ---- context ---------------------------------------------------------- */

@@ -72,6 +72,8 @@ pub struct gzgpu_stats {
     pub midblock_linked: u32,
     pub stored_members: u32,
     pub stored_blocks: u32,
+    pub final_block_pieces: u32,
+    pub pad2_: u32,
 }
 
 #[repr(C)]

@@ -62,6 +62,29 @@ def test_midblock_starts(gpu, kind, level):
         ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
 
 
+@pytest.mark.parametrize("enc", ["libdeflate", "zlib"])
+def test_final_block_split(gpu, enc):
+    """The member's final block (BFINAL = 1, never a hit of the reference's
+    finder predicate) is found by the planner's scan of the member's tail and
+    takes starts inside it like any other block: the output stays exact and
+    those starts link."""
+    ctx = gpu.context()
+    old = ctx.tuning()
+    try:
+        ctx.set_tuning(chunk_bytes=64 << 10, pieces_per_chunk=32)
+        raw = W.generate("text", 6 << 20, seed=91)
+        gz = W.gzip_compress(raw, 6, encoder=enc)
+        st = {}
+        out = gpu.decompress(gz, stats=st)
+        say("final block", enc, gz.size, {k: st[k] for k in ("pieces", "midblock_pieces", "midblock_linked",
+                                                            "final_block_pieces", "redecoded")})
+        assert out == raw.tobytes()
+        assert st["final_block_pieces"] > 0
+        assert st["midblock_linked"] >= 0.9 * st["midblock_pieces"]
+    finally:
+        ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
+
+
 @pytest.mark.parametrize("kind", ["text", "random", "runs", "u32"])
 @pytest.mark.parametrize("level", [1, 6, 9])
 def test_kinds_levels_split(gpu, fine_grid, kind, level):
