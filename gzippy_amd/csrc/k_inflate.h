@@ -392,18 +392,9 @@ __device__ __forceinline__ uint64_t marker_lanes(uint32_t v) { return __builtin_
 // symbols before it.  All index math is 32-bit (segment positions stay
 // below 2^31).
 //
-// Chunk kb (entries kb .. kb + 63 of the copy, lane j holding entry kb + j)
-// goes to the ring and the output.
-template <typename RT>
-__device__ __forceinline__ void copy_chunk_out(RT *ring, const OutSeg &o, uint32_t gpos, uint32_t pos, uint32_t kb,
-                                               uint32_t len, uint32_t v) {
-    const uint32_t lane = lane_id();
-    if (lane + kb < len) {
-        ring_at<RT>(ring, (gpos + kb) * (uint32_t)sizeof(RT) + lane * (uint32_t)sizeof(RT)) = (RT)v;
-        put_e<RT>(o, pos + kb, lane, v);
-    }
-    consumed(v);
-}
+// The output width W (1: u16 entries, 0: u8) is a template parameter of the
+// copy, decided once per copy; chunks are written through copy_chunk, every
+// chunk but the copy's last without a lane mask.
 
 // Marker bookkeeping of a copy while the output is u16.  last_marker_end is
 // a bound: every entry from it to the current position is a literal (the
@@ -416,18 +407,37 @@ struct MarkerTrack {
     bool on;     // the copy may copy markers
     uint64_t m;  // lanes of the chunks so far that copied one
 };
-template <typename RT>
+template <uint32_t W>
 __device__ __forceinline__ MarkerTrack marker_track(const OutSeg &o, uint32_t pos, uint32_t dist,
                                                     uint64_t last_marker_end) {
-    return {is_wide<RT>(o) && (int64_t)(o.seg0 + pos) - (int64_t)dist < (int64_t)last_marker_end, 0ull};
-}
-// chunk kb of the copy holds v (lanes past the copy's end do not count)
-__device__ __forceinline__ void marker_chunk(MarkerTrack &t, uint32_t kb, uint32_t len, uint32_t v) {
-    if (t.on) t.m |= marker_lanes(v) & first_lanes(len - kb);
+    return {W != 0 && (int64_t)(o.seg0 + pos) - (int64_t)dist < (int64_t)last_marker_end, 0ull};
 }
 __device__ __forceinline__ void marker_done(const MarkerTrack &t, const OutSeg &o, uint32_t pos, uint32_t len,
                                             uint64_t &last_marker_end) {
     if (t.m) last_marker_end = o.seg0 + pos + len;
+}
+
+// Entry soff + voff of the segment, width W known at compile time (see put_e).
+template <uint32_t W>
+__device__ __forceinline__ void put_w(const OutSeg &o, uint32_t soff, uint32_t voff, uint32_t v) {
+    if (W) *(gptr<uint16_t>)(o.p + (uint64_t)(soff * 2u + voff * 2u)) = (uint16_t)v;
+    else o.p[(uint64_t)(soff + voff)] = (uint8_t)v;
+}
+
+// Chunk kb (entries kb .. kb + 63 of the copy, lane j holding entry kb + j)
+// to the ring and the output.  FULL: the copy covers the whole chunk (no lane
+// mask); otherwise lanes past the copy's end write nothing.
+template <typename RT, uint32_t W, bool FULL>
+__device__ __forceinline__ void copy_chunk(RT *ring, const OutSeg &o, uint32_t gpos, uint32_t pos, uint32_t kb,
+                                           uint32_t len, uint32_t v, MarkerTrack &t) {
+    const uint32_t lane = lane_id();
+    constexpr uint32_t SZ = (uint32_t)sizeof(RT);
+    if (FULL || lane + kb < len) {
+        ring_at<RT>(ring, (gpos + kb) * SZ + lane * SZ) = (RT)v;
+        put_w<W>(o, pos + kb, lane, v);
+    }
+    if (W && t.on) t.m |= FULL ? marker_lanes(v) : marker_lanes(v) & first_lanes(len - kb);
+    consumed(v);
 }
 
 // Where entries more than a ring back are read from (see InflateShared).
@@ -461,7 +471,7 @@ __device__ __forceinline__ uint32_t far_fetch(const FarSrc &f, int64_t r) {
 // straight-line body, so every load issued is consumed on the same path:
 // no load is left in flight past the copy, where the decode loop would have
 // to wait for it (and, on this ISA, for every store before it).
-template <typename RT, uint32_t NC>
+template <typename RT, uint32_t W, uint32_t NC>
 __device__ __forceinline__ void far_copy(RT *ring, const OutSeg &o, const FarSrc &f, uint32_t pos, uint32_t gpos,
                                          uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
     const uint32_t lane = lane_id();
@@ -473,19 +483,17 @@ __device__ __forceinline__ void far_copy(RT *ring, const OutSeg &o, const FarSrc
         v[k] = i < len ? far_fetch<RT>(f, rel + (int64_t)i) : 0u;
     }
     compiler_fence();
-    MarkerTrack t = marker_track<RT>(o, pos, dist, last_marker_end);
+    MarkerTrack t = marker_track<W>(o, pos, dist, last_marker_end);
 #pragma unroll
-    for (uint32_t k = 0; k < NC; k++) {
-        copy_chunk_out<RT>(ring, o, gpos, pos, WAVE * k, len, v[k]);
-        marker_chunk(t, WAVE * k, len, v[k]);
-    }
+    for (uint32_t k = 0; k + 1 < NC; k++) copy_chunk<RT, W, true>(ring, o, gpos, pos, WAVE * k, len, v[k], t);
+    copy_chunk<RT, W, false>(ring, o, gpos, pos, WAVE * (NC - 1), len, v[NC - 1], t);
     marker_done(t, o, pos, len, last_marker_end);
     compiler_fence();
 }
 
-template <typename RT, typename SH>
-__device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o, uint32_t pos, uint32_t g0,
-                                                         uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
+template <typename RT, uint32_t W, typename SH>
+__device__ __attribute__((always_inline)) void wave_copy_w(SH &sh, const OutSeg &o, uint32_t pos, uint32_t g0,
+                                                           uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
     RT *ring = sh.ring;
     const uint32_t lane = lane_id();
     const uint32_t gpos = g0 + pos;
@@ -493,11 +501,11 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
     if (dist > RING<RT>) {
         const FarSrc f = far_src(sh);
         const uint32_t nc = (len + WAVE - 1) / WAVE;
-        if (nc <= 1) far_copy<RT, 1>(ring, o, f, pos, gpos, len, dist, last_marker_end);
-        else if (nc == 2) far_copy<RT, 2>(ring, o, f, pos, gpos, len, dist, last_marker_end);
-        else if (nc == 3) far_copy<RT, 3>(ring, o, f, pos, gpos, len, dist, last_marker_end);
-        else if (nc == 4) far_copy<RT, 4>(ring, o, f, pos, gpos, len, dist, last_marker_end);
-        else far_copy<RT, 5>(ring, o, f, pos, gpos, len, dist, last_marker_end);
+        if (nc <= 1) far_copy<RT, W, 1>(ring, o, f, pos, gpos, len, dist, last_marker_end);
+        else if (nc == 2) far_copy<RT, W, 2>(ring, o, f, pos, gpos, len, dist, last_marker_end);
+        else if (nc == 3) far_copy<RT, W, 3>(ring, o, f, pos, gpos, len, dist, last_marker_end);
+        else if (nc == 4) far_copy<RT, W, 4>(ring, o, f, pos, gpos, len, dist, last_marker_end);
+        else far_copy<RT, W, 5>(ring, o, f, pos, gpos, len, dist, last_marker_end);
         return;
     }
     const uint32_t sb = (gpos - dist) * SZ;  // ring byte of the source's first entry (before wrapping)
@@ -554,27 +562,38 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
         }
     }
     compiler_fence();
-    MarkerTrack t = marker_track<RT>(o, pos, dist, last_marker_end);
-    copy_chunk_out<RT>(ring, o, gpos, pos, 0, len, v0);
-    marker_chunk(t, 0, len, v0);
-    if (len > 64) {
-        copy_chunk_out<RT>(ring, o, gpos, pos, 64, len, v1);
-        marker_chunk(t, 64, len, v1);
-        if (len > 128) {
-            copy_chunk_out<RT>(ring, o, gpos, pos, 128, len, v2);
-            marker_chunk(t, 128, len, v2);
-            if (len > 192) {
-                copy_chunk_out<RT>(ring, o, gpos, pos, 192, len, v3);
-                marker_chunk(t, 192, len, v3);
-                if (len > 256) {
-                    copy_chunk_out<RT>(ring, o, gpos, pos, 256, len, v4);
-                    marker_chunk(t, 256, len, v4);
+    // chunk k is full when the copy reaches past it (len > 64 (k + 1))
+    MarkerTrack t = marker_track<W>(o, pos, dist, last_marker_end);
+    if (len <= 64) {
+        copy_chunk<RT, W, false>(ring, o, gpos, pos, 0, len, v0, t);
+    } else {
+        copy_chunk<RT, W, true>(ring, o, gpos, pos, 0, len, v0, t);
+        if (len <= 128) {
+            copy_chunk<RT, W, false>(ring, o, gpos, pos, 64, len, v1, t);
+        } else {
+            copy_chunk<RT, W, true>(ring, o, gpos, pos, 64, len, v1, t);
+            if (len <= 192) {
+                copy_chunk<RT, W, false>(ring, o, gpos, pos, 128, len, v2, t);
+            } else {
+                copy_chunk<RT, W, true>(ring, o, gpos, pos, 128, len, v2, t);
+                if (len <= 256) {
+                    copy_chunk<RT, W, false>(ring, o, gpos, pos, 192, len, v3, t);
+                } else {
+                    copy_chunk<RT, W, true>(ring, o, gpos, pos, 192, len, v3, t);
+                    copy_chunk<RT, W, false>(ring, o, gpos, pos, 256, len, v4, t);
                 }
             }
         }
     }
     marker_done(t, o, pos, len, last_marker_end);
     compiler_fence();
+}
+
+template <typename RT, typename SH>
+__device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o, uint32_t pos, uint32_t g0,
+                                                         uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
+    if (is_wide<RT>(o)) wave_copy_w<RT, 1>(sh, o, pos, g0, len, dist, last_marker_end);
+    else wave_copy_w<RT, 0>(sh, o, pos, g0, len, dist, last_marker_end);
 }
 
 // A round whose output fits one wavefront (run <= 64 entries) and holds
