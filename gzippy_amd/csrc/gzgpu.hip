@@ -2385,18 +2385,21 @@ int decompress_device_member(gzgpu_ctx *c, const void *d_in, size_t in_len, void
         HIPCHK(hipSetDevice(c->device));
         c->st = gzgpu_stats{};
         *out_len = 0;
-        // envelope: header from the first bytes, trailer from the last 8
-        size_t head = std::min<size_t>(in_len, 64 * KIB);
-        std::vector<uint8_t> h(head + 8);
-        HIPCHK(hipMemcpyAsync(h.data(), d_in, head, hipMemcpyDeviceToHost, c->stream));
+        // envelope: header from the first bytes, trailer from the last 8,
+        // into pinned staging (one small transfer each, one wait); a header
+        // longer than the first 4 KiB (FEXTRA / FNAME / FCOMMENT) is read
+        // again in full below
+        size_t head = std::min<size_t>(in_len, 4 * KIB);
+        uint8_t *h = (uint8_t *)c->pin.alloc(head + 8);
+        memset(h + head, 0, 8);
+        HIPCHK(hipMemcpyAsync(h, d_in, head, hipMemcpyDeviceToHost, c->stream));
         if (in_len >= 8)
-            HIPCHK(hipMemcpyAsync(h.data() + head, (const uint8_t *)d_in + in_len - 8, 8, hipMemcpyDeviceToHost,
-                                  c->stream));
+            HIPCHK(hipMemcpyAsync(h + head, (const uint8_t *)d_in + in_len - 8, 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         if (in_len < 2 || h[0] != 0x1f || h[1] != 0x8b) return (int)GZGPU_OK;
         size_t hl;
-        int st = parse_header(h.data(), head, &hl);
-        const uint32_t exp_crc = le32(h.data() + head), exp_isize = le32(h.data() + head + 4);
+        int st = parse_header(h, head, &hl);
+        const uint32_t exp_crc = le32(h + head), exp_isize = le32(h + head + 4);
         // a long FEXTRA / FNAME / FCOMMENT: read more of the header
         for (size_t more = head; st == GZGPU_INVALID_HEADER && more < in_len;) {
             more = std::min<size_t>(in_len, 4 * more);

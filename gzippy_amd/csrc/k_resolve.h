@@ -277,16 +277,46 @@ __global__ void __launch_bounds__(256) k_resolve_body(const ResolvePiece *__rest
 #pragma unroll
                 for (int k = 0; k < 8; k++) x[k] = sw[k];
                 x[8] = sh ? sw[8] : 0;
-                uint32_t ob[4];
+                uint32_t e[8];
+                uint32_t mk = 0;  // bit 15 of an entry: a marker (literals are below 256)
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const uint32_t lo = __builtin_amdgcn_alignbit(x[2 * k + 1], x[2 * k], sh);
-                    const uint32_t hi = __builtin_amdgcn_alignbit(x[2 * k + 2], x[2 * k + 1], sh);
-                    const uint32_t b0 = resolve16(lo & 0xFFFF, win, near_start, (int64_t)r.out_pos);
-                    const uint32_t b1 = resolve16(lo >> 16, win, near_start, (int64_t)r.out_pos);
-                    const uint32_t b2 = resolve16(hi & 0xFFFF, win, near_start, (int64_t)r.out_pos);
-                    const uint32_t b3 = resolve16(hi >> 16, win, near_start, (int64_t)r.out_pos);
-                    ob[k] = b0 | b1 << 8 | b2 << 16 | b3 << 24;
+                for (int k = 0; k < 8; k++) {
+                    e[k] = __builtin_amdgcn_alignbit(x[k + 1], x[k], sh);
+                    mk |= e[k];
+                }
+                uint32_t ob[4];
+                if (!ballot((mk & 0x80008000u) != 0)) {
+                    // no marker in the wave's blocks: the low bytes, two
+                    // dwords of entries into one of bytes
+#pragma unroll
+                    for (int k = 0; k < 4; k++) ob[k] = __builtin_amdgcn_perm(e[2 * k + 1], e[2 * k], 0x06040200u);
+                } else if (!near_start) {
+                    // markers: every entry reads the window byte its value
+                    // names, literals a fixed byte inside it (no lane-masked
+                    // branches): window byte m of marker 32768 + m is at
+                    // win[m - 32768], i.e. at wbase + value
+                    gptr<const uint8_t> wbase = win - 2 * (int64_t)WIN;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        uint32_t o = 0;
+#pragma unroll
+                        for (int h = 0; h < 4; h++) {
+                            const uint32_t v = (e[2 * k + (h >> 1)] >> (16 * (h & 1))) & 0xFFFFu;
+                            const uint32_t wv = wbase[(uint64_t)max(v, MARKER_BASE)];
+                            o |= (v < 256 ? v : wv) << (8 * h);
+                        }
+                        ob[k] = o;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const uint32_t lo = e[2 * k], hi = e[2 * k + 1];
+                        const uint32_t b0 = resolve16(lo & 0xFFFF, win, near_start, (int64_t)r.out_pos);
+                        const uint32_t b1 = resolve16(lo >> 16, win, near_start, (int64_t)r.out_pos);
+                        const uint32_t b2 = resolve16(hi & 0xFFFF, win, near_start, (int64_t)r.out_pos);
+                        const uint32_t b3 = resolve16(hi >> 16, win, near_start, (int64_t)r.out_pos);
+                        ob[k] = b0 | b1 << 8 | b2 << 16 | b3 << 24;
+                    }
                 }
                 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
                 u32x4 o;
