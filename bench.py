@@ -314,7 +314,8 @@ def batch_leg(args, dev, ctx):
                       "batched_over_plain": round(p_s / sb_s, 2)},
             "pieces": st["pieces"],
             "autotuned": {"chunk_bytes": tuned.chunk_bytes, "pieces_per_chunk": tuned.pieces_per_chunk,
-                          "waves_per_cu": tuned.waves_per_cu, "batch_split_bytes": tuned.batch_split_bytes},
+                          "waves_per_cu": tuned.waves_per_cu, "batch_split_bytes": tuned.batch_split_bytes,
+                          "lut_bits": tuned.lut_bits},
             "batched_phase_ms": {k: round(st[k], 3) for k in ("ms_find", "ms_inflate", "ms_resolve", "ms_crc",
                                                               "ms_total", "ms_inflate_marker", "ms_inflate_clean")},
             "batched_launches": st["launches"],
@@ -563,7 +564,7 @@ def main():
     if args.autotune:
         t, tms = gzippy_amd.autotune(d_in, [0], [gz.size], d_out, [0], [n], budget_ms=1500, ctx=ctx)
         tuning = {"chunk_bytes": t.chunk_bytes, "pieces_per_chunk": t.pieces_per_chunk, "waves_per_cu": t.waves_per_cu,
-                  "autotune_ms": round(tms, 3)}
+                  "lut_bits": t.lut_bits, "autotune_ms": round(tms, 3)}
         log(f"autotune: {tuning}")
 
     stats = None

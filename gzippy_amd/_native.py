@@ -51,7 +51,7 @@ class ServerStats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("chunk_bytes", ctypes.c_uint64), ("pieces_per_chunk", ctypes.c_uint32),
                 ("waves_per_cu", ctypes.c_uint32), ("batch_split_bytes", ctypes.c_uint64),
-                ("miss_every", ctypes.c_uint32), ("reserved_", ctypes.c_uint32)]
+                ("miss_every", ctypes.c_uint32), ("lut_bits", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):
@@ -192,6 +192,8 @@ class Context:
         self.lib.gzgpu_get_tuning(self.h, ctypes.byref(t))
         return t
 
-    def set_tuning(self, chunk_bytes=0, pieces_per_chunk=0, waves_per_cu=0, batch_split_bytes=0, miss_every=0):
-        t = Tuning(chunk_bytes, pieces_per_chunk, waves_per_cu, batch_split_bytes, miss_every, 0)
-        self.lib.gzgpu_set_tuning(self.h, ctypes.byref(t))
+    def set_tuning(self, chunk_bytes=0, pieces_per_chunk=0, waves_per_cu=0, batch_split_bytes=0, miss_every=0,
+                   lut_bits=0):
+        t = Tuning(chunk_bytes, pieces_per_chunk, waves_per_cu, batch_split_bytes, miss_every, lut_bits)
+        if self.lib.gzgpu_set_tuning(self.h, ctypes.byref(t)) != OK:
+            raise ValueError(self.lib.gzgpu_last_error(self.h).decode())

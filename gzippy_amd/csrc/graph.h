@@ -225,7 +225,7 @@ void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
                        g->members.as<GraphMember>(), g->n, g->pieces.as<PieceDesc>(), g->head.as<GraphHead>(),
                        g->queue.as<uint32_t>());
     if (!plain) {
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(g->n, (uint64_t)c->num_cus * U8_WAVES_PER_CU);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(g->n, (uint64_t)c->num_cus * inflate_waves_per_cu<uint8_t>(c, 9));
         hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(grid), dim3(64), 0, c->stream, g->in_base, g->in_total,
                            g->pieces.as<PieceDesc>(), g->order.as<uint32_t>(), g->n, g->queue.as<uint32_t>(),
                            g->pout.as<PieceOut>());

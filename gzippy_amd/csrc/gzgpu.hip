@@ -330,6 +330,9 @@ struct gzgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 256;
+    // resident inflate wavefronts per CU, [ring u8 / u16][litlen root 9 / 10],
+    // from the runtime's occupancy query (LDS and VGPRs, allocation granules)
+    uint32_t inflate_occ[2][2] = {{16, 16}, {16, 16}};
     gzgpu_tuning tune{};
     std::string err;
     DevBuf d_lists;     // resolve and CRC lists of the current call, one upload
@@ -362,6 +365,10 @@ struct gzgpu_ctx {
     int stream_rc = 0;
     gzgpu_stats st{};
 };
+
+namespace {
+uint32_t lut_bits(const gzgpu_ctx *c) { return c->tune.lut_bits == 10 ? 10u : 9u; }
+}  // namespace
 
 namespace {
 
@@ -555,8 +562,13 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
 
 uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-// inflate wavefronts per CU (10 KB of LDS each, four per SIMD)
-constexpr uint32_t U8_WAVES_PER_CU = 16, U16_WAVES_PER_CU = 16;
+// The litlen LUT root width of the inflate kernel (tuning lut_bits: 0 = 9).
+uint32_t lut_bits(const gzgpu_ctx *c);
+// Resident inflate wavefronts per CU for a ring type and LUT root width.
+template <typename RT>
+uint32_t inflate_waves_per_cu(const gzgpu_ctx *c, uint32_t lw) {
+    return c->inflate_occ[sizeof(RT) == 2 ? 1 : 0][lw == 10 ? 1 : 0];
+}
 // planner: smallest piece (compressed KiB; the reference's block finder
 // keeps 32 KiB, gzip_block_finder.rs:109) and pieces per resident wavefront.
 // With starts inside blocks a member's pieces come out equal, so one launch
@@ -676,13 +688,20 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     c->d_counters.ensure(64);
     HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
     // LDS bounds residency (160 KiB per CU): 10 KB per wavefront
-    uint32_t per_cu = L.u16 ? U16_WAVES_PER_CU : U8_WAVES_PER_CU;
+    const uint32_t lw = lut_bits(c);
+    uint32_t per_cu = inflate_waves_per_cu<RT>(c, lw);
     if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
     L.grid = (uint32_t)std::min<uint64_t>(L.n, (uint64_t)c->num_cus * per_cu);
     if (debug_enabled()) fprintf(stderr, "[gzgpu] host: inflate setup %.3f ms\n", host_ms() - t0);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    hipLaunchKernelGGL(k_inflate<RT>, dim3(L.grid), dim3(64), 0, c->stream, d_in, in_len, c->d_pieces.as<PieceDesc>(),
-                       c->d_order.as<uint32_t>(), L.n, c->d_counters.as<uint32_t>(), c->d_pout.as<PieceOut>());
+    if (lw == 10)
+        hipLaunchKernelGGL((k_inflate<RT, 10>), dim3(L.grid), dim3(64), 0, c->stream, d_in, in_len,
+                           c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), L.n, c->d_counters.as<uint32_t>(),
+                           c->d_pout.as<PieceOut>());
+    else
+        hipLaunchKernelGGL((k_inflate<RT, 9>), dim3(L.grid), dim3(64), 0, c->stream, d_in, in_len,
+                           c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), L.n, c->d_counters.as<uint32_t>(),
+                           c->d_pout.as<PieceOut>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     c->st.launches++;
@@ -922,9 +941,9 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
                             (double)member_cost(cbits, M.exp_isize);
         return std::min<uint64_t>(min_spacing, std::max<uint64_t>(8 * MIN_SPACING_FLOOR_KIB * KIB, (uint64_t)bits));
     };
-    const uint64_t slots = (uint64_t)c->num_cus *  // resident u8-ring inflate wavefronts
-                           (c->tune.waves_per_cu ? std::min<uint32_t>(c->tune.waves_per_cu, U8_WAVES_PER_CU)
-                                                 : U8_WAVES_PER_CU);
+    const uint32_t lut_waves = inflate_waves_per_cu<uint16_t>(c, lut_bits(c));
+    const uint64_t slots = (uint64_t)c->num_cus *  // resident inflate wavefronts
+                           (c->tune.waves_per_cu ? std::min<uint32_t>(c->tune.waves_per_cu, lut_waves) : lut_waves);
     // Piece grid.  Explicit pieces_per_chunk: chunk_bits / ppc compressed bits.
     // Default: balance decode time (member_cost); every piece aims at one
     // resident slot's share of the call's total cost, and a member's pieces
@@ -2260,6 +2279,29 @@ int gzgpu_ctx_create(int device, gzgpu_ctx **out) {
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
         for (auto &e : c->res_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         init_constants(c);
+        {
+            // workgroups of one wavefront: blocks per CU = wavefronts per CU
+            int occ[4] = {0, 0, 0, 0};
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[0], k_inflate<uint8_t, 9>, 64, 0));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[1], k_inflate<uint8_t, 10>, 64, 0));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[2], k_inflate<uint16_t, 9>, 64, 0));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[3], k_inflate<uint16_t, 10>, 64, 0));
+            // LDS goes to workgroups in 1280-byte granules (gfx950's 160 KiB
+            // is 128 of them): at 11952 bytes (the 10-bit root) 12 fit, not
+            // the 13 the query reports (measured: the 13th wavefront of
+            // each CU started only when another had finished)
+            const uint32_t lds[4] = {(uint32_t)sizeof(InflateShared<uint8_t, 9>),
+                                     (uint32_t)sizeof(InflateShared<uint8_t, 10>),
+                                     (uint32_t)sizeof(InflateShared<uint16_t, 9>),
+                                     (uint32_t)sizeof(InflateShared<uint16_t, 10>)};
+            for (int i = 0; i < 4; i++) {
+                const uint32_t by_lds = (160u * 1024u) / (uint32_t)round_up(lds[i], 1280);
+                c->inflate_occ[i / 2][i % 2] = std::max(1u, std::min({16u, (uint32_t)std::max(occ[i], 1), by_lds}));
+            }
+            if (debug_enabled())
+                fprintf(stderr, "[gzgpu] inflate wavefronts per CU: u8 %u/%u, u16 %u/%u (9/10-bit root)\n",
+                        c->inflate_occ[0][0], c->inflate_occ[0][1], c->inflate_occ[1][0], c->inflate_occ[1][1]);
+        }
         c->scratch = device_scratch(device);
         HIPCHK(hipHostMalloc((void **)&c->noalloc_stage, NOALLOC_STAGE_BYTES, hipHostMallocDefault));
         return (int)GZGPU_OK;
@@ -2297,6 +2339,10 @@ void gzgpu_ctx_destroy(gzgpu_ctx *c) {
 
 int gzgpu_set_tuning(gzgpu_ctx *c, const gzgpu_tuning *t) {
     if (!c || !t) return GZGPU_INVALID_ARGUMENT;
+    if (t->lut_bits != 0 && t->lut_bits != 9 && t->lut_bits != 10) {
+        set_err(c, "lut_bits must be 0 (automatic), 9 or 10");
+        return GZGPU_INVALID_ARGUMENT;
+    }
     c->tune = *t;
     return GZGPU_OK;
 }
@@ -2578,11 +2624,12 @@ int gzgpu_autotune(gzgpu_ctx *c, const void *d_in, const uint64_t *in_off, const
     // cost) at two occupancies, then explicit grids; batch split rules
     std::vector<gzgpu_tuning> cand;
     for (uint64_t split : {(uint64_t)0, (uint64_t)(1 * MIB), (uint64_t)(4 * MIB)}) {
-        cand.push_back({0, 0, 0, split});
+        cand.push_back({0, 0, 0, split, 0, 0});
         if (split) continue;
-        cand.push_back({0, 0, 3, split});
+        cand.push_back({0, 0, 0, split, 0, 10});  // a 10-bit litlen LUT root (13 wavefronts per CU)
+        cand.push_back({0, 0, 3, split, 0, 0});
         for (uint64_t chunk : {256 * KIB, 512 * KIB, 1024 * KIB})
-            for (uint32_t ppc : {4u, 8u, 16u}) cand.push_back({chunk, ppc, 0, split});
+            for (uint32_t ppc : {4u, 8u, 16u}) cand.push_back({chunk, ppc, 0, split, 0, 0});
     }
     const gzgpu_tuning keep = c->tune;
     std::vector<int32_t> status(n);

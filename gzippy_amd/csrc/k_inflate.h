@@ -88,16 +88,20 @@ constexpr uint32_t RING = sizeof(RT) == 2 ? 2048u : 4096u;
 template <typename RT>
 constexpr uint32_t RMASK = RING<RT> - 1;
 
-template <typename RT>
+// LW: the litlen LUT root width (9 or 10 bits), a kernel instantiation.
+template <typename RT, uint32_t LW = 9>
 struct __align__(16) InflateShared {
     // Two-level Huffman LUTs: a root of LLR / DR bits, then subtables for
     // the codes longer than the root (zlib's inflate_table layout), in the
     // same array after the root.  9/8-bit roots keep a wavefront's LDS at
-    // 10 KB.  A subtable that does not fit its reserve leaves its codes to
-    // the canonical slow path.
-    static constexpr int LLR = 9;
+    // 10 KB (16 wavefronts per CU); a 10-bit litlen root takes 1.8 KB more
+    // (13 per CU) and sends fewer codes to a subtable.  The subtable reserve
+    // covers the worst case of zlib's bound (ENOUGH_LENS: 852 entries for a
+    // 9-bit root, 1334 for 10); a subtable that does not fit its reserve
+    // leaves its codes to the canonical slow path.
+    static constexpr int LLR = (int)LW;
     static constexpr int DR = 8;
-    static constexpr int LL_SUB = 384;
+    static constexpr int LL_SUB = LW == 9 ? 384 : 320;
     static constexpr int D_SUB = 64;
     RT ring[RING<RT>];
     uint32_t ll[(1 << LLR) + LL_SUB];
@@ -1153,8 +1157,8 @@ __device__ __attribute__((always_inline)) void stored_copy(const uint8_t *in_, u
     compiler_fence();
 }
 
-template <typename RT>
-__device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &sh, const uint8_t *__restrict__ in,
+template <typename RT, uint32_t LW>
+__device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, LW> &sh, const uint8_t *__restrict__ in,
                                                              uint64_t in_len, const PieceDesc &pd, PieceOut *po) {
     const uint32_t lane = lane_id();
     const bool marker = (pd.flags & PF_MARKER) != 0;
@@ -1392,13 +1396,13 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT> &
 // Persistent inflate kernel: every workgroup (one wavefront) pulls piece
 // indices from a device queue until it is drained; `order` lists the pieces
 // longest-first.  RT = ring entry type: uint16_t for MARKER pieces, uint8_t
-// for all others.
-template <typename RT>
+// for all others; LW the litlen LUT root width.
+template <typename RT, uint32_t LW = 9>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
                                                 const PieceDesc *__restrict__ pieces,
                                                 const uint32_t *__restrict__ order, uint32_t n, uint32_t *queue,
                                                 PieceOut *out) {
-    __shared__ InflateShared<RT> sh;
+    __shared__ InflateShared<RT, LW> sh;
     for (;;) {
         if (lane_id() == 0) sh.qi = atomicAdd(queue, 1u);
         __syncthreads();
@@ -1407,7 +1411,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         if (qi >= n) break;
         uint32_t pi = order[qi];
         PieceDesc pd = pieces[pi];
-        inflate_piece<RT>(sh, in, in_len, pd, out + pi);
+        inflate_piece<RT, LW>(sh, in, in_len, pd, out + pi);
         __syncthreads();
     }
 }

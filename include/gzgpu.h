@@ -60,7 +60,8 @@ typedef struct {
     uint64_t batch_split_bytes; /* batch: split only members above this compressed size (0: by expected output) */
     uint32_t miss_every;        /* test hook: plan every n-th speculative start one bit late, so that its
                                    predecessor's end misses it and the re-decode path runs (0: off) */
-    uint32_t reserved_;         /* 0 */
+    uint32_t lut_bits;          /* inflate kernel's litlen LUT root: 0 (automatic: 9), 9, or 10 (a 1.8 KB larger
+                                   LUT per wavefront: 13 resident per CU instead of 16) */
 } gzgpu_tuning;
 
 /* Per-call timing and shape, filled when a non-NULL pointer is passed. */

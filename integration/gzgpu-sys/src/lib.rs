@@ -42,7 +42,7 @@ pub struct gzgpu_tuning {
     pub waves_per_cu: u32,
     pub batch_split_bytes: u64,
     pub miss_every: u32,
-    pub reserved_: u32,
+    pub lut_bits: u32,
 }
 
 #[repr(C)]

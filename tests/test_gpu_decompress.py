@@ -85,6 +85,42 @@ def test_final_block_split(gpu, enc):
         ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
 
 
+@pytest.mark.parametrize("kind,level", [("text", 6), ("u32", 1), ("runs", 9)])
+def test_lut_root_width(gpu, kind, level):
+    """The 10-bit litlen LUT root (tuning lut_bits, an autotuner candidate):
+    a split member (marker pieces, u16-ring kernel) and a batch of whole
+    members (u8-ring kernel) decode to the same bytes as with the default
+    9-bit root; other widths are refused."""
+    import torch
+    ctx = gpu.context()
+    old = ctx.tuning()
+    try:
+        raw = W.generate(kind, 6 << 20, seed=17 * level + len(kind))
+        gz = W.gzip_compress(raw, level)
+        small = [W.generate(kind, (64 << 10) + 77 * i, seed=i) for i in range(8)]
+        sgz = [W.gzip_compress(x, level) for x in small]
+        for bits in (10, 9, 0):
+            ctx.set_tuning(lut_bits=bits)
+            st = {}
+            assert gpu.decompress(gz, stats=st) == raw.tobytes(), bits
+            say("lut root", kind, level, bits, st["pieces"])
+            blob = np.concatenate(sgz)
+            off = np.cumsum([0] + [g.size for g in sgz[:-1]])
+            d_in = torch.from_numpy(blob).cuda()
+            cap = [x.size + 64 for x in small]
+            ooff = np.cumsum([0] + cap[:-1])
+            d_out = torch.zeros(int(sum(cap)), dtype=torch.uint8, device="cuda")
+            status, olen, crc, _ = gpu.decompress_batch(d_in, off, [g.size for g in sgz], d_out, ooff, cap)
+            out = d_out.cpu().numpy()
+            for i, x in enumerate(small):
+                assert status[i] == 0 and olen[i] == x.size
+                assert np.array_equal(out[ooff[i]: ooff[i] + x.size], x), (bits, i)
+        with pytest.raises(ValueError):
+            ctx.set_tuning(lut_bits=11)
+    finally:
+        ctx.set_tuning(old.chunk_bytes, old.pieces_per_chunk, old.waves_per_cu, old.batch_split_bytes)
+
+
 @pytest.mark.parametrize("kind", ["text", "random", "runs", "u32"])
 @pytest.mark.parametrize("level", [1, 6, 9])
 def test_kinds_levels_split(gpu, fine_grid, kind, level):
