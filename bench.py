@@ -271,6 +271,30 @@ def batch_leg(args, dev, ctx):
         bt.append(time.perf_counter() - ts)
     b_s = sorted(bt)[len(bt) // 2]
 
+    # the hipGraph form on the same members: one replay (block finder, piece
+    # plan, inflate launches, link checks, window chain, resolve, CRC32 and
+    # trailer checks, all on the device)
+    d_out.zero_()
+    torch.cuda.synchronize()
+    with gzippy_amd.BatchGraph(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=ctx) as g:
+        g_status, g_olen, _, g_st = g.launch()
+        torch.cuda.synchronize()
+        if not (bool((g_status == 0).all()) and torch.equal(d_out[:D], ref)):
+            log("FAILED: graph output differs (batch members)")
+            sys.exit(1)
+        gt = []
+        for _ in range(args.batch_steps):
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            _, _, _, g_st = g.launch()
+            torch.cuda.synchronize()
+            gt.append(time.perf_counter() - ts)
+    g_s = sorted(gt)[len(gt) // 2]
+    graph = {"graph_gbs": round(D / g_s / 1e9, 3), "graph_ms": round(g_s * 1e3, 3),
+             "device_ms": round(g_st["ms_total"], 3), "pieces": g_st["pieces"], "redecoded": g_st["redecoded"],
+             "candidates": g_st["candidates"]}
+    log(f"batch graph: {graph}")
+
     # plain: one inflate launch per member (one wavefront, whole member),
     # back to back on one stream, no host synchronisation in between; on the
     # first `plain_members` members (a random sample of the same mix: at one
@@ -313,6 +337,7 @@ def batch_leg(args, dev, ctx):
                       "batched_gbs_same_members": round(Ds / sb_s / 1e9, 3), "batched_ms_same_members": round(sb_s * 1e3, 3),
                       "batched_over_plain": round(p_s / sb_s, 2)},
             "pieces": st["pieces"],
+            "graph": graph,
             "autotuned": {"chunk_bytes": tuned.chunk_bytes, "pieces_per_chunk": tuned.pieces_per_chunk,
                           "waves_per_cu": tuned.waves_per_cu, "batch_split_bytes": tuned.batch_split_bytes,
                           "lut_bits": tuned.lut_bits},
@@ -385,9 +410,10 @@ def mixed_leg(args, dev, ctx):
 def graph_leg(args, dev, ctx, members, max_kib):
     """A batch of small members (lognormal around 16 KiB, at most `max_kib`;
     the four kinds, levels 1/6/9): the hipGraph form (one replay: header
-    parse, inflate from a device-side queue, CRC32 and trailer checks on the
-    device; members decoded whole) against gzgpu_decompress_batch (host
-    planning, members split into pieces) on the same members."""
+    parse, block finder and piece plan for the members worth splitting,
+    inflate from a device-side queue, link checks, resolve, CRC32 and trailer
+    checks, all on the device) against gzgpu_decompress_batch (host planning)
+    on the same members."""
     import numpy as np
     import torch
 

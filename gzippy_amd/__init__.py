@@ -280,10 +280,12 @@ def decompress_members_plain(d_in, in_off, in_len, d_out, out_off, out_cap, ctx=
 
 class BatchGraph:
     """A fixed batch layout captured as one hipGraph (gzgpu_batch_graph_*):
-    header parse, the persistent inflate kernel fed by a device-side queue,
-    CRC32 and trailer checks all on the device.  launch() replays it; the
-    contents of d_in may change between launches, the layout may not.  Members
-    are decoded whole, one wavefront each."""
+    header parse, block finder and piece plan for the members worth splitting,
+    the persistent inflate kernel fed by device-side queues, link checks and
+    re-decodes, window chain, marker resolution, CRC32 and trailer checks, all
+    on the device.  launch() replays it; the contents of d_in may change
+    between launches, the layout may not.  stats: pieces (members plus split
+    pieces), redecoded, candidates (block-finder hits)."""
 
     def __init__(self, d_in, in_off, in_len, d_out, out_off, out_cap, ctx=None):
         _check(d_in, "d_in")

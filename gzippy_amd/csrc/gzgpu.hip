@@ -697,11 +697,11 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     if (lw == 10)
         hipLaunchKernelGGL((k_inflate<RT, 10>), dim3(L.grid), dim3(64), 0, c->stream, d_in, in_len,
                            c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), L.n, c->d_counters.as<uint32_t>(),
-                           c->d_pout.as<PieceOut>());
+                           c->d_pout.as<PieceOut>(), (const uint32_t *)nullptr);
     else
         hipLaunchKernelGGL((k_inflate<RT, 9>), dim3(L.grid), dim3(64), 0, c->stream, d_in, in_len,
                            c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), L.n, c->d_counters.as<uint32_t>(),
-                           c->d_pout.as<PieceOut>());
+                           c->d_pout.as<PieceOut>(), (const uint32_t *)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     c->st.launches++;

@@ -1396,13 +1396,19 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, L
 // Persistent inflate kernel: every workgroup (one wavefront) pulls piece
 // indices from a device queue until it is drained; `order` lists the pieces
 // longest-first.  RT = ring entry type: uint16_t for MARKER pieces, uint8_t
-// for all others; LW the litlen LUT root width.
+// for all others; LW the litlen LUT root width.  n_dev (when not null): the
+// piece count is read from device memory (a launch inside a graph whose list
+// a kernel before it wrote), n is its capacity.
 template <typename RT, uint32_t LW = 9>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
                                                 const PieceDesc *__restrict__ pieces,
                                                 const uint32_t *__restrict__ order, uint32_t n, uint32_t *queue,
-                                                PieceOut *out) {
+                                                PieceOut *out, const uint32_t *__restrict__ n_dev) {
     __shared__ InflateShared<RT, LW> sh;
+    if (n_dev) {
+        const uint32_t m = uni(*n_dev);
+        n = m < n ? m : n;
+    }
     for (;;) {
         if (lane_id() == 0) sh.qi = atomicAdd(queue, 1u);
         __syncthreads();

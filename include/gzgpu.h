@@ -170,15 +170,21 @@ int gzgpu_autotune(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_off, con
 
 /* ---- batch as a replayable hipGraph ------------------------------------- */
 /* Captures the whole decode of a fixed batch layout (members as for
- * gzgpu_decompress_batch) into one hipGraph: header parse, a persistent
- * inflate kernel fed by a device-side work queue (one wavefront per member,
- * longest first), CRC32 and the trailer checks all run on the device, so a
- * launch is one hipGraphLaunch and one small copy back.  The contents of d_in
- * may change between launches; offsets, lengths and capacities may not.
- * Members are decoded whole (no speculative split): meant for batches of many
- * small members.  Destroy a graph before its context.  No counterpart in the
- * reference (the batched form of decompress_multi_member_parallel_to_vec,
- * bgzf.rs:2370, with the device keeping every decision). */
+ * gzgpu_decompress_batch) into one hipGraph, so a launch is one
+ * hipGraphLaunch and one small copy back.  On the device: header parse; for
+ * members whose decode cost exceeds one resident wavefront's share, the
+ * block finder, a piece plan (starts at block headers or inside blocks) and
+ * after the inflate the link check, with in-place re-decodes of missed
+ * pieces and a whole-remainder fallback; a persistent inflate kernel fed by
+ * device-side work queues (longest first); the window chain, marker
+ * resolution, CRC32 and the trailer checks.  The contents of d_in may change
+ * between launches; offsets, lengths and capacities may not (piece counts
+ * come from the trailers at creation: any contents decode exactly).  Device
+ * memory: marker slabs of about 6 x the split members' output capacity.
+ * Destroy a graph before its context.  No counterpart in the reference (the
+ * batched form of decompress_multi_member_parallel_to_vec, bgzf.rs:2370, and
+ * of the speculative pipeline, chunk_fetcher.rs:638-1600, with the device
+ * keeping every decision). */
 typedef struct gzgpu_batch_graph gzgpu_batch_graph;
 int gzgpu_batch_graph_create(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_off, const uint64_t *in_len,
                              void *d_out, const uint64_t *out_off, const uint64_t *out_cap, size_t n,
