@@ -1078,20 +1078,20 @@ void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
                            g->pieces.as<PieceDesc>(), g->pool.as<uint8_t>(), cnt, g->miss_every);
         hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(std::min<uint32_t>(g->npieces1, slots16)), dim3(64), 0, c->stream,
                            g->in_base, g->in_total, g->pieces.as<PieceDesc>(), g->order.as<uint32_t>(), g->npieces1,
-                           cnt + GC_QUEUE1, g->pout.as<PieceOut>(), (const uint32_t *)nullptr);
+                           cnt + GC_QUEUE1, g->pout.as<PieceOut>(), (const uint32_t *)nullptr, (uint64_t *)nullptr, 0.f);
         hipLaunchKernelGGL(k_graph_link_local, dim3(g->nsplit), dim3(GRAPH_PLAN_THREADS), 0, c->stream,
                            g->splits.as<GraphSplit>(), g->runs.as<GraphSplitRun>(), g->acc.as<GraphAcc>(),
                            g->pout.as<PieceOut>(), g->pieces.as<PieceDesc>(), cnt, g->order2.as<uint32_t>());
         hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(std::min<uint32_t>(g->npieces1, slots16)), dim3(64), 0, c->stream,
                            g->in_base, g->in_total, g->pieces.as<PieceDesc>(), g->order2.as<uint32_t>(), g->npieces1,
-                           cnt + GC_QUEUE2, g->pout.as<PieceOut>(), (const uint32_t *)(cnt + GC_N2));
+                           cnt + GC_QUEUE2, g->pout.as<PieceOut>(), (const uint32_t *)(cnt + GC_N2), (uint64_t *)nullptr, 0.f);
         hipLaunchKernelGGL(k_graph_link_final, dim3(g->nsplit), dim3(GRAPH_PLAN_THREADS), 0, c->stream,
                            g->splits.as<GraphSplit>(), g->runs.as<GraphSplitRun>(), g->acc.as<GraphAcc>(),
                            g->pout.as<PieceOut>(), g->pieces.as<PieceDesc>(), g->pool.as<uint8_t>(), cnt,
                            g->order3.as<uint32_t>());
         hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(std::min<uint32_t>(g->nsplit, slots16)), dim3(64), 0, c->stream,
                            g->in_base, g->in_total, g->pieces.as<PieceDesc>(), g->order3.as<uint32_t>(), g->nsplit,
-                           cnt + GC_QUEUE3, g->pout.as<PieceOut>(), (const uint32_t *)(cnt + GC_N3));
+                           cnt + GC_QUEUE3, g->pout.as<PieceOut>(), (const uint32_t *)(cnt + GC_N3), (uint64_t *)nullptr, 0.f);
         hipLaunchKernelGGL(k_graph_resolve_prep, dim3(1), dim3(GRAPH_PREP_THREADS), 0, c->stream,
                            g->splits.as<GraphSplit>(), g->nsplit, g->runs.as<GraphSplitRun>(), g->acc.as<GraphAcc>(),
                            g->pout.as<PieceOut>(), g->pieces.as<PieceDesc>(), g->members.as<GraphMember>(), g->d_out,
@@ -1119,7 +1119,7 @@ void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(g->n, (uint64_t)c->num_cus * inflate_waves_per_cu<uint8_t>(c, 9));
         hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(grid), dim3(64), 0, c->stream, g->in_base, g->in_total,
                            g->pieces.as<PieceDesc>(), g->order.as<uint32_t>(), g->n, cnt + GC_QUEUE1,
-                           g->pout.as<PieceOut>(), (const uint32_t *)nullptr);
+                           g->pout.as<PieceOut>(), (const uint32_t *)nullptr, (uint64_t *)nullptr, 0.f);
     } else {
         // member i: piece i, a queue counter of its own (cleared first), order
         // entry 0 of the identity (the kernel decodes pieces[order[0]])
@@ -1127,7 +1127,7 @@ void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
         for (uint32_t i = 0; i < g->n; i++)
             hipLaunchKernelGGL(k_inflate<uint8_t>, dim3(1), dim3(64), 0, c->stream, g->in_base, g->in_total,
                                g->pieces.as<PieceDesc>() + i, g->zero.as<uint32_t>(), 1u, g->counters.as<uint32_t>() + i,
-                               g->pout.as<PieceOut>() + i, (const uint32_t *)nullptr);
+                               g->pout.as<PieceOut>() + i, (const uint32_t *)nullptr, (uint64_t *)nullptr, 0.f);
     }
     hipLaunchKernelGGL(k_graph_members, dim3(nb), dim3(256), 0, c->stream, g->members.as<GraphMember>(), g->n,
                        g->pout.as<PieceOut>(), g->mstat.as<int32_t>(), g->mlen.as<uint64_t>());

@@ -653,6 +653,17 @@ bool midblock_enabled() {
     return v == 1;
 }
 
+// Issue-priority balance inside the inflate launch (progress_report);
+// GZGPU_PROG=0 turns it off.
+bool prog_balance() {
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("GZGPU_PROG");
+        v = !(e && *e == '0');
+    }
+    return v == 1;
+}
+
 // One launch of the persistent inflate kernel over `descs` (pulled
 // longest-first by `cost`) on the context stream; finish_inflate waits for
 // it and reads the per-piece results back.
@@ -692,16 +703,30 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     uint32_t per_cu = inflate_waves_per_cu<RT>(c, lw);
     if (c->tune.waves_per_cu) per_cu = std::min<uint32_t>(per_cu, c->tune.waves_per_cu);
     L.grid = (uint32_t)std::min<uint64_t>(L.n, (uint64_t)c->num_cus * per_cu);
+    // issue-priority balance when every piece is resident from the start
+    // (progress fractions are then comparable): the launch-wide progress
+    // counter (zeroed with the queue) and the bits of all pieces
+    uint64_t *prog = nullptr;
+    float prog_inv = 0.f;
+    if (prog_balance() && L.n <= L.grid) {
+        uint64_t bits = 0;
+        for (const PieceDesc &d : descs) {
+            const uint64_t e = std::min(d.stop_bit, d.limit_bit);
+            bits += e > d.start_bit ? e - d.start_bit : 1;
+        }
+        prog = (uint64_t *)(c->d_counters.as<uint8_t>() + 8);
+        prog_inv = 1.0f / (float)std::max<uint64_t>(bits, 1);
+    }
     if (debug_enabled()) fprintf(stderr, "[gzgpu] host: inflate setup %.3f ms\n", host_ms() - t0);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     if (lw == 10)
         hipLaunchKernelGGL((k_inflate<RT, 10>), dim3(L.grid), dim3(64), 0, c->stream, d_in, in_len,
                            c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), L.n, c->d_counters.as<uint32_t>(),
-                           c->d_pout.as<PieceOut>(), (const uint32_t *)nullptr);
+                           c->d_pout.as<PieceOut>(), (const uint32_t *)nullptr, prog, prog_inv);
     else
         hipLaunchKernelGGL((k_inflate<RT, 9>), dim3(L.grid), dim3(64), 0, c->stream, d_in, in_len,
                            c->d_pieces.as<PieceDesc>(), c->d_order.as<uint32_t>(), L.n, c->d_counters.as<uint32_t>(),
-                           c->d_pout.as<PieceOut>(), (const uint32_t *)nullptr);
+                           c->d_pout.as<PieceOut>(), (const uint32_t *)nullptr, prog, prog_inv);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[6], c->stream));
     c->st.launches++;
@@ -1171,8 +1196,11 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         // tails pulled last, measured slower: a tail piece is all marker
         // phase, and the window chain and marker resolution grow with the
         // piece count.)  Measured on that member: inflate kernel 4.33 ms
-        // flat, 4.20 ms at a slope of 0.22-0.3.
-        const double tilt = TILT;
+        // flat, 4.20 ms at a slope of 0.22-0.3.  With the issue-priority
+        // balance inside the launch (progress_report in k_inflate.h) the
+        // lagging wavefronts issue first, and equal pieces are best (kernel
+        // 3.76 ms sloped without it, 3.61 ms flat with it).
+        const double tilt = prog_balance() ? 0.0 : TILT;
         gpts.resize(np + 1);  // grid point k: where the cumulative cost reaches its share
         {
             size_t j = 0;

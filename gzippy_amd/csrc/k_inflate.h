@@ -905,11 +905,46 @@ __device__ __attribute__((noinline)) uint64_t sync_point(SH &sh, const uint8_t *
 // this round's stores and copies, so their latency overlaps the copies.  A
 // round that can take nothing (a code longer than the LUT root, an invalid
 // code, a full output) is left to scalar_step.
+// ---- balance across the launch: issue priority from progress -------------
+// All pieces of a launch are planned to take equal time and, when they fit
+// the resident wavefronts, start together,
+// but the SIMD's issue arbiter favours its oldest wavefront, so pieces finish
+// spread out and the launch waits for the last.  Every PROG_EVERY rounds a
+// wavefront adds the bits it consumed to a launch-wide counter (one atomic,
+// its return gives the launch's progress) and sets its priority by how far
+// its own piece's progress is behind the launch's: the lagging wavefronts
+// issue first.  A scheduling hint only; the output does not depend on it.
+struct Prog {
+    uint64_t *ctr;            // launch-wide bits consumed (null: off)
+    float inv_total, inv_span;
+    uint64_t start, last;     // piece's first bit, position at the last report
+    uint32_t k;
+};
+// Measured on a 1 GiB text member (3969 pieces): a report every 64 rounds
+// and priority bands 2% of the piece wide took the kernel 3.76 -> 3.61 ms;
+// every 16 or 32 rounds the atomics cost more than the balance gains, every
+// 128 or 256 balances too late; bands of 0.5% or 5% were slower.
+constexpr uint32_t PROG_EVERY = 64;
+constexpr float PROG_BAND = 0.02f;
+
+__device__ __attribute__((noinline)) void progress_report(Prog &pg, uint64_t P) {
+    const uint64_t d = P - pg.last;
+    pg.last = P;
+    uint64_t G = 0;
+    if (lane_id() == 0) G = atomicAdd((unsigned long long *)pg.ctr, (unsigned long long)d) + d;
+    G = uni64(G);
+    const float behind = (float)G * pg.inv_total - (float)(P - pg.start) * pg.inv_span;
+    if (behind > PROG_BAND) __builtin_amdgcn_s_setprio(3);
+    else if (behind > 0.0f) __builtin_amdgcn_s_setprio(2);
+    else if (behind > -PROG_BAND) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 template <typename RT, typename SH>
 __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
                                                                   uint64_t &P, OutSeg &o, uint64_t limit_bit,
                                                                   uint64_t &last_marker_end, uint32_t &err,
-                                                                  uint64_t &stop_at) {
+                                                                  uint64_t &stop_at, Prog &pg) {
     const uint32_t lane = lane_id();
     // every control value is re-asserted wave-uniform (readfirstlane): the
     // compiler's divergence analysis cannot prove it for values that came
@@ -937,6 +972,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
         P = uni64(P);
         pos = uni(pos);
         o.p = (gptr<uint8_t>)uni64((uint64_t)o.p);  // stores then take the base from SGPRs
+        if (pg.ctr && (++pg.k & (PROG_EVERY - 1)) == 0) progress_report(pg, P);
         last_marker_end = uni64(last_marker_end);
         stop_at = uni64(stop_at);
         // 64-bit orderings as the sign of a difference (bit positions are
@@ -1159,7 +1195,8 @@ __device__ __attribute__((always_inline)) void stored_copy(const uint8_t *in_, u
 
 template <typename RT, uint32_t LW>
 __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, LW> &sh, const uint8_t *__restrict__ in,
-                                                             uint64_t in_len, const PieceDesc &pd, PieceOut *po) {
+                                                             uint64_t in_len, const PieceDesc &pd, PieceOut *po,
+                                                             uint64_t *prog, float prog_inv_total) {
     const uint32_t lane = lane_id();
     const bool marker = (pd.flags & PF_MARKER) != 0;
     if (marker && sizeof(RT) == 1) {  // host error: marker pieces need the u16-ring kernel
@@ -1185,6 +1222,16 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, L
     uint64_t t_begin = wall_clock64();
     uint64_t cp_bit = NO_CP, cp_out = 0;
     uint32_t fixed_built = 0;  // the fixed tables are in the LUTs
+    Prog pg;
+    pg.ctr = prog;
+    pg.inv_total = prog_inv_total;
+    {
+        const uint64_t end = pd.stop_bit < pd.limit_bit ? pd.stop_bit : pd.limit_bit;
+        const uint64_t span = end > pd.start_bit ? end - pd.start_bit : 1;
+        pg.inv_span = 1.0f / (float)span;
+    }
+    pg.start = pg.last = pd.start_bit;
+    pg.k = 0;
 
     // ring = the last RING entries of the window before the piece (piece
     // position a at slot a mod RING)
@@ -1309,7 +1356,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, L
                 stop_at = sync_point(sh, in, in_len, pd.stop_bit, first, pd.limit_bit);
                 if (stop_at != NO_SYNC && stop_at <= P) stop_at = NO_SYNC;
             }
-            r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err, stop_at);
+            r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err, stop_at, pg);
             br.init(in, in_len, P);
         }
         if (r == R_STOPMID) {
@@ -1338,7 +1385,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, L
                 // a flip inside a block resumes that block's symbols
                 if (btype != 0) {
                     uint64_t P = br.pos();
-                    r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err, stop_at);
+                    r = decode_symbols<RT>(sh, in, in_len, P, o, pd.limit_bit, last_marker_end, err, stop_at, pg);
                     br.init(in, in_len, P);
                 } else {
                     r = R_EOB;
@@ -1398,12 +1445,15 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, L
 // longest-first.  RT = ring entry type: uint16_t for MARKER pieces, uint8_t
 // for all others; LW the litlen LUT root width.  n_dev (when not null): the
 // piece count is read from device memory (a launch inside a graph whose list
-// a kernel before it wrote), n is its capacity.
+// a kernel before it wrote), n is its capacity.  prog (when not null): the
+// launch-wide progress counter (zeroed before the launch) and 1 / the bits of
+// all its pieces, for the issue-priority balance (progress_report).
 template <typename RT, uint32_t LW = 9>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_inflate(const uint8_t *__restrict__ in, uint64_t in_len,
                                                 const PieceDesc *__restrict__ pieces,
                                                 const uint32_t *__restrict__ order, uint32_t n, uint32_t *queue,
-                                                PieceOut *out, const uint32_t *__restrict__ n_dev) {
+                                                PieceOut *out, const uint32_t *__restrict__ n_dev,
+                                                uint64_t *prog, float prog_inv_total) {
     __shared__ InflateShared<RT, LW> sh;
     if (n_dev) {
         const uint32_t m = uni(*n_dev);
@@ -1417,7 +1467,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         if (qi >= n) break;
         uint32_t pi = order[qi];
         PieceDesc pd = pieces[pi];
-        inflate_piece<RT, LW>(sh, in, in_len, pd, out + pi);
+        inflate_piece<RT, LW>(sh, in, in_len, pd, out + pi, prog, prog_inv_total);
         __syncthreads();
     }
 }
