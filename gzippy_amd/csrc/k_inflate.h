@@ -927,13 +927,14 @@ struct Prog {
 constexpr uint32_t PROG_EVERY = 64;
 constexpr float PROG_BAND = 0.02f;
 
-__device__ __attribute__((noinline)) void progress_report(Prog &pg, uint64_t P) {
-    const uint64_t d = P - pg.last;
-    pg.last = P;
+// (by value: a reference would keep the caller's Prog in scratch memory)
+__device__ __attribute__((noinline)) void progress_report(uint64_t *ctr, float inv_total, float inv_span,
+                                                         uint64_t start, uint64_t last, uint64_t P) {
+    const uint64_t d = P - last;
     uint64_t G = 0;
-    if (lane_id() == 0) G = atomicAdd((unsigned long long *)pg.ctr, (unsigned long long)d) + d;
+    if (lane_id() == 0) G = atomicAdd((unsigned long long *)ctr, (unsigned long long)d) + d;
     G = uni64(G);
-    const float behind = (float)G * pg.inv_total - (float)(P - pg.start) * pg.inv_span;
+    const float behind = (float)G * inv_total - (float)(P - start) * inv_span;
     if (behind > PROG_BAND) __builtin_amdgcn_s_setprio(3);
     else if (behind > 0.0f) __builtin_amdgcn_s_setprio(2);
     else if (behind > -PROG_BAND) __builtin_amdgcn_s_setprio(1);
@@ -972,7 +973,10 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
         P = uni64(P);
         pos = uni(pos);
         o.p = (gptr<uint8_t>)uni64((uint64_t)o.p);  // stores then take the base from SGPRs
-        if (pg.ctr && (++pg.k & (PROG_EVERY - 1)) == 0) progress_report(pg, P);
+        if (pg.ctr && (++pg.k & (PROG_EVERY - 1)) == 0) {
+            progress_report(pg.ctr, pg.inv_total, pg.inv_span, pg.start, pg.last, P);
+            pg.last = P;
+        }
         last_marker_end = uni64(last_marker_end);
         stop_at = uni64(stop_at);
         // 64-bit orderings as the sign of a difference (bit positions are
