@@ -14,9 +14,11 @@
 //      start, into the member's output);
 //   2. split members only: the block finder over their compressed bytes
 //      (k_bf_stage_a / _b, k_bf_estimate; the ranges are fixed at creation);
-//   3. k_graph_bucket: every hit goes to its member's grid cell (np cells of
-//      equal compressed bits); a cell keeps its first hit (a piece start)
-//      and the sum of its hits' output estimates;
+//   3. k_graph_hash_hits, k_graph_bucket: every hit goes to its member's
+//      grid cell (np cells of equal compressed bits); a cell keeps its first
+//      dynamic header, its first stored block that chains (the block after
+//      its payload is another hit, found in a device-side hash set), its last
+//      real header, and the sum of its hits' output estimates;
 //   4. k_graph_plan, one workgroup per split member: marker piece c starts
 //      at cell c's first hit (a cell without one merges into the piece
 //      before it), the piece before it stops there, and its marker slab is
@@ -93,12 +95,14 @@ struct GraphSplitRun {
     uint32_t first_bad, why;          // link check: first piece not accepted (np: none); 1 unlinked, 2 overflow, 3 error
 };
 
-// Planning cell: its first hit that may start a piece (bit_offset << 27 |
-// hit index; ~0 none), its last real block header of any kind (same form; 0
-// none), its first final-block header (bit; ~0 none), the sum (8.8 fixed
-// point) and count of its hits' output estimates.
+// Planning cell: its first dynamic-header hit that may start a piece
+// (bit_offset << 27 | hit index; ~0 none), its last real block header
+// (dynamic, or a chained stored block; same form, 0 none), its first
+// final-block header (bit; ~0 none), its first chained stored-block hit, the
+// sum (8.8 fixed point) and count of its hits' output estimates.
 struct GraphCell {
     uint64_t key, last, fin;
+    uint64_t skey;  // its first chained stored hit (same form as key; ~0 none)
     uint32_t rsum, rcnt;
 };
 
@@ -148,10 +152,53 @@ __device__ __forceinline__ PieceDesc graph_dummy_piece(uint32_t member) {
     return d;
 }
 
-__global__ void k_graph_clear(GraphCell *__restrict__ cells, uint32_t ncells, uint32_t *__restrict__ counters) {
+__global__ void k_graph_clear(GraphCell *__restrict__ cells, uint32_t ncells, uint32_t *__restrict__ counters,
+                              uint64_t *__restrict__ htab, uint32_t ht_size) {
     const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (i0 < GC_N) counters[i0] = 0;
-    for (uint32_t i = i0; i < ncells; i += gridDim.x * blockDim.x) cells[i] = GraphCell{~0ull, 0ull, ~0ull, 0u, 0u};
+    for (uint32_t i = i0; i < ncells; i += gridDim.x * blockDim.x)
+        cells[i] = GraphCell{~0ull, 0ull, ~0ull, ~0ull, 0u, 0u};
+    for (uint32_t i = i0; i < ht_size; i += gridDim.x * blockDim.x) htab[i] = ~0ull;
+}
+
+// ---- the stored-block chain test on the device ---------------------------
+// A LEN / ~LEN coincidence in compressed data passes the stored-block test
+// about once per 2^16 byte positions, so stored hits are only starts when
+// chained: the block after the payload is another hit (a dynamic header at
+// exactly that bit, or a stored block whose LEN field follows it) or the
+// member ends there -- the host planner's test.  Every hit goes into an
+// open-addressing set: a dynamic header by its bit, a stored block by the
+// byte of its LEN field.
+constexpr uint32_t GRAPH_HT_PROBES = 64;
+__device__ __forceinline__ uint64_t graph_hit_key(const gzgpu_boundary &b) {
+    return b.kind == 2 ? b.bit_offset << 1 : (((b.seek_bit + 3) >> 3) << 1) | 1u;
+}
+__device__ __forceinline__ uint32_t graph_hash(uint64_t k) {
+    k *= 0x9E3779B97F4A7C15ull;
+    return (uint32_t)(k >> 32);
+}
+__device__ bool graph_ht_has(const uint64_t *__restrict__ htab, uint32_t mask, uint64_t key) {
+    uint32_t s = graph_hash(key) & mask;
+    for (uint32_t i = 0; i < GRAPH_HT_PROBES; i++, s = (s + 1) & mask) {
+        const uint64_t v = htab[s];
+        if (v == key) return true;
+        if (v == ~0ull) return false;
+    }
+    return false;
+}
+
+__global__ void k_graph_hash_hits(const gzgpu_boundary *__restrict__ hits, const uint32_t *__restrict__ counters,
+                                  uint32_t hit_cap, uint64_t *__restrict__ htab, uint32_t mask) {
+    const uint32_t counted = counters[GC_HITS];
+    const uint32_t nh = counted < hit_cap ? counted : hit_cap;
+    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+        const uint64_t key = graph_hit_key(hits[h]);
+        uint32_t s = graph_hash(key) & mask;
+        for (uint32_t i = 0; i < GRAPH_HT_PROBES; i++, s = (s + 1) & mask) {
+            const uint64_t prev = atomicCAS((unsigned long long *)&htab[s], ~0ull, (unsigned long long)key);
+            if (prev == ~0ull || prev == key) break;  // set full along the probe run: the key is left out
+        }
+    }
 }
 
 __global__ void k_graph_prep(const uint8_t *__restrict__ in, uint8_t *out, const GraphMember *__restrict__ gm,
@@ -225,7 +272,8 @@ __global__ void k_graph_prep(const uint8_t *__restrict__ in, uint8_t *out, const
 __global__ void k_graph_bucket(const gzgpu_boundary *__restrict__ hits, const uint32_t *__restrict__ counters,
                                uint32_t hit_cap, const uint64_t *__restrict__ sp_lo, const uint64_t *__restrict__ sp_hi,
                                const uint32_t *__restrict__ sp_idx, uint32_t nranges, const GraphSplit *__restrict__ sp,
-                               const GraphSplitRun *__restrict__ run, GraphCell *__restrict__ cells) {
+                               const GraphSplitRun *__restrict__ run, GraphCell *__restrict__ cells,
+                               const uint64_t *__restrict__ htab, uint32_t ht_mask) {
     const uint32_t counted = counters[GC_HITS];
     const uint32_t nh = counted < hit_cap ? counted : hit_cap;
     for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
@@ -241,9 +289,18 @@ __global__ void k_graph_bucket(const gzgpu_boundary *__restrict__ hits, const ui
         const GraphSplitRun R = run[s];
         if (!R.ok || b.bit_offset <= R.ds || b.seek_bit >= R.de) continue;
         float ratio;
-        if (b.kind == 2) ratio = exp2f((float)(b.info >> 20) * (1.0f / 256.0f) - 8.0f);
-        else if (b.kind == 0 && (b.info & 1u)) ratio = 0.125f;  // stored: a byte per 8 bits
-        else continue;
+        if (b.kind == 2) {
+            ratio = exp2f((float)(b.info >> 20) * (1.0f / 256.0f) - 8.0f);
+        } else if (b.kind == 0 && (b.info & 1u)) {
+            // chained: the next block starts where the payload ends
+            const uint64_t next = (((b.seek_bit + 3) >> 3) + 4 + (b.info >> 16)) << 3;
+            const bool chained = next == R.de || graph_ht_has(htab, ht_mask, next << 1) ||
+                                 graph_ht_has(htab, ht_mask, ((((next >> 3) + 1)) << 1) | 1u);
+            if (!chained) continue;
+            ratio = 0.125f;  // a byte per 8 bits
+        } else {
+            continue;
+        }
         const bool final_hdr = b.kind == 2 && ((b.info >> HIT_FINAL_SHIFT) & 1u);
         const uint32_t np = sp[s].np;
         uint64_t c = (b.bit_offset - R.ds) / R.spacing;
@@ -257,7 +314,7 @@ __global__ void k_graph_bucket(const gzgpu_boundary *__restrict__ hits, const ui
         // a final block's header is never a piece start (the stop rule stops
         // at non-final headers only), only the header of starts inside it
         if (final_hdr) atomicMin((unsigned long long *)&cell->fin, (unsigned long long)b.bit_offset);
-        else if (c >= 1) atomicMin((unsigned long long *)&cell->key, key);
+        else if (c >= 1) atomicMin((unsigned long long *)(b.kind == 2 ? &cell->key : &cell->skey), key);
     }
 }
 
@@ -327,9 +384,11 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
         a.stop_hdr = ~0ull;
         if (c >= 1 && c < np) {
             const uint64_t g = R.ds + (uint64_t)c * R.spacing;
-            const uint64_t key = C[c].key;
-            const uint64_t hb = key != ~0ull ? key >> 27 : ~0ull;  // first hit that may start a piece
-            const uint64_t next = hb < C[c].fin ? hb : C[c].fin;   // first real header after the grid point
+            const uint64_t dkey = C[c].key, skey = C[c].skey;
+            const uint64_t hd = dkey != ~0ull ? dkey >> 27 : ~0ull;  // first dynamic header
+            const uint64_t hs = skey != ~0ull ? skey >> 27 : ~0ull;  // first chained stored block
+            const uint64_t hb = hd < hs ? hd : hs;
+            const uint64_t next = hb < C[c].fin ? hb : C[c].fin;     // first real header after the grid point
             uint64_t hdr = R.ds;
             bool dyn = true;
             if (lprev) {
@@ -337,7 +396,13 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
                 hdr = h.seek_bit;
                 dyn = h.kind == 2;
             }
-            if (hb != ~0ull && hb - g <= R.spacing / 8) {
+            // a chained stored block near the grid point, else a dynamic
+            // header near it, else a start inside the block holding it, else
+            // the cell's first chained stored block or dynamic header
+            uint64_t key = ~0ull;
+            if (hs != ~0ull && hs - g <= R.spacing / 8) key = skey;
+            else if (hd != ~0ull && hd - g <= R.spacing / 8) key = dkey;
+            if (key != ~0ull) {
                 a.used = 1;
             } else if (midblock && dyn && g >= hdr + GRAPH_MID_HEAD && (next == ~0ull || next >= g + GRAPH_MID_TAIL)) {
                 a.used = 1;
@@ -346,6 +411,7 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
                 a.hdr = hdr;
             } else if (hb != ~0ull) {
                 a.used = 1;
+                key = hs != ~0ull ? skey : dkey;
             }
             if (a.used && !a.mid) {
                 const gzgpu_boundary &h = hits[(uint32_t)(key & ((1u << 27) - 1))];
@@ -793,14 +859,14 @@ struct gzgpu_batch_graph {
     // sequence positions, planning cells, longest chain, finder shape
     uint32_t nsplit = 0, nranges = 0, npieces1 = 0, nslots = 0, npos = 0, ncells = 0, max_chain = 0;
     uint64_t nchunk = 0;
-    uint32_t surv_cap = 0, hit_cap = 0, tile_cap = 0, miss_every = 0;
+    uint32_t surv_cap = 0, hit_cap = 0, tile_cap = 0, miss_every = 0, ht_size = 0;
     const uint8_t *in_base = nullptr;
     uint8_t *d_out = nullptr;
     uint64_t in_total = 0;
     DevBuf members, pieces, order, pout, counters, head, seg_member, seg_start, seg_off, seg_len, seg_tail, mseg, mlen,
         seg_crc, mcrc, status, olen, ocrc, zero, mstat;
     DevBuf splits, runs, cells, acc, ranges, surv, hits, sp_lo, sp_hi, sp_idx, order2, order3, pos_split, plen, ppos, rps,
-        slot_rp, slot_pos, tile_first, tile_crc, scan, pool;
+        slot_rp, slot_pos, tile_first, tile_crc, scan, pool, htab;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void release() {
@@ -811,7 +877,7 @@ struct gzgpu_batch_graph {
         for (DevBuf *b : {&members, &pieces, &order, &pout, &counters, &head, &seg_member, &seg_start, &seg_off, &seg_len,
                           &seg_tail, &mseg, &mlen, &seg_crc, &mcrc, &status, &olen, &ocrc, &zero, &mstat, &splits, &runs,
                           &cells, &acc, &ranges, &surv, &hits, &sp_lo, &sp_hi, &sp_idx, &order2, &order3, &pos_split, &plen,
-                          &ppos, &rps, &slot_rp, &slot_pos, &tile_first, &tile_crc, &scan, &pool})
+                          &ppos, &rps, &slot_rp, &slot_pos, &tile_first, &tile_crc, &scan, &pool, &htab})
             b->release();
     }
 };
@@ -987,6 +1053,12 @@ void graph_setup(gzgpu_batch_graph *g, const void *d_in, const uint64_t *in_off,
         g->hit_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(nbits / 1024, 4096), 1ull << 26);
         g->surv.ensure((size_t)g->surv_cap * 8);
         g->hits.ensure((size_t)g->hit_cap * sizeof(gzgpu_boundary));
+        // the hits' set for the stored-block chain test: a power of two, at
+        // least twice the hits it can hold (capped; a full set only leaves
+        // stored hits unchained)
+        g->ht_size = 1024;
+        while (g->ht_size < 2ull * g->hit_cap && g->ht_size < (1u << 25)) g->ht_size <<= 1;
+        g->htab.ensure((size_t)g->ht_size * 8);
         put(g->pos_split, pos_split.data(), pos_split.size() * 4);
         g->plen.ensure((size_t)g->npos * 8);
         g->ppos.ensure((size_t)g->npos * 8);
@@ -1050,8 +1122,11 @@ void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
     gzgpu_ctx *c = g->c;
     const uint32_t nb = (g->n + 255) / 256, sb = (std::max(g->nseg, 1u) + 255) / 256;
     uint32_t *cnt = g->counters.as<uint32_t>();
-    hipLaunchKernelGGL(k_graph_clear, dim3(std::max<uint32_t>(1, std::min<uint32_t>((g->ncells + 255) / 256, c->num_cus * 4))),
-                       dim3(256), 0, c->stream, g->cells.as<GraphCell>(), g->ncells, cnt);
+    hipLaunchKernelGGL(k_graph_clear,
+                       dim3(std::max<uint32_t>(1, std::min<uint32_t>((std::max(g->ncells, g->ht_size) + 255) / 256,
+                                                                     c->num_cus * 4))),
+                       dim3(256), 0, c->stream, g->cells.as<GraphCell>(), g->ncells, cnt, g->htab.as<uint64_t>(),
+                       g->ht_size);
     hipLaunchKernelGGL(k_graph_prep, dim3(nb), dim3(256), 0, c->stream, g->in_base, g->d_out,
                        g->members.as<GraphMember>(), g->n, g->pieces.as<PieceDesc>(), g->head.as<GraphHead>(),
                        g->splits.as<GraphSplit>(), g->runs.as<GraphSplitRun>());
@@ -1068,10 +1143,14 @@ void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
                            cnt + GC_HITS);
         hipLaunchKernelGGL(k_bf_estimate, dim3(std::min<uint32_t>(g->hit_cap, (uint32_t)c->num_cus * 16)), dim3(64), 0,
                            c->stream, g->in_base, g->in_total, g->hits.as<gzgpu_boundary>(), cnt + GC_HITS, g->hit_cap);
+        hipLaunchKernelGGL(k_graph_hash_hits, dim3(std::min<uint32_t>((g->hit_cap + 255) / 256, (uint32_t)c->num_cus * 8)),
+                           dim3(256), 0, c->stream, g->hits.as<gzgpu_boundary>(), cnt, g->hit_cap,
+                           g->htab.as<uint64_t>(), g->ht_size - 1);
         hipLaunchKernelGGL(k_graph_bucket, dim3(std::min<uint32_t>((g->hit_cap + 255) / 256, (uint32_t)c->num_cus * 8)),
                            dim3(256), 0, c->stream, g->hits.as<gzgpu_boundary>(), cnt, g->hit_cap,
                            g->sp_lo.as<uint64_t>(), g->sp_hi.as<uint64_t>(), g->sp_idx.as<uint32_t>(), g->nranges,
-                           g->splits.as<GraphSplit>(), g->runs.as<GraphSplitRun>(), g->cells.as<GraphCell>());
+                           g->splits.as<GraphSplit>(), g->runs.as<GraphSplitRun>(), g->cells.as<GraphCell>(),
+                           (const uint64_t *)g->htab.p, g->ht_size - 1);
         hipLaunchKernelGGL(k_graph_plan, dim3(g->nsplit), dim3(GRAPH_PLAN_THREADS), 0, c->stream,
                            g->splits.as<GraphSplit>(), g->runs.as<GraphSplitRun>(), g->cells.as<GraphCell>(),
                            g->acc.as<GraphAcc>(), g->hits.as<gzgpu_boundary>(), g->members.as<GraphMember>(),
