@@ -66,7 +66,7 @@ class Stats(ctypes.Structure):
                 ("marker_in_bytes", ctypes.c_uint64), ("marker_out_bytes", ctypes.c_uint64),
                 ("midblock_pieces", ctypes.c_uint32), ("midblock_linked", ctypes.c_uint32),
                 ("stored_members", ctypes.c_uint32), ("stored_blocks", ctypes.c_uint32),
-                ("final_block_pieces", ctypes.c_uint32), ("pad2_", ctypes.c_uint32)]
+                ("final_block_pieces", ctypes.c_uint32), ("stored_runs_skipped", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -84,7 +84,7 @@ struct GraphHead {
 struct GraphSplit {
     uint32_t member, np, first, redo, slot0, cell0, pos0, pad;
     uint64_t slab_off, slab_bytes;  // marker slabs of cells 1 .. np-1 in the slab pool
-    uint64_t redo_off, redo_bytes;  // the re-decode piece's slab
+    uint64_t redo_off, redo_bytes;  // two re-decode slabs of redo_bytes each (an overflowed piece's, the remainder's)
 };
 
 // Per split member, written on the device every replay.
@@ -103,6 +103,7 @@ struct GraphSplitRun {
 struct GraphCell {
     uint64_t key, last, fin;
     uint64_t skey;  // its first chained stored hit (same form as key; ~0 none)
+    uint64_t send;  // the furthest payload end (bit) of the chained stored hits in it (0 none)
     uint32_t rsum, rcnt;
 };
 
@@ -157,7 +158,7 @@ __global__ void k_graph_clear(GraphCell *__restrict__ cells, uint32_t ncells, ui
     const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (i0 < GC_N) counters[i0] = 0;
     for (uint32_t i = i0; i < ncells; i += gridDim.x * blockDim.x)
-        cells[i] = GraphCell{~0ull, 0ull, ~0ull, ~0ull, 0u, 0u};
+        cells[i] = GraphCell{~0ull, 0ull, ~0ull, ~0ull, 0ull, 0u, 0u};
     for (uint32_t i = i0; i < ht_size; i += gridDim.x * blockDim.x) htab[i] = ~0ull;
 }
 
@@ -289,6 +290,7 @@ __global__ void k_graph_bucket(const gzgpu_boundary *__restrict__ hits, const ui
         const GraphSplitRun R = run[s];
         if (!R.ok || b.bit_offset <= R.ds || b.seek_bit >= R.de) continue;
         float ratio;
+        uint64_t pay_end = 0;
         if (b.kind == 2) {
             ratio = exp2f((float)(b.info >> 20) * (1.0f / 256.0f) - 8.0f);
         } else if (b.kind == 0 && (b.info & 1u)) {
@@ -298,6 +300,7 @@ __global__ void k_graph_bucket(const gzgpu_boundary *__restrict__ hits, const ui
                                  graph_ht_has(htab, ht_mask, ((((next >> 3) + 1)) << 1) | 1u);
             if (!chained) continue;
             ratio = 0.125f;  // a byte per 8 bits
+            pay_end = next;
         } else {
             continue;
         }
@@ -311,6 +314,7 @@ __global__ void k_graph_bucket(const gzgpu_boundary *__restrict__ hits, const ui
         atomicAdd(&cell->rsum, fx < 65535.0f ? (uint32_t)fx : 65535u);
         atomicAdd(&cell->rcnt, 1u);
         atomicMax((unsigned long long *)&cell->last, key);
+        if (pay_end) atomicMax((unsigned long long *)&cell->send, (unsigned long long)pay_end);
         // a final block's header is never a piece start (the stop rule stops
         // at non-final headers only), only the header of starts inside it
         if (final_hdr) atomicMin((unsigned long long *)&cell->fin, (unsigned long long)b.bit_offset);
@@ -371,12 +375,17 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
     // pass 1: where each cell's piece starts (the last real header before the
     // cell: an exclusive max-scan of the cells' last headers), and the k-th
     // start (the miss_every test hook plans every n-th start one bit late)
-    uint64_t lcarry = 0, ucarry = 0;
+    uint64_t lcarry = 0, ucarry = 0, scarry = 0;
     for (uint32_t base = 0; base < np; base += NT) {
         const uint32_t c = base + t;
         block_scan_incl<uint64_t>(c < np ? C[c].last : 0ull, sbuf, mx);
         const uint64_t lprev = mx(lcarry, t ? sbuf[t - 1] : 0ull);
         const uint64_t lchunk = sbuf[NT - 1];
+        __syncthreads();
+        // the furthest end of a chained stored payload that starts before the cell
+        block_scan_incl<uint64_t>(c < np ? C[c].send : 0ull, sbuf, mx);
+        const uint64_t sprev = mx(scarry, t ? sbuf[t - 1] : 0ull);
+        const uint64_t schunk = sbuf[NT - 1];
         __syncthreads();
         GraphAcc a{};
         a.end = np;
@@ -384,9 +393,18 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
         a.stop_hdr = ~0ull;
         if (c >= 1 && c < np) {
             const uint64_t g = R.ds + (uint64_t)c * R.spacing;
-            const uint64_t dkey = C[c].key, skey = C[c].skey;
-            const uint64_t hd = dkey != ~0ull ? dkey >> 27 : ~0ull;  // first dynamic header
+            uint64_t dkey = C[c].key;
+            const uint64_t skey = C[c].skey;
+            uint64_t hd = dkey != ~0ull ? dkey >> 27 : ~0ull;       // first dynamic header
             const uint64_t hs = skey != ~0ull ? skey >> 27 : ~0ull;  // first chained stored block
+            // a dynamic header inside a chained stored block's payload is
+            // data (a gzip of deflate data), not a block of this stream:
+            // shadowed when a payload that starts before the cell, or the
+            // cell's own stored blocks after the first, reach past it
+            if (hd != ~0ull && (hd < sprev || (hs < hd && hd < C[c].send))) {
+                hd = ~0ull;
+                dkey = ~0ull;
+            }
             const uint64_t hb = hd < hs ? hd : hs;
             const uint64_t next = hb < C[c].fin ? hb : C[c].fin;     // first real header after the grid point
             uint64_t hdr = R.ds;
@@ -427,6 +445,7 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
         a.plan_mid = a.mid;
         if (c < np) A[c] = a;
         lcarry = mx(lcarry, lchunk);
+        scarry = mx(scarry, schunk);
         ucarry += uchunk;
     }
     if (t == 0) atomicAdd(&counters[GC_PIECES], (uint32_t)ucarry);
@@ -558,14 +577,16 @@ __device__ __forceinline__ void graph_accepted(const GraphAcc &a, const PieceOut
 __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
     k_graph_link_local(const GraphSplit *__restrict__ sp, const GraphSplitRun *__restrict__ run,
                        GraphAcc *__restrict__ acc, const PieceOut *__restrict__ pout, PieceDesc *__restrict__ pieces,
-                       uint32_t *__restrict__ counters, uint32_t *__restrict__ order2) {
+                       uint8_t *__restrict__ pool, uint32_t *__restrict__ counters, uint32_t *__restrict__ order2) {
     __shared__ uint64_t sbuf[GRAPH_PLAN_THREADS];
+    __shared__ uint32_t ovf;
     const uint32_t t = threadIdx.x, NT = blockDim.x;
     const GraphSplit S = sp[blockIdx.x];
     if (!run[blockIdx.x].ok) return;
     const uint32_t np = S.np;
     GraphAcc *A = acc + S.cell0;
     auto out_of = [&](uint64_t k) -> const PieceOut & { return k == 0 ? pout[S.member] : pout[S.first + k - 1]; };
+    if (t == 0) ovf = np;
     uint64_t carry = 0;
     for (uint32_t base = 0; base < np; base += NT) {
         const uint32_t c = base + t;
@@ -579,11 +600,26 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
             const PieceOut &rp = out_of(pred);
             uint64_t lo, hi;
             graph_accepted(A[c], out_of(c), lo, hi);
-            A[c].linked = rp.status == PS_STOPPED && rp.end_bit >= lo && rp.end_bit <= hi;
+            const bool linked = rp.status == PS_STOPPED && rp.end_bit >= lo && rp.end_bit <= hi;
+            A[c].linked = linked;
             A[c].pred = (uint32_t)pred;
+            if (linked && out_of(c).status == PS_OVERFLOW) atomicMin(&ovf, c);
         }
         if (chunk > carry) carry = chunk;
         __syncthreads();
+    }
+    // the first piece with an accepted start that overflowed its slab: again
+    // in place, from its own start (a start inside a block keeps its header
+    // and grid point: the same sync point), into the first half of the
+    // member's re-decode slab
+    if (t == 0 && ovf < np) {
+        const uint32_t pi = S.first + ovf - 1;
+        PieceDesc d = pieces[pi];
+        d.dst = pool + S.redo_off;
+        d.cap = S.redo_bytes;
+        pieces[pi] = d;
+        order2[atomicAdd(&counters[GC_N2], 1u)] = pi;
+        atomicAdd(&counters[GC_REDO], 1u);
     }
     for (uint32_t c = 1 + t; c < np; c += NT) {
         if (!A[c].used || A[c].linked) continue;
@@ -646,6 +682,7 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
         const uint32_t f = fmin;
         uint32_t status = GZGPU_OK, drop = np, why = 0;
         uint64_t redo_bit = ~0ull;
+        bool from_own = false;  // the remainder starts at piece f's own (accepted) start
         if (f == 0) {
             status = pout[S.member].status == PS_OVERFLOW ? GZGPU_OUTPUT_TOO_SMALL : GZGPU_DECODE_FAILED;
             why = pout[S.member].status == PS_OVERFLOW ? 2 : 3;
@@ -663,6 +700,7 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
                 why = 1;
             } else if (rc.status == PS_OVERFLOW) {
                 redo_bit = rc.start_bit;  // a confirmed start whose slab was too small
+                from_own = true;
                 why = 2;
             } else {
                 status = GZGPU_DECODE_FAILED;  // an error at a confirmed start: the stream is corrupt
@@ -674,14 +712,22 @@ __global__ void __launch_bounds__(GRAPH_PLAN_THREADS)
         run[blockIdx.x].why = why;
         if (redo_bit != ~0ull) {
             PieceDesc d{};
-            d.start_bit = redo_bit;
+            if (from_own) {
+                // piece f's own descriptor: a start inside a block must read
+                // that block's header and find the same sync point again
+                d = pieces[S.first + f - 1];
+                d.flags &= ~PF_STOPMID;
+                d.stop_hdr = 0;
+            } else {
+                d.start_bit = redo_bit;  // a block header the piece before stopped at
+                d.flags = PF_MARKER;
+                d.window = nullptr;
+                d.member = S.member;
+            }
             d.stop_bit = ~0ull;
             d.limit_bit = run[blockIdx.x].limit;
-            d.dst = pool + S.redo_off;
+            d.dst = pool + S.redo_off + S.redo_bytes;  // the second half
             d.cap = S.redo_bytes;
-            d.window = nullptr;
-            d.flags = PF_MARKER;
-            d.member = S.member;
             pieces[S.redo] = d;
             run[blockIdx.x].fail = GZGPU_PENDING;
             run[blockIdx.x].redo = 1;
@@ -999,9 +1045,11 @@ void graph_setup(gzgpu_batch_graph *g, const void *d_in, const uint64_t *in_off,
                                     (uint64_t)np[i] * (2 * (4096 + 65536 + 1) + 768) + 4096,
                                 256);
         pool += s.slab_bytes;
+        // re-decode slabs, each for the whole member: an overflowed piece's
+        // second try (first half) and the remainder (second half)
         s.redo_off = pool;
         s.redo_bytes = marker_slab_bytes(out_cap[i]);
-        pool += s.redo_bytes;
+        pool += 2 * s.redo_bytes;
         tiles += out_cap[i] / RESOLVE_TILE + np[i] + 1;
         nbits += (in_len[i] - 18) * 8;
         g->max_chain = std::max(g->max_chain, np[i]);
@@ -1160,7 +1208,8 @@ void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
                            cnt + GC_QUEUE1, g->pout.as<PieceOut>(), (const uint32_t *)nullptr, (uint64_t *)nullptr, 0.f);
         hipLaunchKernelGGL(k_graph_link_local, dim3(g->nsplit), dim3(GRAPH_PLAN_THREADS), 0, c->stream,
                            g->splits.as<GraphSplit>(), g->runs.as<GraphSplitRun>(), g->acc.as<GraphAcc>(),
-                           g->pout.as<PieceOut>(), g->pieces.as<PieceDesc>(), cnt, g->order2.as<uint32_t>());
+                           g->pout.as<PieceOut>(), g->pieces.as<PieceDesc>(), g->pool.as<uint8_t>(), cnt,
+                           g->order2.as<uint32_t>());
         hipLaunchKernelGGL(k_inflate<uint16_t>, dim3(std::min<uint32_t>(g->npieces1, slots16)), dim3(64), 0, c->stream,
                            g->in_base, g->in_total, g->pieces.as<PieceDesc>(), g->order2.as<uint32_t>(), g->npieces1,
                            cnt + GC_QUEUE2, g->pout.as<PieceOut>(), (const uint32_t *)(cnt + GC_N2), (uint64_t *)nullptr, 0.f);

@@ -955,6 +955,97 @@ void stored_members(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
     }
 }
 
+// Stored runs inside members to be split.  A chain of STORED_RUN_MIN or more
+// stored blocks whose byte-aligned headers follow each other (q -> q + 5 +
+// LEN, each LEN == ~NLEN: a mixed member's random segments) holds no block
+// header of this stream in its payloads, so the bit-level block finder skips
+// them: on a 2 GiB mixed member they are two thirds of the compressed bits.
+// (The chain's first block, whose header need not be byte-aligned, is found
+// by the finder as before.)  Only for members whose output is less than
+// STORED_SCAN_MAX_RATIO times their compressed bytes: better-compressed
+// members hold few stored bytes and skip the scan and its read-back.
+// Returns, per member, its payload bit ranges to skip, sorted.
+constexpr uint32_t STORED_RUN_MIN = 3;
+constexpr uint64_t STORED_SCAN_MAX_RATIO = 4, STORED_SCAN_MIN_BYTES = 4 * MIB;
+constexpr uint64_t STORED_SKIP_MARGIN = 64;  // payload bits kept at either end of a skipped range
+
+std::vector<std::vector<std::pair<uint64_t, uint64_t>>> stored_payloads(gzgpu_ctx *c, const uint8_t *d_in,
+                                                                        uint64_t in_len, const std::vector<Member> &mem,
+                                                                        const std::vector<uint8_t> &want) {
+    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> out(mem.size());
+    std::vector<uint32_t> ms;
+    uint64_t cap = 4096;
+    for (uint32_t m = 0; m < mem.size(); m++) {
+        const Member &M = mem[m];
+        const uint64_t cbytes = (M.de - M.ds) / 8;
+        if (!want[m] || cbytes < STORED_SCAN_MIN_BYTES || (uint64_t)M.exp_isize >= STORED_SCAN_MAX_RATIO * cbytes)
+            continue;
+        ms.push_back(m);
+        cap += cbytes / 1024;
+    }
+    if (ms.empty()) return out;
+    cap = std::min<uint64_t>(cap, 1ull << 28);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        c->d_counters.ensure(64);
+        c->d_surv.ensure(cap * 8);
+        HIPCHK(hipMemsetAsync(c->d_counters.p, 0, 64, c->stream));
+        for (uint32_t m : ms) {
+            const uint64_t first = (mem[m].ds + 7) / 8, last = mem[m].de / 8;
+            const uint32_t blocks = (uint32_t)std::min<uint64_t>((last - first + 1023) / 1024 + 1, (uint64_t)c->num_cus * 8);
+            hipLaunchKernelGGL(k_stored_scan, dim3(blocks), dim3(256), 0, c->stream, d_in, in_len, first, last,
+                               c->d_surv.as<unsigned long long>(), (uint32_t)cap, c->d_counters.as<uint32_t>());
+            HIPCHK(hipGetLastError());
+            c->st.launches++;
+        }
+        const uint32_t *h_cnt = download<uint32_t>(c, c->d_counters.p, 1);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const uint32_t nc = h_cnt[0];
+        if (nc > cap) {
+            cap = (uint64_t)nc + 1024;
+            continue;
+        }
+        const uint64_t *hc = download<uint64_t>(c, c->d_surv.p, nc);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<uint64_t> keys(nc);
+        for (uint32_t i = 0; i < nc; i++) keys[i] = hc[i] & ((1ull << 40) - 1);
+        const std::vector<uint32_t> ord = radix_order(keys);
+        std::vector<uint64_t> pos(nc);  // candidate byte offsets, ascending
+        for (uint32_t i = 0; i < nc; i++) pos[i] = keys[ord[i]];
+        std::vector<uint8_t> seen(nc, 0);
+        auto find = [&](uint64_t q) -> int64_t {
+            const auto it = std::lower_bound(pos.begin(), pos.end(), q);
+            return it != pos.end() && *it == q ? (int64_t)(it - pos.begin()) : -1;
+        };
+        for (uint32_t m : ms) {
+            const Member &M = mem[m];
+            const uint64_t first = (M.ds + 7) / 8, last = M.de / 8;
+            size_t k = (size_t)(std::lower_bound(pos.begin(), pos.end(), first) - pos.begin());
+            for (; k < nc && pos[k] < last; k++) {
+                if (seen[k]) continue;
+                // follow the chain from candidate k
+                std::vector<std::pair<uint64_t, uint32_t>> chain;  // (header byte, LEN)
+                int64_t j = (int64_t)k;
+                while (j >= 0 && !seen[j]) {
+                    seen[j] = 1;
+                    const uint64_t v = hc[ord[j]];
+                    const uint32_t len = (uint32_t)(v >> 40) & 0xFFFF;
+                    chain.push_back({pos[j], len});
+                    if (v >> 56) break;  // BFINAL
+                    j = find(pos[j] + 5 + len);
+                }
+                if (chain.size() < STORED_RUN_MIN) continue;
+                for (auto &b : chain) {
+                    const uint64_t p0 = (b.first + 5) * 8 + STORED_SKIP_MARGIN, p1 = (b.first + 5 + b.second) * 8;
+                    if (p1 > p0 + STORED_SKIP_MARGIN) out[m].push_back({p0, p1 - STORED_SKIP_MARGIN});
+                }
+            }
+            std::sort(out[m].begin(), out[m].end());
+        }
+        return out;
+    }
+    return out;  // candidate lists kept overflowing: nothing skipped
+}
+
 void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vector<Member> &mem) {
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     stored_members(c, d_in, in_len, mem);
@@ -999,13 +1090,32 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
     std::vector<std::pair<uint64_t, uint64_t>> ranges;
     std::vector<uint8_t> final_ok;
     bool estimate = false;
-    for (auto &M : mem)
-        if (splits(M)) {
+    std::vector<uint8_t> split_m(mem.size());
+    for (uint32_t m = 0; m < mem.size(); m++) split_m[m] = splits(mem[m]);
+    const auto skip = stored_payloads(c, d_in, in_len, mem, split_m);
+    for (uint32_t m = 0; m < mem.size(); m++)
+        if (split_m[m]) {
+            const Member &M = mem[m];
             const uint64_t tail0 = M.de - std::min<uint64_t>(M.de - M.ds, FINAL_SCAN_BITS);
-            ranges.push_back({M.ds, tail0});
-            final_ok.push_back(0);
-            ranges.push_back({tail0, M.de});
-            final_ok.push_back(1);
+            // the member's bits less its stored payloads, cut at tail0 (the
+            // tail also reports final-block headers)
+            auto add = [&](uint64_t a, uint64_t b) {
+                if (a < tail0 && a < b) {
+                    ranges.push_back({a, std::min(b, tail0)});
+                    final_ok.push_back(0);
+                }
+                if (b > tail0 && a < b) {
+                    ranges.push_back({std::max(a, tail0), b});
+                    final_ok.push_back(1);
+                }
+            };
+            uint64_t a = M.ds;
+            for (auto &iv : skip[m]) {
+                if (iv.first > a) add(a, iv.first);
+                a = std::max(a, iv.second);
+            }
+            add(a, M.de);
+            c->st.stored_runs_skipped += (uint32_t)skip[m].size();
             estimate = estimate || (M.de - M.ds) / spacing_of(M) >= ESTIMATE_MIN_PIECES;
         }
     const double th0 = debug_enabled() ? host_ms() : 0;

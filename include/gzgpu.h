@@ -91,7 +91,7 @@ typedef struct {
     uint32_t stored_members;    /* members copied by the stored-block split (no speculation) */
     uint32_t stored_blocks;     /* their stored blocks */
     uint32_t final_block_pieces; /* pieces planned to start inside a member's final block */
-    uint32_t pad2_;
+    uint32_t stored_runs_skipped; /* stored blocks inside split members whose payload the block finder skipped */
 } gzgpu_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -180,7 +180,7 @@ int gzgpu_autotune(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_off, con
  * resolution, CRC32 and the trailer checks.  The contents of d_in may change
  * between launches; offsets, lengths and capacities may not (piece counts
  * come from the trailers at creation: any contents decode exactly).  Device
- * memory: marker slabs of about 6 x the split members' output capacity.
+ * memory: marker slabs of about 8 x the split members' output capacity.
  * Destroy a graph before its context.  No counterpart in the reference (the
  * batched form of decompress_multi_member_parallel_to_vec, bgzf.rs:2370, and
  * of the speculative pipeline, chunk_fetcher.rs:638-1600, with the device

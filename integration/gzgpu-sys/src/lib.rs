@@ -73,7 +73,7 @@ pub struct gzgpu_stats {
     pub stored_members: u32,
     pub stored_blocks: u32,
     pub final_block_pieces: u32,
-    pub pad2_: u32,
+    pub stored_runs_skipped: u32,
 }
 
 #[repr(C)]

@@ -143,6 +143,32 @@ def test_mixed_segments(gpu, fine_grid):
         assert gpu.decompress(gz) == raw.tobytes()
 
 
+def test_stored_runs_skipped_by_the_finder(gpu):
+    """A member of 4 MiB segments cycling text / random / runs / u32: the
+    random segments are runs of stored blocks, whose payloads the block
+    finder skips (stat stored_runs_skipped); the output stays exact, also
+    with the payload of one stored block corrupted (a CRC error, not silent
+    bytes)."""
+    import torch
+    raw = W.generate_mixed(24 << 20, seed=31, seg=4 << 20)
+    for level in (1, 6):
+        gz = W.gzip_compress(raw, level)
+        out = torch.zeros(raw.size, dtype=torch.uint8, device="cuda")
+        n, crc, st = gpu.decompress_device(dev(gz), out)
+        say("stored runs", level, gz.size, {k: st[k] for k in ("pieces", "redecoded", "stored_runs_skipped",
+                                                              "candidates", "ms_find")})
+        assert n == raw.size and np.array_equal(out.cpu().numpy(), raw)
+        assert st["stored_runs_skipped"] > 20
+    # a flipped byte inside a skipped payload: the stored copy carries it,
+    # the trailer CRC catches it
+    gz = W.gzip_compress(raw, 1)
+    bad = gz.copy()
+    bad[gz.size * 5 // 24] ^= 0x40  # inside the second segment (random: stored blocks)
+    with pytest.raises(gpu.GzippyError):
+        out = torch.zeros(raw.size, dtype=torch.uint8, device="cuda")
+        gpu.decompress_device(dev(bad), out)
+
+
 def test_device_api(gpu):
     import torch
     raw, gz = W.text_member(8 << 20, seed=9)
