@@ -326,6 +326,21 @@ def batch_leg(args, dev, ctx):
     if not (bool((status_p == 0).all()) and torch.equal(d_sout[:Ds], ref_s)):
         log("FAILED: plain per-member output differs")
         sys.exit(1)
+    # the same members, one full single-member call each (gzgpu_decompress_device:
+    # find, plan, inflate split over the GPU, resolve, CRC; a host round trip
+    # per member): the per-member form at its best
+    d_sout.zero_()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for i in range(k):
+        o0, o1 = int(s_out_off[i]), int(s_out_off[i]) + int(s_out_cap[i])
+        a0, a1 = int(s_in_off[i]), int(s_in_off[i]) + int(s_in_len[i])
+        gzippy_amd.decompress_device(d_in[a0:a1], d_sout[o0:o1], ctx=ctx)
+    torch.cuda.synchronize()
+    c_s = time.perf_counter() - ts
+    if not torch.equal(d_sout[:Ds], ref_s):
+        log("FAILED: per-member calls output differs")
+        sys.exit(1)
     del d_sout, ref_s
     return {"members": len(in_off), "compressed_bytes": C, "decompressed_bytes": D, "seed": args.seed + 3,
             "batched_gbs": round(D / b_s / 1e9, 3), "batched_ms": round(b_s * 1e3, 3),
@@ -335,7 +350,11 @@ def batch_leg(args, dev, ctx):
                       "plain_gbs": round(Ds / p_s / 1e9, 3), "plain_ms": round(p_s * 1e3, 3),
                       "plain_device_ms": round(st_p["ms_total"], 3),
                       "batched_gbs_same_members": round(Ds / sb_s / 1e9, 3), "batched_ms_same_members": round(sb_s * 1e3, 3),
-                      "batched_over_plain": round(p_s / sb_s, 2)},
+                      "batched_over_plain": round(p_s / sb_s, 2),
+                      "per_member_calls": {"form": "one gzgpu_decompress_device call per member (the whole "
+                                                   "single-member pipeline, pieces split over the GPU, a host "
+                                                   "round trip each)", "gbs": round(Ds / c_s / 1e9, 3),
+                                           "ms": round(c_s * 1e3, 3), "batched_over_calls": round(c_s / sb_s, 2)}},
             "pieces": st["pieces"],
             "graph": graph,
             "autotuned": {"chunk_bytes": tuned.chunk_bytes, "pieces_per_chunk": tuned.pieces_per_chunk,
