@@ -549,7 +549,9 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
                                       return a.bit_offset == b.bit_offset && a.seek_bit == b.seek_bit;
                                   }),
                       out.end());
-        if (debug_enabled()) fprintf(stderr, "[gzgpu] host: hits read-back + sort %.3f ms\n", host_ms() - ts);
+        if (debug_enabled())
+            fprintf(stderr, "[gzgpu] host: hits read-back + sort %.3f ms (%u stage-A survivors, %u hits, %llu bits)\n",
+                    host_ms() - ts, h_cnt[0], nh, (unsigned long long)nbits);
         return out;
     }
     throw StatusError{GZGPU_OUT_OF_DEVICE_MEMORY, "block finder: candidate buffers kept overflowing"};
