@@ -607,12 +607,15 @@ constexpr int SPEC_FALLBACK_ROUND = 4, SPEC_MAX_ROUNDS = 16;
 constexpr uint64_t ESTIMATE_MIN_PIECES = 64;
 
 // Decode cost of a member in output-entry units.  A wavefront's decode time
-// is about max(40 ns per compressed bit, 11 ns per output entry) at full
-// occupancy: a decode round covers at most 64 input bits, and a round's
-// copies take time in proportion to their bytes.  Measured per piece: text
-// (91k bits, 300k entries) 3.3 ms, long-copy data (247k bits, 1.77M entries)
-// 19.6 ms, u32 deltas (707k bits, 263k entries) 31 ms.  So a Huffman-coded
-// member costs max(4 x bits, ISIZE).  Stored data (ratio below 1.15, the
+// is about max(a per compressed bit, b per output entry) at full occupancy:
+// a decode round covers at most 128 input bits, and a round's copies take
+// time in proportion to their bytes.  Measured per piece with 64-bit rounds:
+// text (91k bits, 300k entries) 3.3 ms, long-copy data (247k bits, 1.77M
+// entries) 19.6 ms, u32 deltas (707k bits, 263k entries) 31 ms, so a
+// Huffman-coded member costs max(4 x bits, ISIZE).  With 128-bit rounds the
+// bit term fell by about a fifth; weights of 3 and 2.5 per bit planned the
+// 10,000-member batch 5-10% slower and left the mixed-content member
+// unchanged, so the weight stays 4.  Stored data (ratio below 1.15, the
 // reference's StoredParallel cut, decompress/mod.rs:143-151) moves at copy
 // speed, ~830 MB/s for a lone wavefront against ~230 MB/s of entries from
 // long copies, and is weighted 1/4 of its bytes.  ISIZE is the size mod 2^32; a

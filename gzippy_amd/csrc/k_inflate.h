@@ -88,6 +88,16 @@ constexpr uint32_t RING = sizeof(RT) == 2 ? 2048u : 4096u;
 template <typename RT>
 constexpr uint32_t RMASK = RING<RT> - 1;
 
+// Issue-priority balance state of a wavefront (progress_report), in LDS: it
+// is read once every PROG_EVERY rounds and would otherwise hold scalar
+// registers through the whole decode loop.
+struct ProgState {
+    uint64_t *ctr;         // launch-wide bits consumed (null: off)
+    uint64_t start, last;  // piece's first bit, position at the last report
+    float inv_total, inv_span;
+};
+typedef __attribute__((address_space(3))) ProgState lds_prog;
+
 // LW: the litlen LUT root width (9 or 10 bits), a kernel instantiation.
 template <typename RT, uint32_t LW = 9>
 struct __align__(16) InflateShared {
@@ -122,6 +132,7 @@ struct __align__(16) InflateShared {
     // 0 the window (far_win_kind 0: none, 1: u8 buffer, 3: markers)
     uint64_t far_out16, far_out8, far_n16, far_win;
     uint32_t far_win_kind;
+    ProgState pg;
 };
 
 // ---- canonical table construction (whole wave) ----------------------------
@@ -476,6 +487,16 @@ __device__ __forceinline__ uint32_t far_fetch(const FarSrc &f, int64_t r) {
 // no load is left in flight past the copy, where the decode loop would have
 // to wait for it (and, on this ISA, for every store before it).
 template <typename RT, uint32_t W, uint32_t NC>
+__device__ __forceinline__ void far_store(RT *ring, const OutSeg &o, uint32_t pos, uint32_t gpos, uint32_t len,
+                                          uint32_t dist, const uint32_t (&v)[NC], uint64_t &last_marker_end) {
+    MarkerTrack t = marker_track<W>(o, pos, dist, last_marker_end);
+#pragma unroll
+    for (uint32_t k = 0; k + 1 < NC; k++) copy_chunk<RT, W, true>(ring, o, gpos, pos, WAVE * k, len, v[k], t);
+    copy_chunk<RT, W, false>(ring, o, gpos, pos, WAVE * (NC - 1), len, v[NC - 1], t);
+    marker_done(t, o, pos, len, last_marker_end);
+    compiler_fence();
+}
+template <typename RT, uint32_t W, uint32_t NC>
 __device__ __forceinline__ void far_copy(RT *ring, const OutSeg &o, const FarSrc &f, uint32_t pos, uint32_t gpos,
                                          uint32_t len, uint32_t dist, uint64_t &last_marker_end) {
     const uint32_t lane = lane_id();
@@ -487,12 +508,7 @@ __device__ __forceinline__ void far_copy(RT *ring, const OutSeg &o, const FarSrc
         v[k] = i < len ? far_fetch<RT>(f, rel + (int64_t)i) : 0u;
     }
     compiler_fence();
-    MarkerTrack t = marker_track<W>(o, pos, dist, last_marker_end);
-#pragma unroll
-    for (uint32_t k = 0; k + 1 < NC; k++) copy_chunk<RT, W, true>(ring, o, gpos, pos, WAVE * k, len, v[k], t);
-    copy_chunk<RT, W, false>(ring, o, gpos, pos, WAVE * (NC - 1), len, v[NC - 1], t);
-    marker_done(t, o, pos, len, last_marker_end);
-    compiler_fence();
+    far_store<RT, W, NC>(ring, o, pos, gpos, len, dist, v, last_marker_end);
 }
 
 template <typename RT, uint32_t W, typename SH>
@@ -600,10 +616,22 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
     else wave_copy_w<RT, 0>(sh, o, pos, g0, len, dist, last_marker_end);
 }
 
+// Packed symbol of a lookahead lane: a literal is SYM_LIT | its value, a
+// copy is distance | length << 16 (distance <= 32768, length <= 258), end-of-
+// block is SYM_EOB and an invalid or incomplete code 0.  A complete literal
+// or copy is thus any value above SYM_EOB.
+constexpr uint32_t SYM_LIT = 0x80000000u, SYM_EOB = 1u;
+__device__ __forceinline__ bool sym_is_lit(uint32_t w) { return (int32_t)w < 0; }
+// m |= 1 << b for a wave-uniform b, as one scalar instruction
+__device__ __forceinline__ void set_bit64(uint64_t &m, uint32_t b) { __asm__("s_bitset1_b64 %0, %1" : "+s"(m) : "s"(b)); }
+__device__ __forceinline__ uint32_t sym_len(uint32_t w) { return sym_is_lit(w) ? 1u : w >> 16; }
+
 // A round whose output fits one wavefront (run <= 64 entries) and holds
-// copies, in one pass: lane j produces entry j.  Each taken symbol leaves a
-// descriptor (literal value, or copy distance) at its first output slot in
-// LDS scratch; a ballot of the slots gives every lane its symbol.  A copy
+// copies, in one pass: lane j produces entry j.  Each taken symbol (lanes ta
+// of the round's first half, tb of its second, packed as wa / wb, output
+// offsets oa / ob) leaves a descriptor (literal value, or copy distance) at
+// its first output slot in LDS scratch; a ballot of the slots gives every
+// lane its symbol.  A copy
 // entry's source j - dist inside the round is chased by pointer jumping
 // over ds_bpermute (each step doubles the hop, at most six); a source before
 // the round is read from the ring or, more than a ring back, from HBM.  All
@@ -612,9 +640,8 @@ __device__ __attribute__((always_inline)) void wave_copy(SH &sh, const OutSeg &o
 // the code-length array, which only the header reader uses.
 template <typename RT, typename SH>
 __device__ __attribute__((always_inline)) void fused_round(SH &sh, const OutSeg &o, uint32_t pos, uint32_t g0,
-                                                           uint32_t run, bool taken, bool is_lit, uint32_t opos,
-                                                           uint32_t e, uint32_t dist, uint32_t olen,
-                                                           uint64_t &last_marker_end) {
+                                                           uint32_t run, bool ta, uint32_t wa, uint32_t oa, bool tb,
+                                                           uint32_t wb, uint32_t ob, uint64_t &last_marker_end) {
     static_assert(sizeof(sh.lens) >= 4 * WAVE, "scratch");
     const uint32_t lane = lane_id();
     uint32_t *scr = reinterpret_cast<uint32_t *>(sh.lens);
@@ -624,7 +651,8 @@ __device__ __attribute__((always_inline)) void fused_round(SH &sh, const OutSeg 
     // treating each lane's store and load as its own (forwarding the 0)
     scr[lane] = 0u;
     compiler_fence();
-    if (taken && olen != 0) scr[opos] = VALID | (is_lit ? LIT | ((e >> 16) & 0xFFu) : dist);
+    if (ta) scr[oa] = VALID | (sym_is_lit(wa) ? LIT | (wa & 0xFFu) : (wa & 0xFFFFu));
+    if (tb) scr[ob] = VALID | (sym_is_lit(wb) ? LIT | (wb & 0xFFu) : (wb & 0xFFFFu));
     compiler_fence();
     const uint32_t d0 = scr[lane];
     const uint64_t S = ballot(d0 != 0u);  // first output slots of the round's symbols
@@ -712,6 +740,29 @@ __device__ __attribute__((always_inline)) void window128(const uint8_t *__restri
     W1 = sh ? (B >> sh) | (C << (64 - sh)) : B;
 }
 
+// 192 bits of input starting at absolute bit P (W0: bits P..P+63, W1 the
+// next 64, W2 the 64 after), from seven words read as in window128.
+__device__ __attribute__((always_inline)) void window192(const uint8_t *__restrict__ in, uint64_t n, uint64_t P,
+                                                         uint64_t &W0, uint64_t &W1, uint64_t &W2) {
+    const uint64_t wP = P >> 5;
+    const uint32_t sh = (uint32_t)(P & 31);
+    uint32_t a[7];
+    if ((wP + 7) * 4 <= n) {
+        typedef __attribute__((address_space(4))) const uint32_t cword;
+        cword *w = (cword *)reinterpret_cast<const uint32_t *>(in) + wP;
+#pragma unroll
+        for (int k = 0; k < 7; k++) a[k] = w[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 7; k++) a[k] = uni(load_word(in, n, wP + (uint64_t)k));
+    }
+    const uint64_t A = uni64((uint64_t)a[0] | (uint64_t)a[1] << 32), B = uni64((uint64_t)a[2] | (uint64_t)a[3] << 32),
+                   C = uni64((uint64_t)a[4] | (uint64_t)a[5] << 32), D = uni64((uint64_t)a[6]);
+    W0 = sh ? (A >> sh) | (B << (64 - sh)) : A;
+    W1 = sh ? (B >> sh) | (C << (64 - sh)) : B;
+    W2 = sh ? (C >> sh) | (D << (64 - sh)) : C;
+}
+
 // One symbol at P, the slow way (codes longer than the LUT roots, and any
 // condition the multi-symbol round leaves to a single step).
 template <typename RT, typename SH>
@@ -787,58 +838,105 @@ __device__ __attribute__((always_inline)) uint32_t scalar_step(SH &sh, const uin
     return R_CONT;
 }
 
-// Per-lane decode of the symbol that would start at bit P + lane, and the
-// wave masks the walk needs.
-struct Lookahead {
-    uint32_t e;      // litlen LUT entry (literal value in bits 16..23)
-    uint32_t olen;   // entries the symbol produces (0 for end-of-block)
-    uint32_t dist;   // copy distance (length codes)
-    uint32_t nxt;    // lane of the following symbol (>= 64: past this window)
-    uint32_t step;   // nxt, plus bit 8 when the walk stops there (past the window, or not a complete code)
-    uint64_t CONT;   // lanes holding a complete literal or length+distance code
-    uint64_t EOBM;   // lanes holding end-of-block
-    uint64_t LITM;   // lanes holding a literal
+// The symbols that would start at the first bit of ch[h], h < NH (lane j:
+// the input from bit Q_h + j on): the packed form, the lane of the following
+// symbol (j + the symbol's bits, at most j + 48), and the wave masks of the
+// lanes holding a complete literal or length+distance code (CONT), end-of-
+// block (EOBM) and a literal (LITM).  Straight-line per lane: the distance
+// code is looked up on every lane (its result kept only where the code is a
+// length), and the lane classes are wave masks from the compares themselves.
+// The NH decodes run in lockstep, so each LUT level is one LDS round trip
+// for all of them.
+template <int NH, typename SH>
+__device__ __forceinline__ void lane_decode(SH &sh, const uint64_t (&ch)[NH], uint32_t lane, uint32_t (&w)[NH],
+                                            uint32_t (&nxt)[NH], uint64_t (&CONT)[NH], uint64_t (&EOBM)[NH],
+                                            uint64_t (&LITM)[NH]) {
+    constexpr int EQ = 32, NE = 33, UGT = 34;
+    uint32_t e[NH], d[NH], off[NH];
+    uint64_t SUB = 0, DSUB = 0, SUBh[NH], DSUBh[NH], LENK[NH];
+#pragma unroll
+    for (int h = 0; h < NH; h++) e[h] = sh.ll[(uint32_t)ch[h] & ((1u << SH::LLR) - 1)];
+#pragma unroll
+    for (int h = 0; h < NH; h++) {
+        SUBh[h] = __builtin_amdgcn_uicmp(e[h] & 15, 0u, EQ) & __builtin_amdgcn_uicmp(e[h], 0xFFFFu, UGT);
+        SUB |= SUBh[h];
+    }
+    // codes longer than the root: one more LDS round trip, only when a lane needs it
+    if (SUB) {
+#pragma unroll
+        for (int h = 0; h < NH; h++)
+            if (lane_in(SUBh[h]))
+                e[h] = sh.ll[(e[h] >> 16) + ((uint32_t)(ch[h] >> SH::LLR) & ((1u << ((e[h] >> 4) & 15)) - 1))];
+    }
+#pragma unroll
+    for (int h = 0; h < NH; h++) {
+        off[h] = (e[h] & 15) + ((e[h] >> 4) & 15);
+        d[h] = sh.dl[(uint32_t)(ch[h] >> off[h]) & ((1u << SH::DR) - 1)];
+    }
+#pragma unroll
+    for (int h = 0; h < NH; h++) {
+        LENK[h] = __builtin_amdgcn_uicmp((e[h] >> 8) & 3, K_LEN, EQ);
+        DSUBh[h] = LENK[h] & __builtin_amdgcn_uicmp(d[h] & 15, 0u, EQ) & __builtin_amdgcn_uicmp(d[h], 0xFFFFu, UGT);
+        DSUB |= DSUBh[h];
+    }
+    if (DSUB) {
+#pragma unroll
+        for (int h = 0; h < NH; h++)
+            if (lane_in(DSUBh[h]))
+                d[h] = sh.dl[(d[h] >> 16) +
+                             ((uint32_t)(ch[h] >> (off[h] + SH::DR)) & ((1u << ((d[h] >> 4) & 15)) - 1))];
+    }
+#pragma unroll
+    for (int h = 0; h < NH; h++) {
+        const uint32_t L = e[h] & 15, kind = (e[h] >> 8) & 3, E = (e[h] >> 4) & 15;
+        const uint32_t len = (e[h] >> 16) + ((uint32_t)(ch[h] >> L) & ((1u << E) - 1));
+        const uint32_t DL = d[h] & 15, DE = (d[h] >> 4) & 15;
+        const uint32_t dist = (d[h] >> 16) + ((uint32_t)(ch[h] >> (off[h] + DL)) & ((1u << DE) - 1));
+        const uint64_t NZ = __builtin_amdgcn_uicmp(L, 0u, NE);
+        const uint64_t LITK = __builtin_amdgcn_uicmp(kind, K_LIT, EQ), EOBK = __builtin_amdgcn_uicmp(kind, K_EOB, EQ);
+        const uint64_t DOK = __builtin_amdgcn_uicmp(DL, 0u, NE) & __builtin_amdgcn_uicmp((d[h] >> 8) & 3, K_LEN, EQ);
+        const bool is_len = lane_in(LENK[h]);
+        CONT[h] = NZ & (LITK | (LENK[h] & DOK));
+        // selects, not branches: every lane computes every form
+        const uint32_t sym = is_len ? dist | len << 16 : SYM_LIT | (e[h] >> 16);
+        w[h] = lane_in(CONT[h]) ? sym : lane_in(NZ & EOBK) ? SYM_EOB : 0u;
+        nxt[h] = lane + (is_len ? off[h] + DL + DE : L);
+        EOBM[h] = NZ & EOBK;
+        LITM[h] = NZ & LITK;
+    }
+}
+
+// A round's lookahead: lane j decodes the symbols that would start at bits
+// P + j (half A) and P + 64 + j (half B).  A step is the lane of the
+// following symbol within the half (up to 111: past the half from 64 on),
+// plus STEP_STOP when that lane is inside the half and holds no complete
+// literal or copy (a walk stops there).  Lane classes travel in the packed
+// symbols and steps rather than in wave masks, which would hold twelve
+// scalar registers through the decode loop.
+constexpr uint32_t STEP_STOP = 128, STEP_LANE = 127;
+struct Look2 {
+    uint32_t wa, wb;  // packed symbols
+    uint32_t sa, sb;  // steps
 };
 
 template <typename SH>
-__device__ __attribute__((always_inline)) void lookahead(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
-                                                         uint64_t P, Lookahead &la) {
+__device__ __attribute__((always_inline)) void lookahead2(SH &sh, const uint8_t *__restrict__ in, uint64_t n,
+                                                          uint64_t P, Look2 &la) {
     const uint32_t lane = lane_id();
-    uint64_t W0, W1;
-    window128(in, n, P, W0, W1);
-    const uint64_t ch = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
-    // Straight-line per lane: the distance code is looked up on every lane
-    // (its result kept only where the code is a length), and the lane
-    // classes are wave masks from the compares themselves.
-    constexpr int EQ = 32, NE = 33, UGT = 34;
-    uint32_t e = sh.ll[(uint32_t)ch & ((1u << SH::LLR) - 1)];
-    // codes longer than the root: one more LDS round trip, only when a lane needs it
-    const uint64_t SUB = __builtin_amdgcn_uicmp(e & 15, 0u, EQ) & __builtin_amdgcn_uicmp(e, 0xFFFFu, UGT);
-    if (SUB && lane_in(SUB)) e = sh.ll[(e >> 16) + ((uint32_t)(ch >> SH::LLR) & ((1u << ((e >> 4) & 15)) - 1))];
-    const uint32_t L = e & 15, kind = (e >> 8) & 3, E = (e >> 4) & 15;
-    const uint32_t off = L + E;
-    const uint32_t len = (e >> 16) + ((uint32_t)(ch >> L) & ((1u << E) - 1));
-    uint32_t d = sh.dl[(uint32_t)(ch >> off) & ((1u << SH::DR) - 1)];
-    const uint64_t LENK = __builtin_amdgcn_uicmp(kind, K_LEN, EQ);
-    const uint64_t DSUB = LENK & __builtin_amdgcn_uicmp(d & 15, 0u, EQ) & __builtin_amdgcn_uicmp(d, 0xFFFFu, UGT);
-    if (DSUB && lane_in(DSUB))
-        d = sh.dl[(d >> 16) + ((uint32_t)(ch >> (off + SH::DR)) & ((1u << ((d >> 4) & 15)) - 1))];
-    const uint32_t DL = d & 15, DE = (d >> 4) & 15;
-    const uint32_t dist = (d >> 16) + ((uint32_t)(ch >> (off + DL)) & ((1u << DE) - 1));
-    const uint64_t NZ = __builtin_amdgcn_uicmp(L, 0u, NE);
-    const uint64_t LITK = __builtin_amdgcn_uicmp(kind, K_LIT, EQ), EOBK = __builtin_amdgcn_uicmp(kind, K_EOB, EQ);
-    const uint64_t DOK = __builtin_amdgcn_uicmp(DL, 0u, NE) & __builtin_amdgcn_uicmp((d >> 8) & 3, K_LEN, EQ);
-    const bool is_len = lane_in(LENK);
-    la.e = e;
-    la.olen = is_len ? len : lane_in(LITK) ? 1u : 0u;
-    la.dist = is_len ? dist : 0u;
-    la.nxt = lane + (is_len ? off + DL + DE : L);  // <= 63 + 48
-    la.CONT = NZ & (LITK | (LENK & DOK));
-    la.EOBM = NZ & EOBK;
-    la.LITM = NZ & LITK;
-    const uint32_t nx = la.nxt;
-    const bool go_on = nx < 64 && ((la.CONT >> (nx & 63)) & 1);
-    la.step = nx | (go_on ? 0u : 0x100u);
+    uint64_t W0, W1, W2;
+    window192(in, n, P, W0, W1, W2);
+    const uint64_t cha = lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0;
+    const uint64_t chb = lane ? (W1 >> lane) | (W2 << (64 - lane)) : W1;
+    const uint64_t ch[2] = {cha, chb};
+    uint32_t w[2], x[2];
+    uint64_t C[2], E[2], L[2];
+    lane_decode<2>(sh, ch, lane, w, x, C, E, L);
+    la.wa = w[0];
+    la.wb = w[1];
+    const bool stop_a = x[0] < 64 && !((C[0] >> (x[0] & 63)) & 1);
+    const bool stop_b = x[1] < 64 && !((C[1] >> (x[1] & 63)) & 1);
+    la.sa = x[0] + (stop_a ? STEP_STOP : 0u);
+    la.sb = x[1] + (stop_b ? STEP_STOP : 0u);
 }
 
 // ---- a symbol boundary inside a block, from any bit ---------------------
@@ -867,9 +965,15 @@ __device__ __attribute__((noinline)) uint64_t sync_point(SH &sh, const uint8_t *
     uint64_t A = (1ull << 48) - 1;  // pending path positions, relative to P
     for (uint32_t w = 0; w < SYNC_WINDOWS; w++) {
         if (P + 128 > uni64(limit_bit)) return NO_SYNC;
-        Lookahead la;
-        lookahead(sh, in, n, P, la);
-        const uint64_t CONT = uni64(la.CONT);
+        uint64_t W0, W1;
+        window128(in, n, P, W0, W1);
+        const uint32_t lane = lane_id();
+        const uint64_t ch[1] = {lane ? (W0 >> lane) | (W1 << (64 - lane)) : W0};
+        uint32_t sw[1], nx[1];
+        uint64_t CM[1], EM[1], LM[1];
+        lane_decode<1>(sh, ch, lane, sw, nx, CM, EM, LM);
+        const uint32_t snx = nx[0];
+        const uint64_t CONT = uni64(CM[0]);
         uint64_t C = 0;  // pending positions in the next window
         while (A) {
             A = uni64(A);
@@ -882,7 +986,7 @@ __device__ __attribute__((noinline)) uint64_t sync_point(SH &sh, const uint8_t *
             // the true one only where the block ends, and then the
             // stopping piece does not land on S
             if (!((CONT >> i) & 1)) continue;
-            const uint32_t t = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)i));
+            const uint32_t t = uni((uint32_t)__builtin_amdgcn_readlane((int)snx, (int)i));
             if (t < 64) A |= 1ull << t;
             else C |= 1ull << (t - 64);
         }
@@ -894,17 +998,22 @@ __device__ __attribute__((noinline)) uint64_t sync_point(SH &sh, const uint8_t *
 }
 
 // Decode the Huffman symbols of the current block until end-of-block, from
-// bit P.  Multi-symbol rounds: lane j looks up the litlen (and, for a length
-// code, the distance) LUT entry of a symbol starting at bit P + j, all 64 in
-// one LDS round trip, and knows where the next symbol would start
-// (nxt = j + code bits).  A scalar walk from lane 0 follows nxt through the
-// lanes that hold complete codes (about six per 64 input bits here), a wave
-// prefix sum gives every taken symbol its output offset, the round's
-// literals go out in one LDS write and one store, and its copies run in
-// order.  The next round's input window and LUT lookups are issued before
+// bit P.  Multi-symbol rounds over 128 input bits: lane j looks up the litlen
+// (and, for a length code, the distance) LUT entries of the symbols starting
+// at bits P + j and P + 64 + j, both halves in one LDS round trip per LUT
+// level, and knows where the next symbol would start (j + code bits).  A
+// scalar walk from lane 0 of the first half follows those steps through the
+// lanes that hold complete codes and on into the second half (about twelve
+// symbols per round on text: a round is bound by its input bits), a wave
+// prefix sum per half gives every taken symbol its output offset, the round's
+// literals go out in one LDS write and one store per half, and its copies run
+// in order.  The next round's input window and LUT lookups are issued before
 // this round's stores and copies, so their latency overlaps the copies.  A
 // round that can take nothing (a code longer than the LUT root, an invalid
-// code, a full output) is left to scalar_step.
+// code, a full output) is left to scalar_step.  Against 64-bit rounds the
+// walk, the prefix sums and the round's bookkeeping run half as often: on the
+// 1 GiB text member 2.52M rounds instead of 4.66M, kernel 3.36 -> 3.23 ms;
+// on u32 deltas (bit-bound) 5.87 -> 4.61 ms per 256 MiB.
 // ---- balance across the launch: issue priority from progress -------------
 // All pieces of a launch are planned to take equal time and, when they fit
 // the resident wavefronts, start together,
@@ -915,10 +1024,8 @@ __device__ __attribute__((noinline)) uint64_t sync_point(SH &sh, const uint8_t *
 // its own piece's progress is behind the launch's: the lagging wavefronts
 // issue first.  A scheduling hint only; the output does not depend on it.
 struct Prog {
-    uint64_t *ctr;            // launch-wide bits consumed (null: off)
-    float inv_total, inv_span;
-    uint64_t start, last;     // piece's first bit, position at the last report
-    uint32_t k;
+    uint32_t on;  // balance on (the state is in ProgState)
+    uint32_t k;   // rounds
 };
 // Measured on a 1 GiB text member (3969 pieces): a report every 64 rounds
 // and priority bands 2% of the piece wide took the kernel 3.76 -> 3.61 ms;
@@ -927,14 +1034,13 @@ struct Prog {
 constexpr uint32_t PROG_EVERY = 64;
 constexpr float PROG_BAND = 0.02f;
 
-// (by value: a reference would keep the caller's Prog in scratch memory)
-__device__ __attribute__((noinline)) void progress_report(uint64_t *ctr, float inv_total, float inv_span,
-                                                         uint64_t start, uint64_t last, uint64_t P) {
-    const uint64_t d = P - last;
+__device__ __attribute__((noinline)) void progress_report(lds_prog *ps, uint64_t P) {
+    const uint64_t d = P - uni64(ps->last);
     uint64_t G = 0;
-    if (lane_id() == 0) G = atomicAdd((unsigned long long *)ctr, (unsigned long long)d) + d;
+    if (lane_id() == 0) G = atomicAdd((unsigned long long *)ps->ctr, (unsigned long long)d) + d;
     G = uni64(G);
-    const float behind = (float)G * inv_total - (float)(P - start) * inv_span;
+    const float behind = (float)G * ps->inv_total - (float)(P - uni64(ps->start)) * ps->inv_span;
+    ps->last = P;
     if (behind > PROG_BAND) __builtin_amdgcn_s_setprio(3);
     else if (behind > 0.0f) __builtin_amdgcn_s_setprio(2);
     else if (behind > -PROG_BAND) __builtin_amdgcn_s_setprio(1);
@@ -967,16 +1073,13 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
     // nothing loaded before (stored blocks, the header reader) is still in
     // flight when the loop starts; see consumed()
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unchanged
-    Lookahead la;
-    lookahead(sh, in, n, P, la);
+    Look2 la;
+    lookahead2(sh, in, n, P, la);
     for (;;) {
         P = uni64(P);
         pos = uni(pos);
         o.p = (gptr<uint8_t>)uni64((uint64_t)o.p);  // stores then take the base from SGPRs
-        if (pg.ctr && (++pg.k & (PROG_EVERY - 1)) == 0) {
-            progress_report(pg.ctr, pg.inv_total, pg.inv_span, pg.start, pg.last, P);
-            pg.last = P;
-        }
+        if (pg.on && (++pg.k & (PROG_EVERY - 1)) == 0) progress_report((lds_prog *)&sh.pg, P);
         last_marker_end = uni64(last_marker_end);
         stop_at = uni64(stop_at);
         // 64-bit orderings as the sign of a difference (bit positions are
@@ -993,44 +1096,83 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             r = R_ERROR;
             break;
         }
-        const uint64_t CONT = uni64(la.CONT), EOBM = uni64(la.EOBM), LITM = uni64(la.LITM);
-        // ---- walk the chain of symbol starts from lane 0: each lane's
-        // `step` names the next start and whether the walk continues there
-        uint64_t M = 0;
+        // ---- walk the chain of symbol starts from bit P through half A, and
+        // on into half B where it leaves A: each lane's step names the next
+        // start.  s ends as the bit (relative to P) after the round's last
+        // symbol; an end-of-block symbol ends the walk and the block.
+        uint64_t MA = 0, MB = 0, XA = 0, XB = 0;  // symbol starts taken; X*: the end-of-block one
         uint32_t s = 0, eob = 0;
-        if (CONT & 1) {
-            for (;;) {
-                M |= 1ull << s;
-                const uint32_t v = uni((uint32_t)__builtin_amdgcn_readlane((int)la.step, (int)s));
-                s = v & 0xFF;
-                if (v & 0x100) break;
+        if (uni((uint32_t)__builtin_amdgcn_readlane((int)la.wa, 0)) > SYM_EOB) {
+            do {
+                set_bit64(MA, s);
+                s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.sa, (int)s));
+            } while (s < 64);
+        } else {
+            s = STEP_STOP;
+        }
+        if (s >= STEP_STOP) {
+            // stopped at lane q of half A: end-of-block, or a code the round
+            // cannot take
+            const uint32_t q = s - STEP_STOP;
+            s = q;
+            if (uni((uint32_t)__builtin_amdgcn_readlane((int)la.wa, (int)q)) == SYM_EOB) {
+                XA = 1ull << q;
+                s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.sa, (int)q)) & STEP_LANE;
+                eob = 1;
             }
+        } else if (s < 128) {
+            uint32_t t = s - 64;
+            if (uni((uint32_t)__builtin_amdgcn_readlane((int)la.wb, (int)t)) > SYM_EOB) {
+                do {
+                    set_bit64(MB, t);
+                    t = uni((uint32_t)__builtin_amdgcn_readlane((int)la.sb, (int)t));
+                } while (t < 64);
+            } else {
+                t += STEP_STOP;
+            }
+            if (t >= STEP_STOP) {
+                const uint32_t q = t - STEP_STOP;
+                t = q;
+                if (uni((uint32_t)__builtin_amdgcn_readlane((int)la.wb, (int)q)) == SYM_EOB) {
+                    XB = 1ull << q;
+                    t = uni((uint32_t)__builtin_amdgcn_readlane((int)la.sb, (int)q)) & STEP_LANE;
+                    eob = 1;
+                }
+            }
+            s = 64 + t;
         }
-        if (s < 64 && ((EOBM >> s) & 1)) {
-            M |= 1ull << s;
-            s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)s));
-            eob = 1;
-        }
-        M = uni64(M);
+        MA = uni64(MA | XA);
+        MB = uni64(MB | XB);
         // ---- a mid-block stop inside this round: keep the symbols before it
         // when one of them ends exactly there, else the stream does not pass
         // through it and it is dropped
         if (stop_at != NO_SYNC && stop_at - P < (uint64_t)s) {
             const uint32_t rel = uni((uint32_t)(stop_at - P));
-            if (rel < 64 && ((M >> rel) & 1)) {
-                M = uni64(M & ((1ull << rel) - 1));
+            const bool at = rel < 64 ? ((MA >> rel) & 1) != 0 : rel < 128 && ((MB >> (rel - 64)) & 1) != 0;
+            if (at) {
+                if (rel < 64) {
+                    MA &= (1ull << rel) - 1;
+                    MB = 0;
+                } else {
+                    MB &= (1ull << (rel - 64)) - 1;
+                }
                 s = rel;
                 eob = 0;
             } else {
                 stop_at = NO_SYNC;
             }
         }
-        // ---- output offsets of the taken symbols
-        const uint32_t v = lane_in(M) ? la.olen : 0u;
-        const uint32_t incl = wave_scan_incl(v);
-        const uint32_t opos = incl - v;
-        uint32_t run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, 63));
-        if (M == 0 || run > cap - pos) {
+        // ---- output offsets of the taken symbols, half A then half B
+        const uint32_t wa = la.wa, wb = la.wb;
+        uint64_t TA = uni64(MA & ~XA), TB = uni64(MB & ~XB);  // literals and copies
+        const uint32_t va = lane_in(TA) ? sym_len(wa) : 0u;
+        const uint32_t vb = lane_in(TB) ? sym_len(wb) : 0u;
+        const uint32_t ia = wave_scan_incl(va);
+        const uint32_t ib0 = wave_scan_incl(vb);
+        const uint32_t runa = uni((uint32_t)__builtin_amdgcn_readlane((int)ia, 63));
+        const uint32_t ib = ib0 + runa;
+        uint32_t run = runa + uni((uint32_t)__builtin_amdgcn_readlane((int)ib0, 63));
+        if ((MA | MB) == 0 || run > cap - pos) {
             // nothing the round can take: one symbol the slow way, which
             // either advances P or ends the block with a status
             const uint32_t sr = scalar_step<RT>(sh, in, n, P, o, pos, cap, g0, hist0, last_marker_end, err);
@@ -1039,7 +1181,7 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
                 break;
             }
             P = uni64(P);
-            lookahead(sh, in, n, P, la);
+            lookahead2(sh, in, n, P, la);
         } else {
             // The round writes its literals to the ring before its copies, so
             // it is cut short where that order would matter:
@@ -1054,66 +1196,98 @@ __device__ __attribute__((always_inline)) uint32_t decode_symbols(SH &sh, const 
             //    round's own output, not yet in HBM: the round ends before
             //    it.
             // None of these can cut before the round's first symbol (lengths
-            // are at most 258, the ring holds 4096 or more entries).
+            // are at most 258, the ring holds 2048 or more entries).
             // A round no longer than the ring can only meet the first: the
             // second needs a symbol ending past the ring, the third a far
             // copy (distance > ring) whose source lies inside the round.
-            const uint64_t COPY = M & ~LITM & ~EOBM;
-            uint64_t BAD, BADF = 0, OVER = 0;
+            const uint64_t LITA = ballot(sym_is_lit(wa)), LITB = ballot(sym_is_lit(wb));
+            const uint64_t CPA = TA & ~LITA, CPB = TB & ~LITB;
+            const uint32_t oa = ia - va, ob = ib - vb;
+            const uint32_t da = wa & 0xFFFFu, db = wb & 0xFFFFu;
+            uint64_t BADA, BADB, BEFA = 0, BEFB = 0;  // BEF*: the round ends before the symbol
             if (run <= RING<RT>) {
-                BAD = COPY & __builtin_amdgcn_uicmp(la.dist, RING<RT>, 37 /* unsigned <= */) &
-                      __builtin_amdgcn_uicmp(la.dist + run - opos, RING<RT>, 34 /* unsigned > */);
+                BADA = CPA & __builtin_amdgcn_uicmp(da, RING<RT>, 37 /* unsigned <= */) &
+                       __builtin_amdgcn_uicmp(da + run - oa, RING<RT>, 34 /* unsigned > */);
+                BADB = CPB & __builtin_amdgcn_uicmp(db, RING<RT>, 37) & __builtin_amdgcn_uicmp(db + run - ob, RING<RT>, 34);
             } else {
-                const bool inM = lane_in(M);
-                const bool is_copy = lane_in(COPY);
-                const bool far = la.dist > RING<RT>;
-                BAD = ballot(is_copy && !far && la.dist + run - opos > RING<RT>);
-                BADF = ballot(lane != 0 && is_copy && far && la.dist < opos + la.olen);
-                OVER = ballot(lane != 0 && inM && incl > RING<RT>);
+                const bool ca = lane_in(CPA), cb = lane_in(CPB);
+                const bool fa = da > RING<RT>, fb = db > RING<RT>;
+                BADA = ballot(ca && !fa && da + run - oa > RING<RT>);
+                BADB = ballot(cb && !fb && db + run - ob > RING<RT>);
+                // the round's first symbol is lane 0 of half A
+                BEFA = ballot(lane != 0 && ((ca && fa && da < ia) || (lane_in(TA) && ia > RING<RT>)));
+                BEFB = ballot((cb && fb && db < ib) || (lane_in(TB) && ib > RING<RT>));
             }
-            if (BAD | BADF | OVER) {  // rare
-                uint32_t cut = 64;  // keep the symbols at lanes below
-                if (BAD) cut = uni((uint32_t)__builtin_ctzll(BAD)) + 1;
-                if (BADF) {
-                    const uint32_t ff = uni((uint32_t)__builtin_ctzll(BADF));
-                    cut = ff < cut ? ff : cut;
+            if (BADA | BADB | BEFA | BEFB) {  // rare
+                // keep the symbols that start before position kp (0..127)
+                uint32_t kp = 128;
+                if (BADA) kp = uni((uint32_t)__builtin_ctzll(BADA)) + 1;
+                else if (BADB) kp = 64 + uni((uint32_t)__builtin_ctzll(BADB)) + 1;
+                if (BEFA) {
+                    const uint32_t q = uni((uint32_t)__builtin_ctzll(BEFA));
+                    kp = q < kp ? q : kp;
+                } else if (BEFB) {
+                    const uint32_t q = 64 + uni((uint32_t)__builtin_ctzll(BEFB));
+                    kp = q < kp ? q : kp;
                 }
-                if (OVER) {
-                    const uint32_t fo = uni((uint32_t)__builtin_ctzll(OVER));
-                    cut = fo < cut ? fo : cut;
-                }
-                if (cut < 64) {
-                    M &= (1ull << cut) - 1;
-                    const uint32_t last = uni((uint32_t)(63 - __builtin_clzll(M)));
-                    run = uni((uint32_t)__builtin_amdgcn_readlane((int)incl, (int)last));
-                    s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.nxt, (int)last));
-                    eob = ((EOBM >> last) & 1) ? eob : 0u;
+                const uint64_t nA = kp >= 64 ? MA : MA & ((1ull << kp) - 1);
+                const uint64_t nB = kp <= 64 ? 0ull : kp >= 128 ? MB : MB & ((1ull << (kp - 64)) - 1);
+                if (nA != MA || nB != MB) {
+                    MA = uni64(nA);
+                    MB = uni64(nB);
+                    if (MB) {
+                        const uint32_t last = uni((uint32_t)(63 - __builtin_clzll(MB)));
+                        run = uni((uint32_t)__builtin_amdgcn_readlane((int)ib, (int)last));
+                        s = 64 + (uni((uint32_t)__builtin_amdgcn_readlane((int)la.sb, (int)last)) & STEP_LANE);
+                        TB = uni64(MB);
+                    } else {
+                        const uint32_t last = uni((uint32_t)(63 - __builtin_clzll(MA)));
+                        run = uni((uint32_t)__builtin_amdgcn_readlane((int)ia, (int)last));
+                        s = uni((uint32_t)__builtin_amdgcn_readlane((int)la.sa, (int)last)) & STEP_LANE;
+                        TB = 0;
+                    }
+                    TA = uni64(MA);
+                    eob = 0;  // the cut is before the end-of-block symbol
                 }
             }
             eob = uni(eob);
-            const uint64_t lits = uni64(M & LITM), matches = uni64(M & ~LITM & ~EOBM);
-            // this round's symbols, kept while the next round is looked up
-            const uint32_t c_e = la.e, c_dist = la.dist, c_olen = la.olen;
+            const uint64_t lita = uni64(TA & LITA), litb = uni64(TB & LITB);
+            uint64_t cpa = uni64(TA & ~LITA), cpb = uni64(TB & ~LITB);
+            const bool fused = run <= WAVE && (cpa | cpb);
             P += s;
-            if (!eob) lookahead(sh, in, n, P, la);
-            if (run <= WAVE && matches) {
-                fused_round<RT>(sh, o, pos, g0, run, lane_in(M), lane_in(lits), opos, c_e, c_dist, c_olen,
-                                last_marker_end);
+            if (!eob) lookahead2(sh, in, n, P, la);
+            if (fused) {
+                fused_round<RT>(sh, o, pos, g0, run, lane_in(TA), wa, oa, lane_in(TB), wb, ob, last_marker_end);
             } else {
-                // ---- literals of the round: one LDS write and one HBM store
-                if (lane_in(lits)) {
-                    ring_at<RT>(sh.ring, (g0 + pos) * (uint32_t)sizeof(RT) + opos * (uint32_t)sizeof(RT)) =
-                        (RT)(c_e >> 16);
-                    put_e<RT>(o, pos, opos, c_e >> 16);
+                // ---- literals of the round: one LDS write and one HBM store per half
+                if (lane_in(lita)) {
+                    ring_at<RT>(sh.ring, (g0 + pos) * (uint32_t)sizeof(RT) + oa * (uint32_t)sizeof(RT)) =
+                        (RT)(wa & 0xFFu);
+                    put_e<RT>(o, pos, oa, wa & 0xFFu);
+                }
+                if (lane_in(litb)) {
+                    ring_at<RT>(sh.ring, (g0 + pos) * (uint32_t)sizeof(RT) + ob * (uint32_t)sizeof(RT)) =
+                        (RT)(wb & 0xFFu);
+                    put_e<RT>(o, pos, ob, wb & 0xFFu);
                 }
                 compiler_fence();
                 // ---- copies, in stream order
-                for (uint64_t mm = matches; mm; mm = uni64(mm & (mm - 1))) {
-                    const uint32_t mi = uni((uint32_t)__builtin_ctzll(mm));
-                    const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)opos, (int)mi);
-                    const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)c_dist, (int)mi);
-                    const uint32_t ln = (uint32_t)__builtin_amdgcn_readlane((int)c_olen, (int)mi);
-                    wave_copy<RT>(sh, o, pos + m, g0, ln, d, last_marker_end);
+                while (cpa | cpb) {
+                    uint32_t m, w;
+                    if (cpa) {
+                        const uint32_t mi = uni((uint32_t)__builtin_ctzll(cpa));
+                        cpa = uni64(cpa & (cpa - 1));
+                        m = (uint32_t)__builtin_amdgcn_readlane((int)oa, (int)mi);
+                        w = (uint32_t)__builtin_amdgcn_readlane((int)wa, (int)mi);
+                    } else {
+                        const uint32_t mi = uni((uint32_t)__builtin_ctzll(cpb));
+                        cpb = uni64(cpb & (cpb - 1));
+                        m = (uint32_t)__builtin_amdgcn_readlane((int)ob, (int)mi);
+                        w = (uint32_t)__builtin_amdgcn_readlane((int)wb, (int)mi);
+                    }
+                    m = uni(m);
+                    w = uni(w);
+                    wave_copy<RT>(sh, o, pos + m, g0, w >> 16, w & 0xFFFFu, last_marker_end);
                 }
             }
             pos += run;
@@ -1227,15 +1401,16 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, L
     uint64_t cp_bit = NO_CP, cp_out = 0;
     uint32_t fixed_built = 0;  // the fixed tables are in the LUTs
     Prog pg;
-    pg.ctr = prog;
-    pg.inv_total = prog_inv_total;
-    {
+    pg.on = prog != nullptr;
+    pg.k = 0;
+    if (lane == 0) {
         const uint64_t end = pd.stop_bit < pd.limit_bit ? pd.stop_bit : pd.limit_bit;
         const uint64_t span = end > pd.start_bit ? end - pd.start_bit : 1;
-        pg.inv_span = 1.0f / (float)span;
+        sh.pg.ctr = prog;
+        sh.pg.inv_total = prog_inv_total;
+        sh.pg.inv_span = 1.0f / (float)span;
+        sh.pg.start = sh.pg.last = pd.start_bit;
     }
-    pg.start = pg.last = pd.start_bit;
-    pg.k = 0;
 
     // ring = the last RING entries of the window before the piece (piece
     // position a at slot a mod RING)
