@@ -607,23 +607,22 @@ constexpr int SPEC_FALLBACK_ROUND = 4, SPEC_MAX_ROUNDS = 16;
 constexpr uint64_t ESTIMATE_MIN_PIECES = 64;
 
 // Decode cost of a member in output-entry units.  A wavefront's decode time
-// is about max(a per compressed bit, b per output entry) at full occupancy:
-// a decode round covers at most 128 input bits, and a round's copies take
-// time in proportion to their bytes.  Measured per piece with 64-bit rounds:
-// text (91k bits, 300k entries) 3.3 ms, long-copy data (247k bits, 1.77M
-// entries) 19.6 ms, u32 deltas (707k bits, 263k entries) 31 ms, so a
-// Huffman-coded member costs max(4 x bits, ISIZE).  With 128-bit rounds the
-// bit term fell by about a fifth; weights of 3 and 2.5 per bit planned the
-// 10,000-member batch 5-10% slower and left the mixed-content member
-// unchanged, so the weight stays 4.  Stored data (ratio below 1.15, the
-// reference's StoredParallel cut, decompress/mod.rs:143-151) moves at copy
-// speed, ~830 MB/s for a lone wavefront against ~230 MB/s of entries from
-// long copies, and is weighted 1/4 of its bytes.  ISIZE is the size mod 2^32; a
-// member whose true size wrapped looks stored-like here, which only changes
-// how it is split.
+// at full occupancy is about a per compressed bit plus b per output entry:
+// a round covers at most 128 input bits, and a round's copies take time in
+// proportion to their bytes.  Fitted on per-piece kernel times of the
+// inflate launch (1 GiB text member: 83k bits, 270k entries, 3.23 ms;
+// 256 MiB of u32 deltas: 182k bits, 68k entries, 4.58 ms): a = 23 ns,
+// b = 4.7 ns, so a Huffman-coded member costs 5 x bits + ISIZE.  Stored
+// data (ratio below 1.15, the reference's StoredParallel cut,
+// decompress/mod.rs:143-151) moves at copy speed, ~440 MB/s for a wavefront
+// among fifteen others, and is weighted 1/16 per bit.  (The earlier
+// max(4 x bits, ISIZE) with stored data at 1/32 per bit, fitted on 64-bit
+// rounds, planned the hipGraph batch 7% slower.)  ISIZE is the size mod 2^32;
+// a member whose true size wrapped looks stored-like here, which only
+// changes how it is split.
 uint64_t member_cost(uint64_t cbits, uint32_t exp_isize) {
     const bool stored_like = (uint64_t)exp_isize * 100 < (cbits / 8) * 115;
-    return std::max<uint64_t>(1, stored_like ? cbits / 32 : std::max<uint64_t>(4 * cbits, exp_isize));
+    return std::max<uint64_t>(1, stored_like ? cbits / 16 : 5 * cbits + exp_isize);
 }
 
 // Entries per compressed bit for each 12-bit output estimate of the block
@@ -1017,9 +1016,19 @@ std::vector<std::vector<std::pair<uint64_t, uint64_t>>> stored_payloads(gzgpu_ct
         std::vector<uint64_t> pos(nc);  // candidate byte offsets, ascending
         for (uint32_t i = 0; i < nc; i++) pos[i] = keys[ord[i]];
         std::vector<uint8_t> seen(nc, 0);
-        auto find = [&](uint64_t q) -> int64_t {
-            const auto it = std::lower_bound(pos.begin(), pos.end(), q);
-            return it != pos.end() && *it == q ? (int64_t)(it - pos.begin()) : -1;
+        // candidate at byte q, searched forward from index `from` (a chain's
+        // next header follows the current one, usually as the very next
+        // candidate) in doubling steps, then by bisection inside the last step
+        auto find = [&](size_t from, uint64_t q) -> int64_t {
+            size_t lo = from, hi = from, step = 1;
+            while (hi < nc && pos[hi] < q) {
+                lo = hi + 1;
+                hi += step;
+                step <<= 1;
+            }
+            hi = std::min<size_t>(hi, nc);  // every candidate before lo is below q, pos[hi] is not
+            const size_t at = (size_t)(std::lower_bound(pos.begin() + lo, pos.begin() + hi, q) - pos.begin());
+            return at < nc && pos[at] == q ? (int64_t)at : -1;
         };
         for (uint32_t m : ms) {
             const Member &M = mem[m];
@@ -1036,7 +1045,7 @@ std::vector<std::vector<std::pair<uint64_t, uint64_t>>> stored_payloads(gzgpu_ct
                     const uint32_t len = (uint32_t)(v >> 40) & 0xFFFF;
                     chain.push_back({pos[j], len});
                     if (v >> 56) break;  // BFINAL
-                    j = find(pos[j] + 5 + len);
+                    j = find((size_t)j + 1, pos[j] + 5 + len);
                 }
                 if (chain.size() < STORED_RUN_MIN) continue;
                 for (auto &b : chain) {
@@ -1187,16 +1196,38 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             bool mid;
         };
         std::vector<Start> chosen;
+        // The first hit at or after `bit`, searched forward from `from` (the
+        // previous answer of the same pass: the targets mostly increase) in
+        // doubling steps, then by bisection inside the last step; a target
+        // before hits[from] takes a full search.  (A full binary search per
+        // stored hit cost ~1 ms per member with 8k stored blocks among 20k
+        // hits: cache misses over the whole hit array each time.)
+        auto hit_seek = [&](size_t from, uint64_t bit) -> size_t {
+            const size_t N = hits.size();
+            from = std::min(from, N);
+            if (from > 0 && hits[from - 1].bit_offset >= bit)
+                return (size_t)(std::lower_bound(hits.begin(), hits.end(), bit, hit_lt) - hits.begin());
+            size_t lo = from, hi = from, step = 1;
+            while (hi < N && hits[hi].bit_offset < bit) {
+                lo = hi + 1;
+                hi += step;
+                step <<= 1;
+            }
+            hi = std::min(hi, N);
+            return (size_t)(std::lower_bound(hits.begin() + lo, hits.begin() + hi, bit, hit_lt) - hits.begin());
+        };
         // chained[i]: stored hit i's payload ends where another hit's
         // accepted range begins, or at the member's end
         chain_ok.assign(hits.size(), 0);
+        size_t cur_chain = 0;
         for (size_t i = 0; i < hits.size(); i++) {
             if (hits[i].kind != 0) continue;
             const uint64_t o = hits[i].seek_bit + 3;  // the LEN field
             const uint64_t next = (o / 8 + 4 + (hits[i].info >> 16)) * 8;
             bool ok = next == M.de;
-            auto k = std::lower_bound(hits.begin(), hits.end(), next > 16 ? next - 16 : 0, hit_lt);
-            for (; !ok && k != hits.end() && k->bit_offset <= next + 1; ++k) ok = k->seek_bit >= next;
+            cur_chain = hit_seek(cur_chain, next > 16 ? next - 16 : 0);
+            for (size_t k = cur_chain; !ok && k < hits.size() && hits[k].bit_offset <= next + 1; ++k)
+                ok = hits[k].seek_bit >= next;
             chain_ok[i] = ok;
         }
         auto chained = [&](size_t i) { return chain_ok[i] != 0; };
@@ -1205,13 +1236,14 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         // .zip, .png): every piece started there would miss.  They are
         // shadowed for planning.
         shadow.assign(hits.size(), 0);
+        size_t cur_pay = 0;
         for (size_t i = 0; i < hits.size(); i++) {
             if (hits[i].kind != 0 || !chained(i)) continue;
             const uint64_t o = hits[i].seek_bit + 3;
             const uint64_t pay = (o / 8 + 4) * 8, next = (o / 8 + 4 + (hits[i].info >> 16)) * 8;
-            for (auto k = std::lower_bound(hits.begin(), hits.end(), pay, hit_lt); k != hits.end() && k->bit_offset < next;
-                 ++k)
-                if (k->kind == 2) shadow[(size_t)(k - hits.begin())] = 1;
+            cur_pay = hit_seek(cur_pay, pay);
+            for (size_t k = cur_pay; k < hits.size() && hits[k].bit_offset < next; ++k)
+                if (hits[k].kind == 2) shadow[k] = 1;
         }
         auto dyn_hit = [&](size_t j) { return hits[j].kind == 2 && !shadow[j]; };
         // a BFINAL = 1 header: a block to start inside of, never a piece start
@@ -1222,10 +1254,14 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
         // a stored block's output is its LEN; a dynamic block's is estimated
         // from its code lengths (entries per bit, from the block finder) and
         // the estimates are scaled so that the member's total matches its
-        // trailer ISIZE.  Cost per bit as in member_cost: max(4, entries per
-        // bit) for Huffman blocks, 1/32 for stored ones.  Mixed content
-        // (stored runs between text and u32 stretches) then gets pieces of
-        // equal decode time, not of equal compressed size.
+        // trailer ISIZE.  Cost per bit as in member_cost: 5 + entries per bit
+        // for Huffman blocks, 1/16 for stored ones.  Mixed content (stored
+        // runs between text and u32 stretches) then gets pieces of equal
+        // decode time, not of equal compressed size.  On the 2 GiB mixed
+        // member (against max(4, entries per bit) and 1/32) the inflate
+        // launch went 17.2 -> 14.9 ms at L1 and 14.5 -> 12.2 at L9; a
+        // constant of 4 or 6, or stored weights of 1/32, 1/12 or 1/8, were
+        // slower on one level or both.
         // (the first stretch, from the member start to its first real hit
         // or at it, takes that hit's kind: a member's first block is almost
         // always of the kind of its second)
@@ -1279,7 +1315,7 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             scost.resize(sent.size());
             for (size_t j = 0; j + 1 < sx.size(); j++) {
                 if (!sstored[j]) sent[j] *= scale;
-                scost[j] = sstored[j] ? 1.0 / 32.0 : std::max(4.0, sent[j]);
+                scost[j] = sstored[j] ? 1.0 / 16.0 : 5.0 + sent[j];
             }
         }
         const double t_str = debug_level() >= 2 ? host_ms() : 0;
