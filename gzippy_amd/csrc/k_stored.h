@@ -23,34 +23,46 @@
 namespace gz {
 
 // candidate: byte offset (bits 0-39), LEN (40-55), BFINAL (56)
+//
+// Each thread tests SCAN_B consecutive header positions per step from six
+// aligned words.  LEN == ~NLEN (the 32 bits after the header byte, one
+// v_alignbyte, a shift and an xor per position) is the rare condition and is
+// tested first; the header bits, the bounds and the append run only for the
+// positions that pass it (a few per MiB besides the true headers), so the
+// scan is bound by its loads rather than by a ballot per byte.
+constexpr uint32_t SCAN_B = 16;
 __global__ void __launch_bounds__(256) k_stored_scan(const uint8_t *__restrict__ in, uint64_t n, uint64_t first,
                                                      uint64_t last, unsigned long long *__restrict__ cand,
                                                      uint32_t cap, uint32_t *count) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 4;
-    for (uint64_t base = first + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-         base < last; base += stride) {
-        // bytes base .. base + 7 cover the headers at base .. base + 3
+    const uint64_t a0 = first & ~(uint64_t)3;  // word-aligned start
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * SCAN_B;
+    for (uint64_t base = a0 + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * SCAN_B; base < last;
+         base += stride) {
+        // bytes base .. base + 23 cover the headers at base .. base + 15 and
+        // their LEN/NLEN
         const uint64_t w = base >> 2;
-        const uint32_t sh = (uint32_t)(base & 3) * 8;
-        const uint32_t a = load_word(in, n, w), b = load_word(in, n, w + 1), c2 = load_word(in, n, w + 2);
-        const uint64_t lo = ((uint64_t)b << 32 | a) >> sh | (sh ? (uint64_t)c2 << (64 - sh) : 0ull);
+        uint32_t x[SCAN_B / 4 + 2];
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint64_t q = base + k;
-            const uint64_t v = lo >> (8 * k);  // bytes q .. q + 4 in the low 40 bits (k <= 3)
-            const uint32_t hdr = (uint32_t)v & 0xFF;
-            const uint32_t len = (uint32_t)(v >> 8) & 0xFFFF, nlen = (uint32_t)(v >> 24) & 0xFFFF;
-            const bool ok = q < last && ((hdr >> 1) & 3) == 0 && (len ^ 0xFFFFu) == nlen && q + 5 + len <= last;
-            uint32_t slot;
-            const uint64_t m = ballot(ok);
-            uint32_t wbase = 0;
-            if (m) {
-                if (lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1))
-                    wbase = atomicAdd(count, (uint32_t)__popcll(m));
-                wbase = __shfl(wbase, __ffsll((unsigned long long)m) - 1);
+        for (uint32_t i = 0; i < SCAN_B / 4 + 2; i++) x[i] = load_word(in, n, w + i);
+        uint32_t m = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < SCAN_B; k++) {
+            const uint32_t j = k + 1;  // byte of LEN's low half, from base
+            const uint32_t t = (j & 3) ? __builtin_amdgcn_alignbyte(x[j / 4 + 1], x[j / 4], j & 3) : x[j / 4];
+            m |= ((t ^ (t >> 16)) & 0xFFFFu) == 0xFFFFu ? 1u << k : 0u;
+        }
+        if (ballot(m != 0)) {
+            for (uint32_t k = 0; k < SCAN_B; k++) {
+                if (!((m >> k) & 1)) continue;
+                const uint64_t q = base + k;
+                const uint32_t hdr = (x[k / 4] >> (8 * (k & 3))) & 0xFF;
+                const uint32_t j = k + 1;
+                const uint32_t t = (j & 3) ? __builtin_amdgcn_alignbyte(x[j / 4 + 1], x[j / 4], j & 3) : x[j / 4];
+                const uint32_t len = t & 0xFFFF;
+                if (q < first || q >= last || ((hdr >> 1) & 3) != 0 || q + 5 + len > last) continue;
+                const uint32_t slot = atomicAdd(count, 1u);
+                if (slot < cap) cand[slot] = q | (uint64_t)len << 40 | (uint64_t)(hdr & 1) << 56;
             }
-            slot = wbase + mbcnt(m);
-            if (ok && slot < cap) cand[slot] = q | (uint64_t)len << 40 | (uint64_t)(hdr & 1) << 56;
         }
     }
 }
