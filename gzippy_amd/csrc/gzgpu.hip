@@ -709,17 +709,12 @@ InflateLaunch enqueue_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len
     L.grid = (uint32_t)std::min<uint64_t>(L.n, (uint64_t)c->num_cus * per_cu);
     // issue-priority balance when every piece is resident from the start
     // (progress fractions are then comparable): the launch-wide progress
-    // counter (zeroed with the queue) and the bits of all pieces
+    // counter (zeroed with the queue; each piece adds up to PROG_UNIT)
     uint64_t *prog = nullptr;
     float prog_inv = 0.f;
     if (prog_balance() && L.n <= L.grid) {
-        uint64_t bits = 0;
-        for (const PieceDesc &d : descs) {
-            const uint64_t e = std::min(d.stop_bit, d.limit_bit);
-            bits += e > d.start_bit ? e - d.start_bit : 1;
-        }
         prog = (uint64_t *)(c->d_counters.as<uint8_t>() + 8);
-        prog_inv = 1.0f / (float)std::max<uint64_t>(bits, 1);
+        prog_inv = 1.0f / ((float)L.n * PROG_UNIT);
     }
     if (debug_enabled()) fprintf(stderr, "[gzgpu] host: inflate setup %.3f ms\n", host_ms() - t0);
     HIPCHK(hipEventRecord(c->ev[5], c->stream));

@@ -1019,10 +1019,15 @@ __device__ __attribute__((noinline)) uint64_t sync_point(SH &sh, const uint8_t *
 // the resident wavefronts, start together,
 // but the SIMD's issue arbiter favours its oldest wavefront, so pieces finish
 // spread out and the launch waits for the last.  Every PROG_EVERY rounds a
-// wavefront adds the bits it consumed to a launch-wide counter (one atomic,
-// its return gives the launch's progress) and sets its priority by how far
-// its own piece's progress is behind the launch's: the lagging wavefronts
-// issue first.  A scheduling hint only; the output does not depend on it.
+// wavefront adds its piece's progress since the last report (a fraction of
+// its piece, in units of 1/PROG_UNIT) to a launch-wide counter (one atomic,
+// whose return over the piece count is the launch's mean progress) and sets
+// its priority by how far its own fraction is behind that mean: the lagging
+// wavefronts issue first.  A scheduling hint only; the output does not
+// depend on it.  (Counting consumed bits instead, the launch's progress was
+// the share of all bits consumed, which stored pieces, many bits each and
+// fast ones, dominate in mixed content: every Huffman piece then looked
+// equally behind.)
 struct Prog {
     uint32_t on;  // balance on (the state is in ProgState)
     uint32_t k;   // rounds
@@ -1033,13 +1038,15 @@ struct Prog {
 // 128 or 256 balances too late; bands of 0.5% or 5% were slower.
 constexpr uint32_t PROG_EVERY = 64;
 constexpr float PROG_BAND = 0.02f;
+constexpr float PROG_UNIT = 65536.0f;  // counter units per piece
 
 __device__ __attribute__((noinline)) void progress_report(lds_prog *ps, uint64_t P) {
-    const uint64_t d = P - uni64(ps->last);
+    const float inv_span = ps->inv_span;
+    const uint64_t d = (uint64_t)((float)(P - uni64(ps->last)) * inv_span * PROG_UNIT + 0.5f);
     uint64_t G = 0;
     if (lane_id() == 0) G = atomicAdd((unsigned long long *)ps->ctr, (unsigned long long)d) + d;
     G = uni64(G);
-    const float behind = (float)G * ps->inv_total - (float)(P - uni64(ps->start)) * ps->inv_span;
+    const float behind = (float)G * ps->inv_total - (float)(P - uni64(ps->start)) * inv_span;
     ps->last = P;
     if (behind > PROG_BAND) __builtin_amdgcn_s_setprio(3);
     else if (behind > 0.0f) __builtin_amdgcn_s_setprio(2);
