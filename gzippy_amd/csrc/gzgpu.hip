@@ -795,6 +795,14 @@ void launch_inflate(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std::vec
         cost.push_back(P[pi].plan_cost ? P[pi].plan_cost : (uint64_t)(per_bit * 256.0 * (double)bits));
         any_marker |= (P[pi].d.flags & PF_MARKER) != 0;
     }
+    // issue-priority weights for a launch with more pieces than wavefronts
+    // (progress_report); with every piece resident the launch-wide progress
+    // counter is used instead and the weights are ignored
+    if (prog_balance()) {
+        uint64_t mx = 1;
+        for (uint64_t v : cost) mx = std::max(mx, v);
+        for (size_t i = 0; i < d.size(); i++) d[i].weight = (float)((double)cost[i] / (double)mx);
+    }
     float ms = 0;
     if (debug_enabled()) fprintf(stderr, "[gzgpu] host: launch lists %.3f ms\n", host_ms() - tl);
     std::vector<PieceOut> r = any_marker ? finish_inflate(c, enqueue_inflate<uint16_t>(c, d_in, in_len, d, cost), d, ms)

@@ -54,6 +54,9 @@ struct PieceDesc {
     uint32_t member;
     uint64_t hdr_bit;        // MIDBLOCK: header of the block holding start_bit
     uint64_t stop_hdr;       // STOPMID: header of the block holding stop_bit
+    float weight;            // issue priority in a launch with more pieces than wavefronts: the
+                             // piece's planned cost over the launch's largest (0: none)
+    uint32_t pad_w;
 };
 
 struct PieceOut {
@@ -92,9 +95,9 @@ constexpr uint32_t RMASK = RING<RT> - 1;
 // is read once every PROG_EVERY rounds and would otherwise hold scalar
 // registers through the whole decode loop.
 struct ProgState {
-    uint64_t *ctr;         // launch-wide bits consumed (null: off)
+    uint64_t *ctr;         // launch-wide progress (null: by remaining work, with `weight`)
     uint64_t start, last;  // piece's first bit, position at the last report
-    float inv_total, inv_span;
+    float inv_total, inv_span, weight;
 };
 typedef __attribute__((address_space(3))) ProgState lds_prog;
 
@@ -1040,8 +1043,26 @@ constexpr uint32_t PROG_EVERY = 64;
 constexpr float PROG_BAND = 0.02f;
 constexpr float PROG_UNIT = 65536.0f;  // counter units per piece
 
+// Where pieces outnumber the resident wavefronts (a batch), they are pulled
+// over the launch and fractions are not comparable; every wavefront starts
+// at once, so the arbiter's oldest-first order says nothing about which
+// piece has the most work left either.  There each piece's priority follows
+// its remaining planned work, (1 - its fraction done) x its weight (its cost
+// over the launch's largest), in bands: the big pieces, which end the launch
+// when they run slow, issue first.
+__device__ __forceinline__ void prio_by_remaining(float rem) {
+    if (rem > 0.5f) __builtin_amdgcn_s_setprio(3);
+    else if (rem > 0.25f) __builtin_amdgcn_s_setprio(2);
+    else if (rem > 0.1f) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 __device__ __attribute__((noinline)) void progress_report(lds_prog *ps, uint64_t P) {
     const float inv_span = ps->inv_span;
+    if (!ps->ctr) {
+        prio_by_remaining((1.0f - (float)(P - uni64(ps->start)) * inv_span) * ps->weight);
+        return;
+    }
     const uint64_t d = (uint64_t)((float)(P - uni64(ps->last)) * inv_span * PROG_UNIT + 0.5f);
     uint64_t G = 0;
     if (lane_id() == 0) G = atomicAdd((unsigned long long *)ps->ctr, (unsigned long long)d) + d;
@@ -1408,7 +1429,7 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, L
     uint64_t cp_bit = NO_CP, cp_out = 0;
     uint32_t fixed_built = 0;  // the fixed tables are in the LUTs
     Prog pg;
-    pg.on = prog != nullptr;
+    pg.on = prog != nullptr || pd.weight > 0.0f;
     pg.k = 0;
     if (lane == 0) {
         const uint64_t end = pd.stop_bit < pd.limit_bit ? pd.stop_bit : pd.limit_bit;
@@ -1416,8 +1437,10 @@ __device__ __attribute__((always_inline)) void inflate_piece(InflateShared<RT, L
         sh.pg.ctr = prog;
         sh.pg.inv_total = prog_inv_total;
         sh.pg.inv_span = 1.0f / (float)span;
+        sh.pg.weight = pd.weight;
         sh.pg.start = sh.pg.last = pd.start_bit;
     }
+    if (!prog && pd.weight > 0.0f) prio_by_remaining(pd.weight);  // a new piece of a batch
 
     // ring = the last RING entries of the window before the piece (piece
     // position a at slot a mod RING)
