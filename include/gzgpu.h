@@ -160,7 +160,9 @@ int gzgpu_decompress_batch(gzgpu_ctx *ctx, const void *d_in, const uint64_t *in_
 /* Times candidate tunings (piece grid, wavefronts per chunk, waves per CU,
  * batch split rule) on the caller's own members, laid out as for
  * gzgpu_decompress_batch (n = 1 for a single member), keeps the fastest by
- * device time in the context and returns it in *best.  d_out is overwritten.
+ * device time (best of two runs after a warm-up; the planner's default unless
+ * another candidate is more than 2% faster) in the context and returns it in
+ * *best.  d_out is overwritten.
  * Stops after about budget_ms of device time.  No counterpart in the
  * reference, whose chunk size comes from a fixed formula
  * (adjusted_chunk_size_bytes, single_member.rs:128). */
