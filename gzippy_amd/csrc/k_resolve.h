@@ -16,8 +16,11 @@
 // window in LDS, and writes each piece's last 32 KiB (the end window its
 // successor needs: get_last_window, chunk_data.rs:1295); a chain starts anew
 // at a flipped piece, and chains are independent workgroups.  k_resolve_body
-// then resolves every other entry in parallel, reading each piece's window
-// from the output (final by then) or from that flipped predecessor's slab.
+// then resolves every entry in parallel, reading each piece's window from
+// the output (final by then) or from that flipped predecessor's slab; it
+// writes the end windows k_chain wrote once more, with the same bytes (an
+// entry depends only on the window before its piece), so that their CRC is
+// folded in the same pass.
 #pragma once
 #include "gz_common.h"
 #include "k_crc.h"
@@ -30,7 +33,7 @@ struct ResolvePiece {
     uint64_t out_pos;   // piece offset in the member output
     uint64_t n16, n8, u8_off;
     uint64_t tail;      // member output bytes after the piece
-    uint64_t body;      // entries [0, body) are k_resolve_body's, the rest k_chain's
+    uint64_t body;      // entries [0, body) are k_resolve_body's (all of them)
     // the byte after the 32 KiB window before the piece: in the member
     // output (win_out, positions before the member start read 0), or the
     // end of the predecessor's u8 entries in its slab when that piece
@@ -40,9 +43,8 @@ struct ResolvePiece {
 };
 
 // Body tiles: entries [RESOLVE_TILE k, RESOLVE_TILE (k+1)) of a piece's
-// first `body` entries (all of a piece that flipped, else all but the last
-// 32768); tile_first[i] is the index of piece i's first tile (a prefix sum,
-// tile_first[npieces] = all tiles).
+// first `body` entries; tile_first[i] is the index of piece i's first tile
+// (a prefix sum, tile_first[npieces] = all tiles).
 constexpr uint64_t RESOLVE_TILE = 64 * 1024;
 
 __device__ __forceinline__ uint8_t piece_entry(const ResolvePiece &r, uint64_t t, const uint8_t *win) {
@@ -198,8 +200,8 @@ __device__ __forceinline__ uint32_t resolve16(uint32_t e, gptr<const uint8_t> wi
     return (near_start && out_pos + q < 0) ? 0u : (uint32_t)win[q];
 }
 
-// Body pass: every entry of a marker piece except its last 32768 (written
-// by k_chain).  One thread per 16-byte-aligned block of the output, written
+// Body pass: every entry of a marker piece (the last 32768 of a chained
+// piece again: k_chain wrote them first, the same bytes).  One thread per 16-byte-aligned block of the output, written
 // with one 16-byte store when the block lies inside the tile:
 //   * all u8 entries: five aligned dword loads from the slab, realigned with
 //     v_alignbyte (the slab's u8 part is 256-byte aligned, the output is not);
