@@ -1189,7 +1189,7 @@ void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
         hipLaunchKernelGGL(k_bf_stage_b, dim3(nbb), dim3(256), 0, c->stream, g->in_base, g->in_total,
                            g->surv.as<uint64_t>(), cnt + GC_SURV, g->surv_cap, g->hits.as<gzgpu_boundary>(), g->hit_cap,
                            cnt + GC_HITS);
-        hipLaunchKernelGGL(k_bf_estimate, dim3(std::min<uint32_t>(g->hit_cap, (uint32_t)c->num_cus * 16)), dim3(64), 0,
+        hipLaunchKernelGGL(k_bf_estimate, dim3(std::min<uint32_t>(g->hit_cap, (uint32_t)c->num_cus * 64)), dim3(64), 0,
                            c->stream, g->in_base, g->in_total, g->hits.as<gzgpu_boundary>(), cnt + GC_HITS, g->hit_cap);
         hipLaunchKernelGGL(k_graph_hash_hits, dim3(std::min<uint32_t>((g->hit_cap + 255) / 256, (uint32_t)c->num_cus * 8)),
                            dim3(256), 0, c->stream, g->hits.as<gzgpu_boundary>(), cnt, g->hit_cap,

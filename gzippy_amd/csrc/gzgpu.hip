@@ -502,7 +502,7 @@ std::vector<gzgpu_boundary> find_hits(gzgpu_ctx *c, const uint8_t *d_in, uint64_
         HIPCHK(hipGetLastError());
         c->st.launches++;
         if (estimate) {
-            const uint32_t be = std::min<uint32_t>(hit_cap, (uint32_t)c->num_cus * 16);
+            const uint32_t be = std::min<uint32_t>(hit_cap, (uint32_t)c->num_cus * 64);  // one wavefront per hit in flight, several per CU
             hipLaunchKernelGGL(k_bf_estimate, dim3(be), dim3(64), 0, c->stream, d_in, in_len,
                                c->d_hits.as<gzgpu_boundary>(), cnt + 1, hit_cap);
             HIPCHK(hipGetLastError());
