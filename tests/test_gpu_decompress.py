@@ -201,6 +201,44 @@ def test_window_chain_prefix_equals_walk(gpu, kind, level):
             os.environ["GZGPU_CHAIN_SCAN_MIN"] = old
 
 
+def test_issue_priority_balance_off_same_output(gpu):
+    """GZGPU_PROG=0 turns off the inflate launch's issue-priority balance
+    (the launch-wide progress counter of a single member, the remaining-work
+    weights of a batch whose pieces outnumber the resident wavefronts): the
+    bytes are the same.  The switch is read once per process, so a child
+    process decodes with it off."""
+    import os
+    import subprocess
+    import sys
+    code = r"""
+import numpy as np, torch
+import gzippy_amd
+from gzippy_amd import workloads as W
+raw = W.generate("text", 8 << 20, 23)
+gz = W.gzip_compress(raw, 6)
+assert gzippy_amd.decompress(gz) == raw.tobytes()
+mem = W.batch_members(5000, seed=29, median=8 << 10, max_size=64 << 10)
+raws = [m[0] for m in mem]
+ins = [m[1] for m in mem]
+in_len = np.array([x.size for x in ins], dtype=np.uint64)
+in_off = np.concatenate([[0], np.cumsum(in_len)[:-1]]).astype(np.uint64)
+out_cap = np.array([r.size for r in raws], dtype=np.uint64)
+out_off = np.concatenate([[0], np.cumsum(out_cap)[:-1]]).astype(np.uint64)
+d_in = torch.from_numpy(np.concatenate(ins)).cuda()
+D = int(out_cap.sum())
+d_out = torch.empty(D + 1, dtype=torch.uint8, device="cuda")
+status, olen, crc, st = gzippy_amd.decompress_batch(d_in, in_off, in_len, d_out, out_off, out_cap)
+assert (status == 0).all()
+assert torch.equal(d_out[:D].cpu(), torch.from_numpy(np.concatenate(raws)))
+print("ok", st["pieces"])
+"""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, GZGPU_PROG="0"), cwd=root,
+                       capture_output=True, text=True, timeout=300)
+    say("balance off:", r.stdout.strip()[-200:], r.stderr.strip()[-300:] if r.returncode else "")
+    assert r.returncode == 0 and "ok" in r.stdout
+
+
 def test_writer_api(gpu):
     raw, gz = W.text_member(2 << 20, seed=10)
     sink = io.BytesIO()
