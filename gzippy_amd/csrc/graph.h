@@ -1230,16 +1230,8 @@ void graph_enqueue(gzgpu_batch_graph *g, bool plain) {
         // the bodies
         const dim3 sg(std::min<uint32_t>(g->nslots, (uint32_t)c->num_cus * 8));
         uint16_t *wb[2] = {g->scan.as<uint16_t>(), g->scan.as<uint16_t>() + (size_t)g->nslots * WIN};
-        hipLaunchKernelGGL(k_chain_scan_init, sg, dim3(256), 0, c->stream, (const ResolvePiece *)g->rps.p,
-                           g->slot_rp.as<uint32_t>(), g->slot_pos.as<uint32_t>(), g->nslots, wb[0]);
-        int cur = 0;
-        for (uint32_t step = 1; step < g->max_chain; step *= CHAIN_RADIX) {
-            hipLaunchKernelGGL(k_chain_scan_step, sg, dim3(256), 0, c->stream, g->slot_pos.as<uint32_t>(), g->nslots, step,
-                               (const uint16_t *)wb[cur], wb[cur ^ 1]);
-            cur ^= 1;
-        }
-        hipLaunchKernelGGL(k_chain_scan_out, sg, dim3(256), 0, c->stream, (const ResolvePiece *)g->rps.p,
-                           g->slot_rp.as<uint32_t>(), g->nslots, (const uint16_t *)wb[cur]);
+        launch_chain_scan(c->stream, sg, (const ResolvePiece *)g->rps.p, g->slot_rp.as<uint32_t>(),
+                          g->slot_pos.as<uint32_t>(), g->nslots, g->max_chain, wb);
         hipLaunchKernelGGL(k_resolve_body, dim3(std::min<uint32_t>(std::max<uint32_t>(g->tile_cap, 1), (uint32_t)c->num_cus * 32)),
                            dim3(256), 0, c->stream, (const ResolvePiece *)g->rps.p, g->tile_first.as<uint32_t>(),
                            g->nslots, c->d_crc_tab.as<uint32_t>(), g->tile_crc.as<uint32_t>(), 0u, 0xFFFFFFFFu);

@@ -1918,22 +1918,9 @@ void run_members_aligned(gzgpu_ctx *c, const uint8_t *d_in, uint64_t in_len, std
             c->d_scan.ensure((size_t)ns * WIN * 2 * 2);
             uint16_t *wb[2] = {c->d_scan.as<uint16_t>(), c->d_scan.as<uint16_t>() + (size_t)ns * WIN};
             const dim3 g(std::min<uint32_t>(ns, (uint32_t)c->num_cus * 8));
-            hipLaunchKernelGGL(k_chain_scan_init, g, dim3(256), 0, c->stream, (const ResolvePiece *)part(0),
-                               (const uint32_t *)part(9), (const uint32_t *)part(10), ns, wb[0]);
+            c->st.launches += launch_chain_scan(c->stream, g, (const ResolvePiece *)part(0), (const uint32_t *)part(9),
+                                                (const uint32_t *)part(10), ns, max_chain, wb);
             HIPCHK(hipGetLastError());
-            c->st.launches++;
-            int cur = 0;
-            for (uint32_t step = 1; step < max_chain; step *= CHAIN_RADIX) {
-                hipLaunchKernelGGL(k_chain_scan_step, g, dim3(256), 0, c->stream, (const uint32_t *)part(10), ns, step,
-                                   (const uint16_t *)wb[cur], wb[cur ^ 1]);
-                HIPCHK(hipGetLastError());
-                c->st.launches++;
-                cur ^= 1;
-            }
-            hipLaunchKernelGGL(k_chain_scan_out, g, dim3(256), 0, c->stream, (const ResolvePiece *)part(0),
-                               (const uint32_t *)part(9), ns, (const uint16_t *)wb[cur]);
-            HIPCHK(hipGetLastError());
-            c->st.launches++;
         } else if (nch) {
             hipLaunchKernelGGL(k_chain, dim3(std::min<uint32_t>(nch, (uint32_t)c->num_cus * 4)), dim3(1024), 0,
                                c->stream, (const ResolvePiece *)part(0), (const uint32_t *)part(1), nch);

@@ -178,13 +178,16 @@ def test_device_api(gpu):
     assert np.array_equal(out.cpu().numpy(), raw)
 
 
-@pytest.mark.parametrize("kind,level", [("text", 6), ("u32", 1), ("runs", 9)])
-def test_window_chain_prefix_equals_walk(gpu, kind, level):
+@pytest.mark.parametrize("kind,level,kib", [("text", 6, 24576), ("u32", 1, 24576), ("runs", 9, 24576),
+                                            ("text", 6, 1024), ("u32", 1, 3072), ("text", 6, 160), ("u32", 1, 96)])
+def test_window_chain_prefix_equals_walk(gpu, kind, level, kib):
     """Pieces too short to flip form one long window chain: resolved as a
     parallel prefix (every chain) and piece by piece (no chain), the output
-    is the same, byte-exact."""
+    is the same, byte-exact.  The small members give chains of a few pieces,
+    whose prefix is a single step (slab reads and output writes in one
+    launch)."""
     import os
-    raw = W.generate(kind, 24 << 20, seed=17 + level)
+    raw = W.generate(kind, kib << 10, seed=17 + level)
     gz = W.gzip_compress(raw, level)
     old = os.environ.get("GZGPU_CHAIN_SCAN_MIN")
     try:
