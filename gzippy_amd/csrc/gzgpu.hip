@@ -126,7 +126,6 @@ struct CrcConst {
     uint32_t tab[8][256];
     uint32_t mx[4][256];  // follows tab: c -> c * x^(8 * 4096) mod P, one table per byte of c
     uint32_t xbyte[16][16];
-    uint32_t kseg[256];  // x^(8 * 256 * (255 - l)) mod P
     CrcConst() {
         for (uint32_t i = 0; i < 256; i++) {
             uint32_t c = i;
@@ -147,13 +146,6 @@ struct CrcConst {
                     if (j & (1 << b)) p = mulmod(x2n[4 * k + 3 + b], p);
                 xbyte[k][j] = p;
             }
-        for (uint32_t l = 0; l < 256; l++) {
-            uint64_t nbytes = 256ull * (255 - l);
-            uint32_t p = 1u << 31;
-            for (int k = 0; nbytes; k++, nbytes >>= 4)
-                if (nbytes & 15) p = mulmod(xbyte[k][nbytes & 15], p);
-            kseg[l] = p;
-        }
         {
             uint64_t nbytes = 4096;
             uint32_t X = 1u << 31;
@@ -2067,7 +2059,6 @@ int guarded(gzgpu_ctx *c, F &&f, bool reset_staging = true) {
 void init_constants(gzgpu_ctx *c) {
     const CrcConst &cc = crc_const();
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(gz::c_xbyte), cc.xbyte, sizeof(cc.xbyte)));
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(gz::c_kseg), cc.kseg, sizeof(cc.kseg)));
     (void)c;
 }
 

@@ -1,13 +1,11 @@
 // k_crc.h — gzip CRC32 (reflected 0xEDB88320) on the device.
 //
 // Stage 1 (k_crc_segments): every workgroup folds one segment of <= 64 KiB;
-// each of its 256 lanes runs a slicing-by-8 table CRC over 256 contiguous
-// bytes from a zero register.  The lane CRCs combine with the GF(2) shift
+// each of its 256 threads folds the segment's 16-byte blocks 4096 bytes
+// apart (two slicing-by-8 steps per block, from a zero register), and the
+// partial CRCs combine with the GF(2) shift
 // crc(A||B) = crc(A) * x^(8|B|) mod P ^ crc(B) (combine_crc32,
-// crc32.rs:127-133; polynomial_multiply_modulo :64; x_power_modulo :104):
-// lane l's CRC is multiplied by the constant x^(8 * 256 * (255 - l)) and the
-// 256 products are XORed.  A segment shorter than 64 KiB is placed at the end
-// of the 64 KiB frame behind zeros, which a raw CRC register ignores.
+// crc32.rs:127-133; polynomial_multiply_modulo :64; x_power_modulo :104).
 // Each segment's CRC leaves stage 1 already shifted over the member bytes
 // after it (seg_tail), so stage 2 (k_crc_members) only XORs a member's
 // segment values and applies the init/final inversion:
@@ -42,9 +40,6 @@ __device__ __forceinline__ uint32_t xpow_bytes(uint64_t n) {
     return p;
 }
 
-// c_kseg[l] = x^(8 * 256 * (255 - l)) mod P: lane l's shift to the end of a
-// full 64 KiB segment, filled by the host
-__constant__ uint32_t c_kseg[256];
 
 // shift a raw CRC register over `len` zero bytes
 __device__ __forceinline__ uint32_t crc_shift(uint32_t c, uint64_t len) { return len ? gf2_mulmod(xpow_bytes(len), c) : c; }
@@ -74,116 +69,61 @@ __device__ __forceinline__ uint32_t mul_x4k(const uint32_t (*mx)[256], uint32_t 
     return mx[0][c & 0xFF] ^ mx[1][(c >> 8) & 0xFF] ^ mx[2][(c >> 16) & 0xFF] ^ mx[3][c >> 24];
 }
 
-// Segments: seg_off[i] bytes into `base`, seg_len[i] <= CRC_SEG.  Every
-// segment is folded as the last seg_len bytes of a 64 KiB frame whose first
-// bytes are zero: leading zeros leave a raw CRC register at 0, so the frame's
-// CRC is the segment's.  The frame is staged in LDS (coalesced loads), then
-// lane l folds frame bytes [256 l, 256 l + 256) with slicing-by-8 and the
-// lanes combine with the constant shifts c_kseg.  A full, 16-byte-aligned
-// segment is staged with 16-byte loads; any other with dword loads realigned
-// by v_alignbit (only dwords that hold segment bytes are read).
+// Segments: seg_off[i] bytes into `base`, seg_len[i] <= CRC_SEG bytes each,
+// one workgroup per segment, folded the way k_resolve_body folds its tiles:
+// the 16-byte-aligned blocks [A0, A1) of the address space that hold segment
+// bytes are read with coalesced 16-byte loads (bytes outside the segment
+// masked to zero), thread t takes blocks t, t + 256, ... (4096 bytes apart)
+// and folds them in order with acc = acc * x^(8 * 4096) ^ crc(block), then
+// shifts once to A1; the threads' values XOR together.  Zero bytes before the
+// segment leave a raw CRC register at 0; the zeros after it (A1 - end) are
+// divided out, and the value is shifted over the member bytes after the
+// segment (seg_tail).  Only the CRC tables sit in LDS (12 KiB), so many
+// workgroups share a CU and keep their loads in flight.
 __global__ void __launch_bounds__(256) k_crc_segments(const uint8_t *__restrict__ base, const uint64_t *__restrict__ seg_off,
                                                       const uint32_t *__restrict__ seg_len,
                                                       const uint64_t *__restrict__ seg_tail, uint32_t nseg,
                                                       const uint32_t *__restrict__ tables, uint32_t *__restrict__ seg_crc) {
-    __shared__ uint32_t t[8][256];
-    __shared__ uint32_t buf[256 * 65];  // frame row r (bytes [256 r, 256 r + 256)) at dword 65 r (conflict-free reads)
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    __shared__ uint32_t t8[8][256];
+    __shared__ uint32_t mx[4][256];
     __shared__ uint32_t red;
-    for (uint32_t i = threadIdx.x; i < 8 * 256; i += blockDim.x) t[i >> 8][i & 255] = tables[i];
+    for (uint32_t i = threadIdx.x; i < 8 * 256; i += blockDim.x) t8[i >> 8][i & 255] = tables[i];
+    for (uint32_t i = threadIdx.x; i < 4 * 256; i += blockDim.x) mx[i >> 8][i & 255] = tables[8 * 256 + i];
     __syncthreads();
     for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
-        const uint8_t *p = base + seg_off[s];
-        const uint32_t len = seg_len[s];
-        uint32_t f0 = CRC_SEG - len;  // frame byte of the segment's first byte
-        // A segment whose 16-byte chunks fit the frame (lead bytes before it
-        // in its first chunk, pad zero bytes after it to the next 16-byte
-        // boundary) is staged with 16-byte loads, masked at both ends; the
-        // frame then holds the segment followed by `pad` zeros, whose shift
-        // is undone at the end.
-        const uint32_t lead = (uint32_t)((uintptr_t)p & 15), pad = (uint32_t)(-(uintptr_t)(p + len) & 15);
-        const bool chunked = len && lead + len + pad <= CRC_SEG && !(len == CRC_SEG && lead == 0);
-        if (chunked) f0 = CRC_SEG - len - pad;
-        if (chunked) {
-            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-            const uint32_t fc = f0 - lead;  // frame byte of the first chunk (16-byte aligned)
-            const uint32_t nch = (lead + len + pad) >> 4;
-            gptr<const u32x4> src = (gptr<const u32x4>)to_global(p - lead);
-            for (uint32_t k = threadIdx.x; k < nch; k += blockDim.x) {
-                u32x4 v = src[k];
-                if (k == 0 || k == nch - 1) {
-                    // zero the bytes outside [lead, lead + len) of the chunk run
-                    uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                    for (int j = 0; j < 4; j++)
-#pragma unroll
-                        for (int b = 0; b < 4; b++) {
-                            const uint32_t q = 16 * k + 4 * j + b;  // byte of the chunk run
-                            if (q < lead || q >= lead + len) w[j] &= ~(0xFFu << (8 * b));
-                        }
-                    v.x = w[0];
-                    v.y = w[1];
-                    v.z = w[2];
-                    v.w = w[3];
-                }
-                const uint32_t o = fc + 16 * k;
-                uint32_t *d = &buf[65 * (o >> 8) + ((o & 255) >> 2)];
-                d[0] = v.x;
-                d[1] = v.y;
-                d[2] = v.z;
-                d[3] = v.w;
-            }
-            // the first data row's bytes before the chunk run read as zero
-            const uint32_t r0 = fc >> 8;
-            for (uint32_t D = r0 * 64 + threadIdx.x; D < (fc >> 2); D += blockDim.x) buf[65 * (D >> 6) + (D & 63)] = 0;
-        } else if (len == CRC_SEG && (((uintptr_t)p) & 15) == 0) {
-            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-            gptr<const u32x4> src = (gptr<const u32x4>)to_global(p);
-#pragma unroll 4
-            for (uint32_t k = 0; k < 16; k++) {
-                const uint32_t o = k * 4096 + threadIdx.x * 16;  // byte offset in the segment
-                const u32x4 v = src[o >> 4];
-                uint32_t *d = &buf[65 * (o >> 8) + ((o & 255) >> 2)];
-                d[0] = v.x;
-                d[1] = v.y;
-                d[2] = v.z;
-                d[3] = v.w;
-            }
-        } else if (len) {
-            // frame dword D = bytes [4D, 4D + 4) = segment bytes [4D - f0, ..):
-            // aligned source dwords a and a + 1, shifted by the misalignment
-            const uintptr_t g0 = (uintptr_t)p - f0;  // address of frame byte 0 (never dereferenced)
-            const uint32_t sh = (uint32_t)(g0 & 3) * 8;
-            const uintptr_t lo_dw = (uintptr_t)p >> 2, hi_dw = ((uintptr_t)p + len - 1) >> 2;  // dwords holding segment bytes
-            for (uint32_t D = (f0 >> 2) + threadIdx.x; D < CRC_SEG / 4; D += blockDim.x) {
-                const uintptr_t a = (g0 >> 2) + D;
-                const uint32_t w0 = a >= lo_dw && a <= hi_dw ? *(gptr<const uint32_t>)to_global((const uint32_t *)(a << 2)) : 0u;
-                const uint32_t w1 = sh && a + 1 >= lo_dw && a + 1 <= hi_dw
-                                        ? *(gptr<const uint32_t>)to_global((const uint32_t *)((a + 1) << 2))
-                                        : 0u;
-                uint32_t v = sh ? __builtin_amdgcn_alignbit(w1, w0, sh) : w0;
-                if (4 * D < f0) v &= 0xFFFFFFFFu << (8 * (f0 - 4 * D));  // bytes before the segment
-                buf[65 * (D >> 6) + (D & 63)] = v;
-            }
-            // the first data row's bytes before the segment read as zero
-            const uint32_t r0 = f0 >> 8;
-            for (uint32_t D = r0 * 64 + threadIdx.x; D < (f0 >> 2); D += blockDim.x) buf[65 * (D >> 6) + (D & 63)] = 0;
-        }
         if (threadIdx.x == 0) red = 0;
         __syncthreads();
-        uint32_t c = 0;
-        if (len && threadIdx.x >= (f0 >> 8)) {  // rows before the segment are all zero: CRC 0
-            const uint32_t *my = &buf[65 * threadIdx.x];
+        const uintptr_t p = (uintptr_t)(base + seg_off[s]), e = p + seg_len[s];
+        const uintptr_t A0 = p & ~(uintptr_t)15, A1 = (e + 15) & ~(uintptr_t)15;
+        uint32_t acc = 0;
+        uintptr_t my_end = 0;  // end of this thread's last block
 #pragma unroll 4
-            for (uint32_t j = 0; j < 32; j++) {
-                const uint32_t lo = c ^ my[2 * j], hi = my[2 * j + 1];
-                c = t[7][lo & 0xFF] ^ t[6][(lo >> 8) & 0xFF] ^ t[5][(lo >> 16) & 0xFF] ^ t[4][lo >> 24] ^
-                    t[3][hi & 0xFF] ^ t[2][(hi >> 8) & 0xFF] ^ t[1][(hi >> 16) & 0xFF] ^ t[0][hi >> 24];
+        for (uintptr_t blk = A0 + 16 * (uintptr_t)threadIdx.x; blk < A1; blk += 16 * (uintptr_t)blockDim.x) {
+            u32x4 v = *(gptr<const u32x4>)to_global((const u32x4 *)blk);
+            if (blk < p || blk + 16 > e) {
+                // keep bytes [lo, hi) of the block
+                const uint32_t lo = blk < p ? (uint32_t)(p - blk) : 0u, hi = blk + 16 > e ? (uint32_t)(e - blk) : 16u;
+                uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    uint32_t m = 0xFFFFFFFFu;
+                    if (lo > 4u * j) m = lo >= 4u * j + 4 ? 0u : m << (8 * (lo - 4 * j));
+                    if (hi < 4u * j + 4) m &= hi <= 4u * j ? 0u : 0xFFFFFFFFu >> (8 * (4 * j + 4 - hi));
+                    w[j] &= m;
+                }
+                v.x = w[0];
+                v.y = w[1];
+                v.z = w[2];
+                v.w = w[3];
             }
-            c = gf2_mulmod(c_kseg[threadIdx.x], c);
+            acc = mul_x4k(mx, acc) ^ crc16b(t8, v.x, v.y, v.z, v.w);
+            my_end = blk + 16;
         }
-        if (c) atomicXor(&red, c);
+        if (my_end) acc = crc_shift(acc, (uint64_t)(A1 - my_end));
+        if (acc) atomicXor(&red, acc);
         __syncthreads();
-        if (threadIdx.x == 0) seg_crc[s] = crc_shift(chunked ? crc_unshift(red, pad) : red, seg_tail[s]);
+        if (threadIdx.x == 0) seg_crc[s] = crc_shift(crc_unshift(red, (uint32_t)(A1 - e)), seg_tail[s]);
         __syncthreads();
     }
 }
