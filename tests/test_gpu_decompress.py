@@ -644,3 +644,18 @@ def test_streamed_output_and_late_crc_error(gpu):
     got = b"".join(rec.parts)
     assert raw.tobytes().startswith(got)
     say("streamed before the CRC error", len(got), "of", raw.size)
+
+
+def test_crc32_entry_point_offsets_and_lengths(gpu):
+    """gzgpu_crc32 (crc32_fold, crc32.rs:155) over a device buffer at every
+    16-byte phase of its start, with lengths across the 16-byte block, the
+    4096-byte fold stride and the 64 KiB segment boundaries, equals zlib's
+    CRC32: the masked edge blocks and the divided-out pad bytes."""
+    import torch
+    rng = np.random.default_rng(5)
+    host = rng.integers(0, 256, 300_000, dtype=np.uint8)
+    d = torch.from_numpy(host).cuda()
+    for off in (0, 1, 3, 8, 15, 16, 17, 4095):
+        for n in (0, 1, 15, 16, 17, 255, 4096, 4097, 65535, 65536, 65537, 131077, 200_000):
+            got = gpu.crc32(d[off:], length=n)
+            assert got == zlib.crc32(host[off:off + n].tobytes()), (off, n)
